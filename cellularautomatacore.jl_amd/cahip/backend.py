@@ -1,0 +1,234 @@
+"""HIPBackend: the MI355X backend behind the reference's backend interface.
+
+Each method is the counterpart of one generic function of the backend contract
+(src/backends.jl:1-14, :89-148) and calls the C ABI of libcahip.so
+(include/cahip.h).  Julia's bang functions keep their names without the bang:
+advance! -> advance, set_state! -> set_state, write_cell! -> write_cell.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from . import model as M
+
+_ERRORS = {
+    L.CAH_ERR_ARGUMENT: M.ArgumentError,
+    L.CAH_ERR_INCOMPATIBLE_SPACE: M.IncompatibleSpaceError,
+    L.CAH_ERR_UNSUPPORTED: M.UnsupportedCombinationError,
+    L.CAH_ERR_OUT_OF_MEMORY: M.OutOfMemoryError,
+    L.CAH_ERR_STATE_SHAPE: M.StateShapeError,
+    L.CAH_ERR_INVALID_STATE: M.InvalidStateError,
+    L.CAH_ERR_DEVICE: M.DeviceError,
+}
+
+
+def check(rc):
+    if rc != L.CAH_OK:
+        raise _ERRORS.get(rc, M.DeviceError)(L.last_error())
+    return rc
+
+
+FIELD_METHODS = {"seq": L.FIELD_DIRECT_SEQ, "f32": L.FIELD_DIRECT_F32,
+                 "mfma_f16": L.FIELD_MFMA_F16, "mfma_bf16": L.FIELD_MFMA_BF16,
+                 "mfma_f16x2": L.FIELD_MFMA_F16X2, "fft_f32": L.FIELD_FFT_F32}
+
+
+def _p_value(p, name):
+    if p is None:
+        raise M.ArgumentError(f"the forest fire reads p.{name}, and the problem has no parameters")
+    return p[name] if isinstance(p, dict) else getattr(p, name)
+
+
+def make_params(prob, p):
+    """cah_params from the integrator's p (only the forest fire reads any)."""
+    cp = L.CahParams()
+    if isinstance(prob.rule, M.ForestFire):
+        pv, fv = _p_value(p, "p"), _p_value(p, "f")
+        cp.ff_p, cp.ff_f = float(pv), float(fv)
+        cp.ff_p_is_f32 = int(isinstance(pv, np.float32))
+        cp.ff_f_is_f32 = int(isinstance(fv, np.float32))
+    return cp
+
+
+def make_desc(prob, scheme, field_method=L.FIELD_DIRECT_F32):
+    """A cah_desc for (prob, scheme) and the host arrays it points into."""
+    d = L.CahDesc()
+    keep = []
+    sp = prob.space
+    d.ndims = sp.ndims
+    rule = prob.rule
+    d.dtype = L.DTYPE_F32 if isinstance(rule, M.Lenia) else L.DTYPE_I8
+    d.dims[0] = sp.dims[0]
+    d.dims[1] = sp.dims[1] if sp.ndims == 2 else 1
+    for a, b in enumerate(sp.boundaries):
+        d.boundary[a] = b.code
+        d.boundary_value[a] = float(b.value) if isinstance(b, M.Constant) else 0.0
+    if sp.ndims == 1:
+        d.boundary[1] = L.BOUNDARY_PERIODIC
+    d.scheme = getattr(scheme, "code", -1)
+    if isinstance(scheme, M.Stochastic):
+        d.scheme_p = float(scheme.p)
+        d.scheme_p_is_f32 = int(scheme.is_f32)
+    d.seed = prob.seed
+    d.rule = rule.code
+    if isinstance(rule, (M.Life, M.Generations)):
+        d.birth, d.survive = rule.birth, rule.survive
+    if isinstance(rule, M.Generations):
+        d.gen_states = rule.states
+    offs = np.array([(o + (0,))[:2] for o in prob.neighbours.offsets], dtype=np.int32).ravel()
+    keep.append(offs)
+    d.n_offsets = len(prob.neighbours.offsets)
+    d.offsets = offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    if isinstance(rule, M.Lenia):
+        taps, w = prob.taps
+        to = np.array(taps, dtype=np.int32).ravel()
+        tw = np.ascontiguousarray(w, dtype=np.float64)
+        keep += [to, tw]
+        d.n_taps = len(taps)
+        d.tap_offsets = to.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        d.tap_weights = tw.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        d.mu, d.sigma, d.dt = rule.mu, rule.sigma, rule.dt
+        d.lenia_precision = rule.precision
+        d.field_method = L.FIELD_FFT_F32 if rule.method == "fft" else field_method
+    if prob.mask is not None:
+        m = np.asfortranarray(prob.mask).ravel(order="F").astype(np.uint8)
+        keep.append(m)
+        d.mask = m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    return d, keep
+
+
+class HIPState:
+    """The backend state: a handle to device buffers owned by libcahip."""
+
+    def __init__(self, handle, dims, dtype):
+        self.handle, self.dims, self.dtype = handle, dims, dtype
+
+    @property
+    def nbytes(self):
+        return int(np.prod(self.dims)) * np.dtype(self.dtype).itemsize
+
+
+class HIPBackend:
+    """HIPBackend(device=0; lenia_field="f32").
+
+    `lenia_field` chooses how the Lenia potential is computed on the device:
+    "seq" sums the taps in order in the rule's precision (as _field_at does),
+    "f32" uses FP32 FMAs over a dense weight table.
+    """
+
+    def __init__(self, device=0, lenia_field="f32", tune=None):
+        self.lib = L.load()
+        self.device = device
+        if lenia_field not in FIELD_METHODS:
+            raise M.ArgumentError(f"unknown lenia_field {lenia_field!r}")
+        self.lenia_field = FIELD_METHODS[lenia_field]
+        self.tune = tune
+        self._ctx = None
+
+    def ctx(self):
+        if self._ctx is None:
+            h = ctypes.c_void_p()
+            check(self.lib.cah_open(self.device, ctypes.byref(h)))
+            self._ctx = h
+        return self._ctx
+
+    # supports(backend, scheme) (src/backends.jl:95, :153)
+    def supports(self, scheme):
+        return bool(self.lib.cah_supports(getattr(scheme, "code", -1)))
+
+    # check_problem(backend, prob, scheme) (src/backends.jl:105)
+    def check_problem(self, prob, scheme):
+        d, _keep = make_desc(prob, scheme, self.lenia_field)
+        check(self.lib.cah_check_problem(ctypes.byref(d)))
+
+    # allocate_state(backend, prob, scheme) (src/backends.jl:119)
+    def allocate_state(self, prob, scheme):
+        d, _keep = make_desc(prob, scheme, self.lenia_field)
+        h = ctypes.c_void_p()
+        check(self.lib.cah_state_create(self.ctx(), ctypes.byref(d), ctypes.byref(h)))
+        st = HIPState(h, prob.space.dims, prob.u0.dtype)
+        if self.tune:
+            check(self.lib.cah_state_tune(h, *self.tune))
+        self.set_state(st, prob.u0)
+        return st
+
+    # advance!(backend, st, prob, scheme, p, t) (src/backends.jl:130), nsteps times
+    def advance(self, st, prob, scheme, p, t, nsteps=1):
+        cp = make_params(prob, p)
+        check(self.lib.cah_advance(st.handle, int(t), int(nsteps), ctypes.byref(cp)))
+
+    # current_state(backend, st) (src/backends.jl:139): a copy the caller may keep
+    def current_state(self, st):
+        out = np.empty(st.dims, dtype=st.dtype, order="F")
+        check(self.lib.cah_state_get(st.handle, out.ctypes.data, st.nbytes))
+        return out
+
+    # set_state!(backend, st, u) (src/backends.jl:148)
+    def set_state(self, st, u):
+        u = np.asfortranarray(np.asarray(u, dtype=st.dtype))
+        if tuple(u.shape) != tuple(st.dims):
+            raise M.StateShapeError(f"expected size {st.dims}, got {u.shape}")
+        check(self.lib.cah_state_set(st.handle, u.ctypes.data, st.nbytes))
+
+    # read_cell / write_cell! (src/agents.jl:165, :174), 0-based index tuple
+    def read_cell(self, st, I):
+        v = np.zeros(1, dtype=st.dtype)
+        i2 = I[1] if len(I) > 1 else 0
+        check(self.lib.cah_state_read_cell(st.handle, int(I[0]), int(i2), v.ctypes.data))
+        return v[0]
+
+    def write_cell(self, st, I, value):
+        v = np.array([value], dtype=st.dtype)
+        i2 = I[1] if len(I) > 1 else 0
+        check(self.lib.cah_state_write_cell(st.handle, int(I[0]), int(i2), v.ctypes.data))
+
+    def sync(self, st):
+        check(self.lib.cah_state_sync(st.handle))
+
+    def stream(self, st):
+        s = ctypes.c_void_p()
+        check(self.lib.cah_state_stream(st.handle, ctypes.byref(s)))
+        return s.value
+
+    def free_state(self, st):
+        if st.handle:
+            self.lib.cah_state_destroy(st.handle)
+            st.handle = None
+
+    # -------------------------------------------------------------- batches
+
+    def solve_batch(self, probs, scheme, use_graph=True):
+        """Final states of several problems of one Int8 rule, advanced together."""
+        descs, keeps = [], []
+        t0, t1 = probs[0].tspan
+        for pr in probs:
+            if pr.tspan != probs[0].tspan:
+                raise M.ArgumentError("a batch advances problems over one tspan")
+            d, k = make_desc(pr, scheme, self.lenia_field)
+            descs.append(d)
+            keeps.append(k)
+        arr = (L.CahDesc * len(descs))(*descs)
+        h = ctypes.c_void_p()
+        check(self.lib.cah_batch_create(self.ctx(), len(descs), arr, ctypes.byref(h)))
+        try:
+            for i, pr in enumerate(probs):
+                u = np.asfortranarray(pr.u0)
+                check(self.lib.cah_batch_set(h, i, u.ctypes.data, u.nbytes))
+            cp = make_params(probs[0], probs[0].p)
+            check(self.lib.cah_batch_advance(h, t0, t1 - t0, ctypes.byref(cp), int(use_graph)))
+            outs = []
+            for i, pr in enumerate(probs):
+                out = np.empty(pr.space.dims, dtype=pr.u0.dtype, order="F")
+                check(self.lib.cah_batch_get(h, i, out.ctypes.data, out.nbytes))
+                outs.append(out)
+            return outs
+        finally:
+            self.lib.cah_batch_destroy(h)
+
+    def __del__(self):
+        if getattr(self, "_ctx", None) is not None:
+            try:
+                self.lib.cah_close(self._ctx)
+            except Exception:
+                pass

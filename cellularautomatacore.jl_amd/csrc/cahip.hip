@@ -1,0 +1,770 @@
+// cahip.hip -- the C ABI (include/cahip.h) over HIP for gfx950.
+//
+// One state = two dense device buffers (current / next) with a 64-byte-aligned
+// row pitch, one HIP stream, and the resolved rule, neighbourhood, kernel-field
+// taps and mask in device memory.  advance queues one sweep launch per gate and
+// swaps the buffers; boundary conditions are applied inside the sweep kernels
+// while tiles or rows are loaded, so there is no halo buffer and no halo launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/cahip.h"
+#include "device.hpp"
+#include "kernels_i8.hpp"
+#include "kernels_lenia.hpp"
+
+// ---------------------------------------------------------------- errors
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? CAH_ERR_OUT_OF_MEMORY : CAH_ERR_DEVICE, \
+                        "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+extern "C" const char* cah_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------- context
+
+struct cah_ctx {
+    int device;
+};
+
+extern "C" int cah_open(int device, cah_ctx** out) {
+    if (!out) return fail(CAH_ERR_ARGUMENT, "cah_open: out is NULL");
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+        return fail(CAH_ERR_ARGUMENT, "device %d does not exist (%d visible)", device, n);
+    HIPCHK(hipSetDevice(device));
+    *out = new cah_ctx{device};
+    return CAH_OK;
+}
+
+extern "C" void cah_close(cah_ctx* ctx) { delete ctx; }
+
+extern "C" int cah_supports(int scheme) {
+    return scheme == CAH_SCHEME_SYNCHRONOUS || scheme == CAH_SCHEME_STOCHASTIC ||
+           scheme == CAH_SCHEME_CHECKERBOARD;
+}
+
+// ---------------------------------------------------------------- problem checks
+
+static int32_t max_abs(const int32_t* o, int32_t n) {
+    int32_t h = 0;
+    for (int32_t k = 0; k < 2 * n; ++k) h = std::max(h, std::abs(o[k]));
+    return h;
+}
+
+constexpr int32_t GEN_MAX_HALO = 32;
+constexpr int32_t LENIA_MAX_HALO = 16;
+
+extern "C" int cah_check_problem(const cah_desc* d) {
+    if (!d) return fail(CAH_ERR_ARGUMENT, "descriptor is NULL");
+    if (d->ndims != 1 && d->ndims != 2)
+        return fail(CAH_ERR_INCOMPATIBLE_SPACE,
+                    "the HIP backend runs one- and two-dimensional square lattices, got %d dimensions",
+                    d->ndims);
+    const int64_t n2 = d->ndims == 2 ? d->dims[1] : 1;
+    if (d->dims[0] < 1 || n2 < 1)
+        return fail(CAH_ERR_ARGUMENT, "lattice dimensions must be positive");
+    if (d->dims[0] >= (1LL << 30) || n2 > (1LL << 20))
+        return fail(CAH_ERR_ARGUMENT, "lattice sides must be below 2^30 (first axis) and 2^20 (second)");
+    for (int a = 0; a < d->ndims; ++a)
+        if (d->boundary[a] < CAH_BOUNDARY_PERIODIC || d->boundary[a] > CAH_BOUNDARY_CONSTANT)
+            return fail(CAH_ERR_ARGUMENT, "unknown boundary on axis %d", a + 1);
+    if (!cah_supports(d->scheme))
+        return fail(CAH_ERR_UNSUPPORTED, "the HIP backend runs Synchronous, Stochastic and Checkerboard");
+    if (d->scheme == CAH_SCHEME_STOCHASTIC && !(d->scheme_p > 0.0 && d->scheme_p <= 1.0))
+        return fail(CAH_ERR_ARGUMENT, "update probability must be in (0, 1], got %g", d->scheme_p);
+    switch (d->rule) {
+    case CAH_RULE_LIFE:
+    case CAH_RULE_GENERATIONS:
+    case CAH_RULE_FOREST_FIRE:
+        if (d->dtype != CAH_DTYPE_I8) return fail(CAH_ERR_ARGUMENT, "this rule runs on Int8 states");
+        if (d->n_offsets < 1 || !d->offsets)
+            return fail(CAH_ERR_ARGUMENT, "a neighbourhood needs at least one offset");
+        if (max_abs(d->offsets, d->n_offsets) > GEN_MAX_HALO)
+            return fail(CAH_ERR_ARGUMENT, "neighbourhoods wider than %d cells are not supported",
+                        GEN_MAX_HALO);
+        if (d->rule == CAH_RULE_GENERATIONS && d->gen_states < 2)
+            return fail(CAH_ERR_ARGUMENT, "a Generations rule needs at least 2 states");
+        for (int a = 0; a < d->ndims; ++a)
+            if (d->boundary[a] == CAH_BOUNDARY_CONSTANT &&
+                (d->boundary_value[a] != std::floor(d->boundary_value[a]) ||
+                 d->boundary_value[a] < -128 || d->boundary_value[a] > 127))
+                return fail(CAH_ERR_INCOMPATIBLE_SPACE,
+                            "the boundary Constant(%g): a state of Int8 cannot hold it",
+                            d->boundary_value[a]);
+        break;
+    case CAH_RULE_LENIA:
+        if (d->dtype != CAH_DTYPE_F32)
+            return fail(CAH_ERR_INVALID_STATE, "Lenia runs on Float32 states here");
+        if (d->n_taps < 1 || !d->tap_offsets || !d->tap_weights)
+            return fail(CAH_ERR_ARGUMENT, "Lenia needs its kernel taps");
+        if (max_abs(d->tap_offsets, d->n_taps) > LENIA_MAX_HALO)
+            return fail(CAH_ERR_ARGUMENT, "kernel fields wider than %d cells are not supported",
+                        LENIA_MAX_HALO);
+        if (d->lenia_precision != 32 && d->lenia_precision != 64)
+            return fail(CAH_ERR_ARGUMENT, "Lenia precision must be 32 or 64");
+        if (!(d->sigma > 0)) return fail(CAH_ERR_ARGUMENT, "sigma must be positive");
+        if (d->field_method != CAH_FIELD_DIRECT_SEQ && d->field_method != CAH_FIELD_DIRECT_F32)
+            return fail(CAH_ERR_UNSUPPORTED, "field method %d is not available", d->field_method);
+        break;
+    default:
+        return fail(CAH_ERR_ARGUMENT,
+                    "the HIP backend runs Life, Generations, the forest fire and Lenia (rule %d)",
+                    d->rule);
+    }
+    return CAH_OK;
+}
+
+// ---------------------------------------------------------------- state
+
+enum Path { P_GENERIC, P_LIFE_SWAR, P_FOREST, P_LENIA_SEQ, P_LENIA_F32COL };
+
+struct cah_state {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cah_desc d{};
+    std::vector<int32_t> offsets, taps;
+    std::vector<double> weights;
+    int64_t n1 = 0, n2 = 0, pitch = 0;  // pitch in elements
+    size_t es = 1;
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint8_t* mask_d = nullptr;
+    int32_t* ofs_d = nullptr;
+    int32_t* taps_d = nullptr;
+    double* w64_d = nullptr;
+    float* w32_d = nullptr;    // per-tap Float32 weights
+    float* wtab_d = nullptr;   // dense table for k_lenia_f32col
+    void* field_d = nullptr;   // field buffer for the sequential path
+    int32_t halo = 1;
+    int32_t lenia_H = 0;
+    Path path = P_GENERIC;
+    int32_t life_R = 0;
+    bool bin_clean = false;  // every cell is 0 or 1 (Life only)
+    LifeArgs life{};
+    int32_t tune_vb = 0, tune_rows = 0;
+};
+
+static Lat lat_of(const cah_state* s) {
+    Lat L;
+    L.pitch = s->pitch;
+    L.n1 = (int32_t)s->n1;
+    L.n2 = (int32_t)s->n2;
+    L.b1 = s->d.boundary[0];
+    L.b2 = s->d.ndims == 2 ? s->d.boundary[1] : CAH_BOUNDARY_PERIODIC;
+    L.two_d = s->d.ndims == 2;
+    return L;
+}
+
+// Offsets as a set equal to Moore(R) (radius returned) or VonNeumann(1).
+static int moore_radius(const std::vector<int32_t>& o) {
+    const int m = (int)o.size() / 2;
+    int r = 0;
+    for (int k = 0; k < 2 * m; ++k) r = std::max(r, std::abs(o[k]));
+    if (r < 1 || m != (2 * r + 1) * (2 * r + 1) - 1) return 0;
+    std::set<std::pair<int, int>> s;
+    for (int k = 0; k < m; ++k) s.insert({o[2 * k], o[2 * k + 1]});
+    if ((int)s.size() != m || s.count({0, 0})) return 0;
+    return r;
+}
+
+static bool is_vn1(const std::vector<int32_t>& o) {
+    if (o.size() != 8) return false;
+    std::set<std::pair<int, int>> s;
+    for (int k = 0; k < 4; ++k) s.insert({o[2 * k], o[2 * k + 1]});
+    return s == std::set<std::pair<int, int>>{{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
+}
+
+// Runs of set bits of a count mask -> [lo, hi] intervals.
+static int intervals(uint64_t m, uint32_t* lo, uint32_t* hi) {
+    int k = 0;
+    for (int c = 0; c < 64;) {
+        if (!((m >> c) & 1u)) {
+            ++c;
+            continue;
+        }
+        int e = c;
+        while (e + 1 < 64 && ((m >> (e + 1)) & 1u)) ++e;
+        if (k == LIFE_MAX_IV) return -1;
+        lo[k] = (uint32_t)(128 - c) * 0x01010101u;
+        hi[k] = (uint32_t)(127 - e) * 0x01010101u;
+        ++k;
+        c = e + 1;
+    }
+    return k;
+}
+
+static void free_state(cah_state* s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
+                    (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d})
+        if (p) hipFree(p);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+template <typename T>
+static int upload(T** dst, const T* src, size_t n) {
+    HIPCHK(hipMalloc((void**)dst, std::max<size_t>(n, 1) * sizeof(T)));
+    if (n) HIPCHK(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return CAH_OK;
+}
+
+static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s) {
+    s->device = ctx->device;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->d = *d;
+    s->n1 = d->dims[0];
+    s->n2 = d->ndims == 2 ? d->dims[1] : 1;
+    s->es = d->dtype == CAH_DTYPE_F32 ? 4 : 1;
+    s->pitch = d->dtype == CAH_DTYPE_F32 ? (s->n1 + 15) / 16 * 16 : (s->n1 + 63) / 64 * 64;
+    s->offsets.assign(d->offsets, d->offsets + 2 * (size_t)d->n_offsets);
+    if (d->rule == CAH_RULE_LENIA) {
+        s->taps.assign(d->tap_offsets, d->tap_offsets + 2 * (size_t)d->n_taps);
+        s->weights.assign(d->tap_weights, d->tap_weights + d->n_taps);
+    }
+    s->d.offsets = nullptr;
+    s->d.tap_offsets = nullptr;
+    s->d.tap_weights = nullptr;
+    s->d.mask = nullptr;
+    const size_t bytes = (size_t)s->pitch * (size_t)s->n2 * s->es;
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(hipMalloc(&s->buf[b], bytes));
+        HIPCHK(hipMemsetAsync(s->buf[b], 0, bytes, s->stream));
+    }
+    if (d->mask) {
+        int rc = upload(&s->mask_d, d->mask, (size_t)(s->n1 * s->n2));
+        if (rc) return rc;
+    }
+    if (int rc = upload(&s->ofs_d, s->offsets.data(), s->offsets.size())) return rc;
+    s->halo = std::max(1, max_abs(s->offsets.data(), d->n_offsets));
+
+    if (d->rule == CAH_RULE_LENIA) {
+        s->halo = std::max(s->halo, max_abs(s->taps.data(), d->n_taps));
+        if (int rc = upload(&s->taps_d, s->taps.data(), s->taps.size())) return rc;
+        if (int rc = upload(&s->w64_d, s->weights.data(), s->weights.size())) return rc;
+        std::vector<float> w32(s->weights.begin(), s->weights.end());
+        if (int rc = upload(&s->w32_d, w32.data(), w32.size())) return rc;
+        const int ht = max_abs(s->taps.data(), d->n_taps);
+        s->lenia_H = ht <= 4 ? 4 : ht <= 8 ? 8 : ht <= 12 ? 12 : 16;
+        const int K = 2 * s->lenia_H + 1;
+        std::vector<float> tab((size_t)K * K, 0.0f);
+        for (int k = 0; k < d->n_taps; ++k)
+            tab[(size_t)(s->taps[2 * k + 1] + s->lenia_H) * K + (s->taps[2 * k] + s->lenia_H)] +=
+                (float)s->weights[k];
+        if (int rc = upload(&s->wtab_d, tab.data(), tab.size())) return rc;
+        const bool fused = d->field_method == CAH_FIELD_DIRECT_F32 &&
+                           d->scheme != CAH_SCHEME_CHECKERBOARD;
+        s->path = fused ? P_LENIA_F32COL : P_LENIA_SEQ;
+        if (!fused)
+            HIPCHK(hipMalloc(&s->field_d, (size_t)(s->n1 * s->n2) * (d->lenia_precision == 32 ? 4 : 8)));
+        return CAH_OK;
+    }
+
+    s->path = P_GENERIC;
+    const bool plain2d = d->ndims == 2 && d->scheme == CAH_SCHEME_SYNCHRONOUS && !d->mask;
+    if (plain2d && d->rule == CAH_RULE_LIFE) {
+        const int R = moore_radius(s->offsets);
+        LifeArgs& la = s->life;
+        int nb = intervals(d->birth, la.b_lo, la.b_hi);
+        int ns = intervals(d->survive, la.s_lo, la.s_hi);
+        if (R >= 1 && R <= 5 && nb >= 0 && ns >= 0) {
+            la.nb = nb;
+            la.ns = ns;
+            s->life_R = R;
+            s->path = P_LIFE_SWAR;
+        }
+    } else if (plain2d && d->rule == CAH_RULE_FOREST_FIRE && is_vn1(s->offsets)) {
+        s->path = P_FOREST;
+    }
+    s->bin_clean = true;  // the buffers start all zero
+    return CAH_OK;
+}
+
+extern "C" int cah_state_create(cah_ctx* ctx, const cah_desc* d, cah_state** out) {
+    if (!ctx || !out) return fail(CAH_ERR_ARGUMENT, "cah_state_create: NULL argument");
+    if (int rc = cah_check_problem(d)) return rc;
+    cah_state* s = new cah_state();
+    int rc = create_state(ctx, d, s);
+    if (rc) {
+        std::string keep = g_err;
+        free_state(s);
+        g_err = keep;
+        return rc;
+    }
+    *out = s;
+    return CAH_OK;
+}
+
+extern "C" int cah_state_destroy(cah_state* s) {
+    free_state(s);
+    return CAH_OK;
+}
+
+extern "C" int cah_state_tune(cah_state* s, int32_t vb, int32_t rows) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "state is NULL");
+    if (vb != 0 && vb != 1 && vb != 2 && vb != 4)
+        return fail(CAH_ERR_ARGUMENT, "vb must be 0, 1, 2 or 4");
+    if (rows < 0) return fail(CAH_ERR_ARGUMENT, "rows must be >= 0");
+    s->tune_vb = vb;
+    s->tune_rows = rows;
+    return CAH_OK;
+}
+
+static size_t state_bytes(const cah_state* s) { return (size_t)(s->n1 * s->n2) * s->es; }
+
+static bool scan_binary(const int8_t* h, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if ((uint8_t)h[i] > 1) return false;
+    return true;
+}
+
+extern "C" int cah_state_set(cah_state* s, const void* host, size_t nbytes) {
+    if (!s || !host) return fail(CAH_ERR_ARGUMENT, "cah_state_set: NULL argument");
+    if (nbytes != state_bytes(s))
+        return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpy2DAsync(s->buf[s->cur], (size_t)s->pitch * s->es, host, (size_t)s->n1 * s->es,
+                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->es == 1) s->bin_clean = scan_binary((const int8_t*)host, nbytes);
+    return CAH_OK;
+}
+
+extern "C" int cah_state_get(cah_state* s, void* host, size_t nbytes) {
+    if (!s || !host) return fail(CAH_ERR_ARGUMENT, "cah_state_get: NULL argument");
+    if (nbytes != state_bytes(s))
+        return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpy2DAsync(host, (size_t)s->n1 * s->es, s->buf[s->cur], (size_t)s->pitch * s->es,
+                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+extern "C" int cah_state_set_device(cah_state* s, const void* dptr, size_t nbytes) {
+    if (!s || !dptr) return fail(CAH_ERR_ARGUMENT, "cah_state_set_device: NULL argument");
+    if (nbytes != state_bytes(s))
+        return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpy2DAsync(s->buf[s->cur], (size_t)s->pitch * s->es, dptr, (size_t)s->n1 * s->es,
+                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->es == 1) s->bin_clean = false;  // not scanned
+    return CAH_OK;
+}
+
+extern "C" int cah_state_get_device(cah_state* s, void* dptr, size_t nbytes) {
+    if (!s || !dptr) return fail(CAH_ERR_ARGUMENT, "cah_state_get_device: NULL argument");
+    if (nbytes != state_bytes(s))
+        return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpy2DAsync(dptr, (size_t)s->n1 * s->es, s->buf[s->cur], (size_t)s->pitch * s->es,
+                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+static int cell_ptr(cah_state* s, int64_t i1, int64_t i2, char** p) {
+    if (i1 < 0 || i1 >= s->n1 || i2 < 0 || i2 >= s->n2)
+        return fail(CAH_ERR_ARGUMENT, "cell (%lld, %lld) is off the lattice", (long long)i1,
+                    (long long)i2);
+    *p = (char*)s->buf[s->cur] + ((size_t)i2 * (size_t)s->pitch + (size_t)i1) * s->es;
+    return CAH_OK;
+}
+
+extern "C" int cah_state_read_cell(cah_state* s, int64_t i1, int64_t i2, void* value) {
+    if (!s || !value) return fail(CAH_ERR_ARGUMENT, "NULL argument");
+    char* p;
+    if (int rc = cell_ptr(s, i1, i2, &p)) return rc;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpyAsync(value, p, s->es, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+extern "C" int cah_state_write_cell(cah_state* s, int64_t i1, int64_t i2, const void* value) {
+    if (!s || !value) return fail(CAH_ERR_ARGUMENT, "NULL argument");
+    char* p;
+    if (int rc = cell_ptr(s, i1, i2, &p)) return rc;
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipMemcpyAsync(p, value, s->es, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->es == 1 && *(const uint8_t*)value > 1) s->bin_clean = false;
+    return CAH_OK;
+}
+
+extern "C" int cah_state_sync(cah_state* s) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "state is NULL");
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+extern "C" int cah_state_stream(cah_state* s, void** stream) {
+    if (!s || !stream) return fail(CAH_ERR_ARGUMENT, "NULL argument");
+    *stream = (void*)s->stream;
+    return CAH_OK;
+}
+
+// ---------------------------------------------------------------- launches
+
+static int waves_to_blocks(int64_t nwaves) { return (int)((nwaves + 3) / 4); }
+
+// Rows per wave: enough waves to fill 256 CUs several times over, but at least
+// a few windows tall so the (2R)-row warm-up stays a small share of the loads.
+static int32_t band_rows(const cah_state* s, int64_t nbands, int R) {
+    if (s->tune_rows > 0) return s->tune_rows;
+    const int64_t target_waves = 8192;
+    int64_t rows = (s->n2 * nbands + target_waves - 1) / target_waves;
+    rows = std::max<int64_t>(rows, 8 * R + 8);
+    rows = std::min<int64_t>(rows, 256);
+    return (int32_t)std::max<int64_t>(1, std::min<int64_t>(rows, s->n2));
+}
+
+template <int R, int VB>
+static void launch_life_vb(LifeArgs a, bool bin, int blocks, hipStream_t st) {
+    if (bin) k_life_swar<R, VB, true><<<blocks, 256, 0, st>>>(a);
+    else k_life_swar<R, VB, false><<<blocks, 256, 0, st>>>(a);
+}
+
+template <int R>
+static int launch_life_r(cah_state* s, LifeArgs a, int vb) {
+    const int blocks = waves_to_blocks(a.nwaves);
+    if (R == 5 && vb < 2) vb = 2;
+    if (vb == 4) launch_life_vb<R, 4>(a, s->bin_clean, blocks, s->stream);
+    else if (vb == 2) launch_life_vb<R, 2>(a, s->bin_clean, blocks, s->stream);
+    else if constexpr (R <= 4) launch_life_vb<R, 1>(a, s->bin_clean, blocks, s->stream);
+    return CAH_OK;
+}
+
+static int life_vb(const cah_state* s) {
+    int vb = s->tune_vb ? s->tune_vb : (s->n1 >= 2048 ? 2 : 1);
+    if (s->life_R == 5 && vb < 2) vb = 2;
+    return vb;
+}
+
+static int launch_life(cah_state* s) {
+    LifeArgs a = s->life;
+    a.cur = (const int8_t*)s->buf[s->cur];
+    a.next = (int8_t*)s->buf[1 - s->cur];
+    a.L = lat_of(s);
+    a.cv1 = (int32_t)s->d.boundary_value[0];
+    a.cv2 = (int32_t)s->d.boundary_value[1];
+    // the no-conversion kernel needs every value it may read, the Constant
+    // edges included, to be 0 or 1
+    const bool edges01 = (s->d.boundary[0] != CAH_BOUNDARY_CONSTANT || (a.cv1 == 0 || a.cv1 == 1)) &&
+                         (s->d.boundary[1] != CAH_BOUNDARY_CONSTANT || (a.cv2 == 0 || a.cv2 == 1));
+    const bool saved = s->bin_clean;
+    s->bin_clean = saved && edges01;
+    const int vb = life_vb(s);
+    const int64_t ow = 62 * 4 * vb;
+    a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
+    a.rows = band_rows(s, a.nbands, s->life_R);
+    a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
+    int rc;
+    switch (s->life_R) {
+    case 1: rc = launch_life_r<1>(s, a, vb); break;
+    case 2: rc = launch_life_r<2>(s, a, vb); break;
+    case 3: rc = launch_life_r<3>(s, a, vb); break;
+    case 4: rc = launch_life_r<4>(s, a, vb); break;
+    default: rc = launch_life_r<5>(s, a, vb); break;
+    }
+    s->bin_clean = saved;
+    return rc;
+}
+
+static int launch_forest(cah_state* s, int64_t t, const cah_params* p) {
+    ForestArgs a{};
+    a.cur = (const int8_t*)s->buf[s->cur];
+    a.next = (int8_t*)s->buf[1 - s->cur];
+    a.L = lat_of(s);
+    a.cv1 = (int32_t)s->d.boundary_value[0];
+    a.cv2 = (int32_t)s->d.boundary_value[1];
+    a.seed = s->d.seed;
+    a.t = t;
+    a.p = p->ff_p;
+    a.f = p->ff_f;
+    a.p_f32 = p->ff_p_is_f32;
+    a.f_f32 = p->ff_f_is_f32;
+    const int vb = s->tune_vb ? s->tune_vb : 1;
+    const int64_t ow = 62 * 4 * vb;
+    a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
+    a.rows = band_rows(s, a.nbands, 1);
+    a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
+    const int blocks = waves_to_blocks(a.nwaves);
+    const bool small = (uint64_t)(s->n1 * s->n2) < 0xFFFFFFFFull;
+    if (vb == 4) {
+        if (small) k_forest_vn1<4, true><<<blocks, 256, 0, s->stream>>>(a);
+        else k_forest_vn1<4, false><<<blocks, 256, 0, s->stream>>>(a);
+    } else if (vb == 2) {
+        if (small) k_forest_vn1<2, true><<<blocks, 256, 0, s->stream>>>(a);
+        else k_forest_vn1<2, false><<<blocks, 256, 0, s->stream>>>(a);
+    } else {
+        if (small) k_forest_vn1<1, true><<<blocks, 256, 0, s->stream>>>(a);
+        else k_forest_vn1<1, false><<<blocks, 256, 0, s->stream>>>(a);
+    }
+    return CAH_OK;
+}
+
+static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p) {
+    GenArgs a{};
+    a.cur = (const int8_t*)s->buf[s->cur];
+    a.next = (int8_t*)s->buf[1 - s->cur];
+    a.mask = s->mask_d;
+    a.ofs = s->ofs_d;
+    a.L = lat_of(s);
+    a.h = s->halo;
+    a.nofs = s->d.n_offsets;
+    a.cv1 = (int32_t)s->d.boundary_value[0];
+    a.cv2 = (int32_t)s->d.boundary_value[1];
+    a.rule = s->d.rule;
+    a.C = s->d.gen_states;
+    a.birth = s->d.birth;
+    a.survive = s->d.survive;
+    a.seed = s->d.seed;
+    a.t = t;
+    a.scheme = s->d.scheme;
+    a.gate = gate;
+    a.sp = s->d.scheme_p;
+    a.sp_f32 = s->d.scheme_p_is_f32;
+    if (p) {
+        a.ffp = p->ff_p;
+        a.fff = p->ff_f;
+        a.ffp_f32 = p->ff_p_is_f32;
+        a.fff_f32 = p->ff_f_is_f32;
+    }
+    dim3 grid((unsigned)((s->n1 + GEN_TX - 1) / GEN_TX), (unsigned)((s->n2 + GEN_TY - 1) / GEN_TY));
+    const size_t lds = (size_t)(GEN_TX + 2 * a.h) * (GEN_TY + 2 * a.h);
+    k_generic_i8<<<grid, 256, lds, s->stream>>>(a);
+    return CAH_OK;
+}
+
+static LeniaArgs lenia_args(cah_state* s, int64_t t, int gate) {
+    LeniaArgs a{};
+    a.cur = (const float*)s->buf[s->cur];
+    a.next = (float*)s->buf[1 - s->cur];
+    a.mask = s->mask_d;
+    a.taps = s->taps_d;
+    a.w64 = s->w64_d;
+    a.w32 = s->w32_d;
+    a.field = s->field_d;
+    a.L = lat_of(s);
+    a.h = s->halo;
+    a.ntaps = s->d.n_taps;
+    a.cv1 = (float)s->d.boundary_value[0];
+    a.cv2 = (float)s->d.boundary_value[1];
+    a.prec32 = s->d.lenia_precision == 32;
+    a.mu = s->d.mu;
+    a.sigma = s->d.sigma;
+    a.dt = s->d.dt;
+    a.seed = s->d.seed;
+    a.t = t;
+    a.scheme = s->d.scheme;
+    a.gate = gate;
+    a.sp = s->d.scheme_p;
+    a.sp_f32 = s->d.scheme_p_is_f32;
+    return a;
+}
+
+static int launch_lenia_f32col(cah_state* s, int64_t t) {
+    LeniaArgs a = lenia_args(s, t, 0);
+    a.w32 = s->wtab_d;
+    dim3 grid((unsigned)((s->n1 + 63) / 64), (unsigned)((s->n2 + 31) / 32));
+    switch (s->lenia_H) {
+    case 4: k_lenia_f32col<4><<<grid, 256, 0, s->stream>>>(a); break;
+    case 8: k_lenia_f32col<8><<<grid, 256, 0, s->stream>>>(a); break;
+    case 12: k_lenia_f32col<12><<<grid, 256, 0, s->stream>>>(a); break;
+    default: k_lenia_f32col<16><<<grid, 256, 0, s->stream>>>(a); break;
+    }
+    return CAH_OK;
+}
+
+static int launch_lenia_field_seq(cah_state* s, int64_t t) {
+    LeniaArgs a = lenia_args(s, t, 0);
+    dim3 grid((unsigned)((s->n1 + LEN_TX - 1) / LEN_TX), (unsigned)((s->n2 + LEN_TY - 1) / LEN_TY));
+    const size_t lds = (size_t)(LEN_TX + 2 * a.h) * (LEN_TY + 2 * a.h) * sizeof(float);
+    if (a.prec32) k_lenia_field_seq<float><<<grid, 256, lds, s->stream>>>(a);
+    else k_lenia_field_seq<double><<<grid, 256, lds, s->stream>>>(a);
+    return CAH_OK;
+}
+
+static int launch_lenia_growth(cah_state* s, int64_t t, int gate) {
+    LeniaArgs a = lenia_args(s, t, gate);
+    dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
+    k_lenia_growth<<<grid, 256, 0, s->stream>>>(a);
+    return CAH_OK;
+}
+
+static int ngates(const cah_desc& d) { return d.scheme == CAH_SCHEME_CHECKERBOARD ? 2 : 1; }
+
+extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_params* p) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "state is NULL");
+    if (nsteps < 0 || t0 < 0) return fail(CAH_ERR_ARGUMENT, "t0 and nsteps must be >= 0");
+    if (s->d.rule == CAH_RULE_FOREST_FIRE && !p)
+        return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
+    HIPCHK(hipSetDevice(s->device));
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const int64_t t = t0 + k;
+        if (s->path == P_LENIA_SEQ) launch_lenia_field_seq(s, t);
+        for (int g = 0; g < ngates(s->d); ++g) {
+            switch (s->path) {
+            case P_LIFE_SWAR: launch_life(s); break;
+            case P_FOREST: launch_forest(s, t, p); break;
+            case P_LENIA_SEQ: launch_lenia_growth(s, t, g); break;
+            case P_LENIA_F32COL: launch_lenia_f32col(s, t); break;
+            default: launch_generic(s, t, g, p); break;
+            }
+            s->cur = 1 - s->cur;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    // Life maps {0,1} to {0,1}; any other rule may leave other values
+    if (s->d.rule != CAH_RULE_LIFE && nsteps > 0) s->bin_clean = false;
+    return CAH_OK;
+}
+
+// ---------------------------------------------------------------- synthetic inputs
+
+extern "C" int cah_state_fill_bernoulli(cah_state* s, uint64_t seed, double p) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "state is NULL");
+    if (s->es != 1) return fail(CAH_ERR_ARGUMENT, "fill_bernoulli needs an Int8 state");
+    HIPCHK(hipSetDevice(s->device));
+    dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
+    k_fill_bernoulli<<<grid, 256, 0, s->stream>>>((int8_t*)s->buf[s->cur], lat_of(s), seed, p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->bin_clean = true;
+    return CAH_OK;
+}
+
+extern "C" int cah_state_fill_uniform_box(cah_state* s, uint64_t seed, int64_t lo1, int64_t len1,
+                                          int64_t lo2, int64_t len2) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "state is NULL");
+    if (s->es != 4) return fail(CAH_ERR_ARGUMENT, "fill_uniform_box needs a Float32 state");
+    HIPCHK(hipSetDevice(s->device));
+    dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
+    k_fill_uniform_box<<<grid, 256, 0, s->stream>>>((float*)s->buf[s->cur], lat_of(s), seed,
+                                                    (int32_t)lo1, (int32_t)len1, (int32_t)lo2,
+                                                    (int32_t)len2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+// ---------------------------------------------------------------- batches
+
+struct cah_batch {
+    int device = 0;
+    std::vector<cah_state*> members;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" int cah_batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, cah_batch** out) {
+    if (!ctx || !descs || !out || n < 1) return fail(CAH_ERR_ARGUMENT, "cah_batch_create: bad arguments");
+    cah_batch* b = new cah_batch();
+    b->device = ctx->device;
+    for (int32_t i = 0; i < n; ++i) {
+        if (descs[i].rule != descs[0].rule || descs[i].dtype != CAH_DTYPE_I8) {
+            for (auto* m : b->members) free_state(m);
+            delete b;
+            return fail(CAH_ERR_ARGUMENT, "a batch holds trajectories of one Int8 rule");
+        }
+        cah_state* s = nullptr;
+        int rc = cah_state_create(ctx, &descs[i], &s);
+        if (rc) {
+            std::string keep = g_err;
+            for (auto* m : b->members) free_state(m);
+            delete b;
+            g_err = keep;
+            return rc;
+        }
+        b->members.push_back(s);
+    }
+    b->stream = b->members[0]->stream;
+    *out = b;
+    return CAH_OK;
+}
+
+static int member(cah_batch* b, int32_t i, cah_state** s) {
+    if (!b || i < 0 || i >= (int32_t)b->members.size())
+        return fail(CAH_ERR_ARGUMENT, "no trajectory %d in the batch", i);
+    *s = b->members[i];
+    return CAH_OK;
+}
+
+extern "C" int cah_batch_set(cah_batch* b, int32_t i, const void* host, size_t nbytes) {
+    cah_state* s;
+    if (int rc = member(b, i, &s)) return rc;
+    return cah_state_set(s, host, nbytes);
+}
+
+extern "C" int cah_batch_get(cah_batch* b, int32_t i, void* host, size_t nbytes) {
+    cah_state* s;
+    if (int rc = member(b, i, &s)) return rc;
+    return cah_state_get(s, host, nbytes);
+}
+
+extern "C" int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, double p) {
+    cah_state* s;
+    if (int rc = member(b, i, &s)) return rc;
+    return cah_state_fill_bernoulli(s, seed, p);
+}
+
+extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const cah_params* p,
+                                 int32_t use_graph) {
+    if (!b) return fail(CAH_ERR_ARGUMENT, "batch is NULL");
+    (void)use_graph;
+    for (auto* s : b->members)
+        if (int rc = cah_advance(s, t0, nsteps, p)) return rc;
+    return CAH_OK;
+}
+
+extern "C" int cah_batch_sync(cah_batch* b) {
+    if (!b) return fail(CAH_ERR_ARGUMENT, "batch is NULL");
+    for (auto* s : b->members)
+        if (int rc = cah_state_sync(s)) return rc;
+    return CAH_OK;
+}
+
+extern "C" int cah_batch_stream(cah_batch* b, void** stream) {
+    if (!b || !stream) return fail(CAH_ERR_ARGUMENT, "NULL argument");
+    *stream = (void*)b->stream;
+    return CAH_OK;
+}
+
+extern "C" int cah_batch_destroy(cah_batch* b) {
+    if (!b) return CAH_OK;
+    for (auto* s : b->members) free_state(s);
+    delete b;
+    return CAH_OK;
+}

@@ -1,0 +1,123 @@
+// device.hpp -- per-cell building blocks shared by every sweep kernel (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cahip.h"
+
+#define CAH_DEV __device__ __forceinline__
+
+// Philox4x32-10 (src/random.jl:6-37).  The key schedule is the same for every
+// lane of a wave, so the compiler keeps it in scalar registers; counter words
+// that are wave-uniform (the time, the high half of the cell id) are multiplied
+// on the scalar unit in the first two rounds.
+CAH_DEV void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
+                      uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// _philox_draw + _uniform (src/random.jl:102-120): FP64 in [0, 1).
+CAH_DEV double uniform64(uint64_t seed, uint64_t cellid, int64_t t, uint32_t k) {
+    uint32_t c0 = (uint32_t)cellid, c1 = (uint32_t)(cellid >> 32), c2 = (uint32_t)(uint64_t)t,
+             c3 = k;
+    philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t bits = ((uint64_t)c0 << 32) | (uint64_t)c1;
+    return (double)(bits >> 11) * 0x1.0p-53;
+}
+
+// _uniform32 (src/random.jl:109-114): 24 random bits as a Float32.
+CAH_DEV float uniform32(uint64_t seed, uint64_t cellid, int64_t t, uint32_t k) {
+    uint32_t c0 = (uint32_t)cellid, c1 = (uint32_t)(cellid >> 32), c2 = (uint32_t)(uint64_t)t,
+             c3 = k;
+    philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return (float)(c0 >> 8) * (1.0f / 16777216.0f);
+}
+
+// bernoulli(ctx, p, k) (src/random.jl:129-130): the comparison stays in FP64 for a
+// Float64 p, and uses the 24-bit draw for a Float32 p.
+CAH_DEV bool bernoulli(uint64_t seed, uint64_t cellid, int64_t t, uint32_t k, double p,
+                       int p_f32) {
+    if (p_f32) return uniform32(seed, cellid, t, k) < (float)p;
+    return uniform64(seed, cellid, t, k) < p;
+}
+
+// _map_axis (src/spaces.jl:46-51) on 0-based indices.  `out` is set only for a
+// Constant axis and an index off the lattice.
+CAH_DEV int32_t map_axis0(int32_t kind, int32_t g, int32_t n, bool& out) {
+    out = false;
+    if (g >= 0 && g < n) return g;
+    if (kind == CAH_BOUNDARY_PERIODIC) {
+        int32_t m = g % n;
+        return m < 0 ? m + n : m;
+    }
+    if (kind == CAH_BOUNDARY_REFLECTING) {
+        const int32_t n2 = 2 * n;
+        int32_t m = g % n2;
+        if (m < 0) m += n2;
+        return m < n ? m : n2 - 1 - m;
+    }
+    out = true;
+    return g;
+}
+
+// Lattice geometry and edges as the kernels see it.  The state is stored dense
+// per column-major row (first axis fastest) with a row pitch in elements.
+struct Lat {
+    int64_t pitch;
+    int32_t n1, n2;
+    int32_t b1, b2;
+    int32_t two_d;
+};
+
+// _boundary_read (src/backends.jl:177-200): the value at 0-based (g1, g2), which may
+// lie off the lattice.  When both axes are off a Constant edge, axis 2 wins.
+template <typename T>
+CAH_DEV T read_mapped(const T* u, const Lat& L, T cv1, T cv2, int32_t g1, int32_t g2) {
+    bool o1, o2;
+    const int32_t m2 = map_axis0(L.b2, g2, L.n2, o2);
+    if (o2) return cv2;
+    const int32_t m1 = map_axis0(L.b1, g1, L.n1, o1);
+    if (o1) return cv1;
+    return u[(int64_t)m2 * L.pitch + m1];
+}
+
+// _gate_open (src/schemes.jl:84-89).  `gate` is the Checkerboard half (0: even
+// index sum first, 1: odd); cellid is 1-based on the unpadded lattice.
+CAH_DEV bool gate_open(int32_t scheme, int32_t gate, double sp, int32_t sp_f32, uint64_t seed,
+                       uint64_t cellid, int64_t t, int32_t x, int32_t y, int32_t two_d) {
+    if (scheme == CAH_SCHEME_STOCHASTIC) {
+        if (sp_f32) return uniform32(seed, cellid, t, 0) < (float)sp;
+        return uniform64(seed, cellid, t, 0) < sp;
+    }
+    if (scheme == CAH_SCHEME_CHECKERBOARD) {
+        const int32_t s = (x + 1) + (two_d ? (y + 1) : 0);
+        return (s & 1) == gate;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------- SWAR on bytes
+
+// Bytes equal to `v` (v in 0..255 replicated) -> 0x01 in those bytes, 0 elsewhere.
+// Exact for every byte value (no borrow crosses a byte).
+CAH_DEV uint32_t bytes_eq(uint32_t x, uint32_t vrep) {
+    const uint32_t t = x ^ vrep;                                   // 0 where equal
+    const uint32_t nz = (((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
+    return (nz >> 7) ^ 0x01010101u;
+}
+
+CAH_DEV uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t sh) {
+    return __builtin_amdgcn_alignbit(hi, lo, sh);
+}

@@ -1,0 +1,227 @@
+/*
+ * cahip.h -- C ABI of the MI355X backend for the synchronous stencil sweep of
+ * CellularAutomataCore.jl.
+ *
+ * The reference backend interface is a set of Julia generic functions dispatched
+ * on the backend type (src/backends.jl:1-14, src/backends.jl:89-148).  Julia
+ * reaches this library through `ccall`; every entry point below names the
+ * reference function it replaces.  Plain pointers and sizes only: the caller owns
+ * host buffers, the library owns device buffers.
+ *
+ * Conventions
+ *   - Lattices are stored column-major (first axis fastest), as Julia arrays are.
+ *   - `cellid` is the 1-based linear index of a cell on the unpadded lattice
+ *     (src/backends.jl:446, :458); `t` is the time of the state being read.
+ *   - Every call returns CAH_OK (0) or a negative CAH_ERR_* code, and leaves a
+ *     message for cah_last_error() on failure.
+ *   - Calls on distinct handles may run concurrently from different host
+ *     threads; each state owns one HIP stream.  Calls on one handle must not
+ *     overlap.
+ */
+#ifndef CAHIP_H
+#define CAHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- error codes
+ * Mapped by the Julia glue onto the reference's exception types
+ * (src/errors.jl:28-132): */
+#define CAH_OK                      0
+#define CAH_ERR_ARGUMENT           -1  /* ArgumentError                  */
+#define CAH_ERR_INCOMPATIBLE_SPACE -2  /* IncompatibleSpaceError         */
+#define CAH_ERR_UNSUPPORTED        -3  /* UnsupportedCombinationError    */
+#define CAH_ERR_OUT_OF_MEMORY      -4  /* OutOfMemoryError               */
+#define CAH_ERR_STATE_SHAPE        -5  /* StateShapeError                */
+#define CAH_ERR_INVALID_STATE      -6  /* InvalidStateError              */
+#define CAH_ERR_DEVICE             -7  /* a HIP runtime failure          */
+#define CAH_ERR_BUSY               -8  /* server: admission refused now  */
+
+/* ---------------------------------------------------------------- enums */
+#define CAH_DTYPE_I8   1   /* Int8 states                                 */
+#define CAH_DTYPE_F32  2   /* Float32 states (Lenia)                      */
+
+/* Boundary of one lattice axis (src/spaces.jl:16-51). */
+#define CAH_BOUNDARY_PERIODIC   0
+#define CAH_BOUNDARY_REFLECTING 1
+#define CAH_BOUNDARY_CONSTANT   2
+
+/* Gated schemes (src/schemes.jl:22-93). */
+#define CAH_SCHEME_SYNCHRONOUS  0
+#define CAH_SCHEME_STOCHASTIC   1
+#define CAH_SCHEME_CHECKERBOARD 2
+
+/* Rule families the backend runs (check_problem refuses anything else). */
+#define CAH_RULE_LIFE        1  /* Life(birth, survive)       src/rules.jl:354-389 */
+#define CAH_RULE_GENERATIONS 2  /* Generations(b, s, C)       src/rules.jl:424-456 */
+#define CAH_RULE_FOREST_FIRE 3  /* the forest-fire LocalRule of README.md:56-60,
+                                   Int8 states empty=0, tree=1, fire=2 */
+#define CAH_RULE_LENIA       4  /* Lenia(; R, mu, sigma, dt)  src/lenia.jl:17-47   */
+
+/* How the Lenia potential (its :potential KernelField) is computed. */
+#define CAH_FIELD_DIRECT_SEQ  0  /* one sequential sum per cell in tap order, in the
+                                    rule's precision (FP64 for Lenia(), FP32 for
+                                    precision=Float32), as _field_at
+                                    (src/environment.jl:163-169)                   */
+#define CAH_FIELD_DIRECT_F32  1  /* FP32 FMAs over a dense weight table             */
+#define CAH_FIELD_MFMA_F16    2  /* implicit GEMM on MFMA, FP16 in, FP32 accumulate  */
+#define CAH_FIELD_MFMA_BF16   3  /* implicit GEMM on MFMA, BF16 in, FP32 accumulate  */
+#define CAH_FIELD_FFT_F32     4  /* FP32 FFT cross-correlation, periodic lattices
+                                    (ext/CellularAutomataCoreFFTWExt.jl:26-30)     */
+#define CAH_FIELD_MFMA_F16X2  5  /* split-FP16 (hi+lo) MFMA, FP32 accumulate         */
+
+/* ---------------------------------------------------------------- descriptors */
+
+/* Everything a CAProblem fixes for the sweep (src/problem.jl:38-92), resolved on
+ * the host: the neighbourhood as column-major offsets (src/neighbourhoods.jl:124-153),
+ * the Lenia kernel as read offsets and normalised weights
+ * (src/environment.jl:59-69, src/kernels.jl:174-189). */
+typedef struct cah_desc {
+    int32_t  ndims;              /* 1 or 2                                      */
+    int32_t  dtype;              /* CAH_DTYPE_*                                 */
+    int64_t  dims[2];            /* size(space); dims[1] = 1 when ndims == 1    */
+    int32_t  boundary[2];        /* CAH_BOUNDARY_* per axis                      */
+    double   boundary_value[2];  /* Constant(value) per axis                    */
+    int32_t  scheme;             /* CAH_SCHEME_*                                */
+    int32_t  scheme_p_is_f32;    /* Stochastic{Float32}: the gate uses the
+                                    24-bit draw (src/schemes.jl:86-88)          */
+    double   scheme_p;           /* Stochastic(p)                               */
+    uint64_t seed;               /* prob.seed                                   */
+    int32_t  rule;               /* CAH_RULE_*                                  */
+    int32_t  gen_states;         /* Generations: C                              */
+    uint64_t birth;              /* Life/Generations birth mask (bit n)         */
+    uint64_t survive;            /* Life/Generations survive mask               */
+    int32_t  n_offsets;          /* length of the resolved neighbourhood        */
+    int32_t  lenia_precision;    /* Lenia: 64 (Lenia()) or 32 (precision=Float32) */
+    const int32_t* offsets;      /* n_offsets pairs (o1, o2), in order          */
+    int32_t  n_taps;             /* Lenia field: number of taps                 */
+    int32_t  field_method;       /* CAH_FIELD_*                                 */
+    const int32_t* tap_offsets;  /* n_taps pairs: read offsets (-o)             */
+    const double*  tap_weights;  /* n_taps FP64 weights, summing to one         */
+    double   mu, sigma, dt;      /* Lenia growth parameters                     */
+    const uint8_t* mask;         /* NULL, or prod(dims) bytes; 0 = never update
+                                    (src/environment.jl:126-148)                */
+} cah_desc;
+
+/* Parameters that may change between steps: the integrator's `p`
+ * (src/backends.jl:121-130).  Only the forest fire reads any. */
+typedef struct cah_params {
+    double  ff_p;      /* growth probability  ctx.p.p  (draw k = 1) */
+    double  ff_f;      /* lightning probability ctx.p.f (draw k = 2) */
+    int32_t ff_p_is_f32;  /* p::Float32: bernoulli uses the 24-bit draw (src/random.jl:129-130) */
+    int32_t ff_f_is_f32;
+} cah_params;
+
+typedef struct cah_ctx   cah_ctx;
+typedef struct cah_state cah_state;
+typedef struct cah_batch cah_batch;
+
+/* ---------------------------------------------------------------- context */
+
+/* Open device `device` (HIP ordinal).  No reference counterpart: it is the
+ * device half of GPUBackend(device) (src/backends.jl:83-87). */
+int  cah_open(int device, cah_ctx** out);
+void cah_close(cah_ctx* ctx);
+
+/* Thread-local message of the last failed call on this thread. */
+const char* cah_last_error(void);
+
+/* ---------------------------------------------------------------- backend interface */
+
+/* supports(backend, scheme) (src/backends.jl:95, :153): 1 for the gated
+ * schemes, 0 otherwise.  `scheme` is CAH_SCHEME_* or any other integer. */
+int cah_supports(int scheme);
+
+/* check_problem(backend, prob, scheme) (src/backends.jl:105;
+ * ext/CellularAutomataCoreKernelAbstractionsExt.jl:42-49): CAH_OK, or the error
+ * the reference raises for a problem the backend cannot run. */
+int cah_check_problem(const cah_desc* d);
+
+/* allocate_state(backend, prob, scheme) (src/backends.jl:119, :408-422;
+ * ext/...KernelAbstractionsExt.jl:51-59): device buffers for the state and
+ * its double buffer.  The state starts all zero; load u0 with cah_state_set. */
+int cah_state_create(cah_ctx* ctx, const cah_desc* d, cah_state** out);
+
+/* set_state!(backend, st, u) (src/backends.jl:148, :525-530): copy the whole
+ * state from a host array in the layout of u0 (nbytes = prod(dims)*eltype). */
+int cah_state_set(cah_state* st, const void* host, size_t nbytes);
+
+/* current_state(backend, st) (src/backends.jl:139, :521-524): synchronise and
+ * copy the whole state into a host array the caller owns. */
+int cah_state_get(cah_state* st, void* host, size_t nbytes);
+
+/* The same two copies from / into a dense device array on the state's device
+ * (column-major, prod(dims) elements), for callers whose data already sits in
+ * HBM. */
+int cah_state_set_device(cah_state* st, const void* dptr, size_t nbytes);
+int cah_state_get_device(cah_state* st, void* dptr, size_t nbytes);
+
+/* read_cell(backend, st, I) / write_cell!(backend, st, I, v)
+ * (src/agents.jl:165, :174; src/backends.jl:531-534): one cell, 0-based
+ * coordinates (i1, i2); the value is an Int8 or a Float32 per dtype. */
+int cah_state_read_cell(cah_state* st, int64_t i1, int64_t i2, void* value);
+int cah_state_write_cell(cah_state* st, int64_t i1, int64_t i2, const void* value);
+
+/* advance!(backend, st, prob, scheme, p, t) (src/backends.jl:130, :494-506;
+ * ext/...KernelAbstractionsExt.jl:92-110), `nsteps` times: steps t0, t0+1, ...
+ * Work is queued on the state's stream; it returns before the GPU finishes
+ * (current_state / cah_state_get synchronise). */
+int cah_advance(cah_state* st, int64_t t0, int64_t nsteps, const cah_params* p);
+
+/* Block until the state's queued work is done. */
+int cah_state_sync(cah_state* st);
+
+/* The state's HIP stream (a hipStream_t), so callers can time its work with
+ * HIP events or order their own work after it. */
+int cah_state_stream(cah_state* st, void** stream);
+
+/* Launch shape of the band-walk kernels (no reference counterpart; set by the
+ * autotuner): vb = dwords per lane (1, 2 or 4), rows = rows per wave; 0 keeps
+ * the default.  Results do not depend on it. */
+int cah_state_tune(cah_state* st, int32_t vb, int32_t rows);
+
+/* Release a state; safe from any host thread (Julia finalizers). */
+int cah_state_destroy(cah_state* st);
+
+/* ---------------------------------------------------------------- synthetic inputs
+ * No reference counterpart.  The build's documented generator of initial states:
+ * cell `cellid` gets the draw of an independent Philox4x32-10 stream,
+ * ctr = (cellid lo32, cellid hi32, 0xFFFFFFFF, 0), key = (seed lo32, seed hi32),
+ * turned into u = ((r0 << 32 | r1) >> 11) * 2^-53. */
+
+/* Int8 state: cell = (u < p) ? 1 : 0. */
+int cah_state_fill_bernoulli(cah_state* st, uint64_t seed, double p);
+/* Float32 state: cell = Float32(u) inside the box [lo1, lo1+len1) x [lo2, lo2+len2)
+ * (0-based), 0 elsewhere. */
+int cah_state_fill_uniform_box(cah_state* st, uint64_t seed, int64_t lo1, int64_t len1,
+                               int64_t lo2, int64_t len2);
+
+/* ---------------------------------------------------------------- batched ensembles
+ * No single reference counterpart: the device form of
+ * solve(EnsembleProblem(prob), alg, EnsembleThreads(); trajectories=n)
+ * (src/problem.jl:571-589) for trajectories that share a rule and a scheme and
+ * differ in seed (and in lattice size).  Trajectory i advances exactly as a
+ * cah_state with descriptor descs[i] would. */
+
+/* n descriptors of one rule family (Int8 rules only). */
+int cah_batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, cah_batch** out);
+int cah_batch_set(cah_batch* b, int32_t i, const void* host, size_t nbytes);
+int cah_batch_get(cah_batch* b, int32_t i, void* host, size_t nbytes);
+int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, double p);
+/* One launch per step over every trajectory; with use_graph != 0 the nsteps
+ * launches are captured once into a hipGraph and replayed. */
+int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const cah_params* p,
+                      int32_t use_graph);
+int cah_batch_sync(cah_batch* b);
+int cah_batch_stream(cah_batch* b, void** stream);
+int cah_batch_destroy(cah_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAHIP_H */

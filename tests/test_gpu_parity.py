@@ -1,0 +1,285 @@
+"""Parity of the HIP backend with the CPU oracle, saved state by saved state
+(the conformance principle of test/backend_conformance.jl:207-231): bit-exact
+for every Int8 rule, within a stated tolerance for Lenia."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from cahip import model as M
+from cahip.backend import HIPBackend, make_desc, make_params
+from cahip.problem import CAProblem, solve
+from helpers import desc_for, forest_problem, ltl_rule, params_for, rng_state
+from test_oracle import CASES, SCHEMES
+
+pytestmark = pytest.mark.gpu
+NT = min(16, os.cpu_count() or 1)
+
+
+def oracle_saves(oracle, prob, scheme, times, halo=True):
+    d, keep = desc_for(prob, scheme, field_method=0)
+    out, u, t = [prob.u0], prob.u0, times[0]
+    for ts in times[1:]:
+        if halo and ts - t > 0:
+            u = oracle.advance_halo(d, u, t, ts - t, params_for(prob), nthreads=NT)
+        else:
+            u = oracle.advance_plain(d, u, t, ts - t, params_for(prob))
+        out.append(u)
+        t = ts
+    return out
+
+
+def assert_same(sol, ref):
+    assert len(sol.u) == len(ref)
+    for k, (a, b) in enumerate(zip(sol.u, ref)):
+        if not np.array_equal(a, b):
+            bad = np.argwhere(a != b)
+            raise AssertionError(f"save {k} (t={sol.t[k]}): {len(bad)} cells differ, first at "
+                                 f"{bad[0].tolist()}: gpu {a[tuple(bad[0])]} oracle {b[tuple(bad[0])]}")
+
+
+# ------------------------------------------------------------------ Conway Life, 512^2, 1000 steps
+
+def test_conway_512_1000_steps(oracle, backend):
+    u0 = oracle.fill_bernoulli(1, (512, 512), 0.5)
+    prob = CAProblem(M.Life("conway"), u0, 1000, space=M.Torus(512, 512))
+    sol = solve(prob, M.Synchronous(), backend=backend, saveat=250)
+    assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t))
+
+
+def test_device_generator_matches_oracle(oracle, backend):
+    prob = CAProblem(M.Life(), np.zeros((300, 257), np.int8), 1)
+    st = backend.allocate_state(prob, M.Synchronous())
+    try:
+        from cahip.backend import check
+        check(backend.lib.cah_state_fill_bernoulli(st.handle, 1, 0.5))
+        assert np.array_equal(backend.current_state(st), oracle.fill_bernoulli(1, (300, 257), 0.5))
+    finally:
+        backend.free_state(st)
+    lp = CAProblem(M.Lenia(), np.zeros((70, 66), np.float32), 1)
+    st = backend.allocate_state(lp, M.Synchronous())
+    try:
+        check(backend.lib.cah_state_fill_uniform_box(st.handle, 3, 10, 20, 5, 30))
+        assert np.array_equal(backend.current_state(st),
+                              oracle.fill_uniform_box(3, (70, 66), 10, 20, 5, 30))
+    finally:
+        backend.free_state(st)
+
+
+# ------------------------------------------------------------------ every rule x scheme (generic path)
+
+@pytest.mark.parametrize("case", [c[0] for c in CASES])
+@pytest.mark.parametrize("si", range(len(SCHEMES)))
+def test_rules_and_schemes(oracle, backend, case, si):
+    rule, space, vals = dict(CASES)[case]()
+    u = rng_state(11 + si, space.dims, vals)
+    p = {"p": 0.2, "f": 0.05} if isinstance(rule, M.ForestFire) else None
+    mask = None
+    if si == 1:
+        mask = np.ones(space.dims, bool)
+        mask.flat[::7] = False
+    prob = CAProblem(rule, u, (3, 15), p, space=space, seed=3, mask=mask)
+    sol = solve(prob, SCHEMES[si], backend=backend, saveat=4)
+    assert_same(sol, oracle_saves(oracle, prob, SCHEMES[si], sol.t, halo=False))
+
+
+# ------------------------------------------------------------------ band-walk Life (Moore R <= 5)
+
+LIFE_SWAR = [
+    (1, "B3/S23", (64, 48), (M.Periodic(), M.Periodic())),
+    (1, "B36/S23", (1000, 37), (M.Reflecting(), M.Constant(1))),
+    (1, "B3/S23", (13, 9), (M.Constant(0), M.Periodic())),
+    (2, "B2/S13", (257, 131), (M.Periodic(), M.Reflecting())),
+    (3, "B5/S45", (2050, 40), (M.Constant(1), M.Constant(0))),
+    (4, "B0/S8", (333, 71), (M.Periodic(), M.Periodic())),
+    (5, None, (1100, 97), (M.Periodic(), M.Periodic())),
+    (5, None, (517, 60), (M.Reflecting(), M.Constant(1))),
+    (5, None, (7, 5), (M.Periodic(), M.Reflecting())),
+]
+
+
+@pytest.mark.parametrize("i", range(len(LIFE_SWAR)))
+@pytest.mark.parametrize("vb", [0, 1, 2, 4])
+def test_life_band_walk(oracle, i, vb):
+    r, text, dims, bounds = LIFE_SWAR[i]
+    if r == 5 and vb == 1:
+        pytest.skip("radius 5 needs two dwords per lane")
+    rule = ltl_rule() if text is None else M.Life(text, neighbourhood=M.Moore(r))
+    space = M.Lattice(dims, bounds)
+    u = rng_state(i, dims, [0, 1])
+    if r == 5:  # dense enough for births at 34..45
+        u = np.asfortranarray((np.random.default_rng(i).random(dims) < 0.35).astype(np.int8))
+    prob = CAProblem(rule, u, 12, space=space, seed=1)
+    be = HIPBackend(tune=(vb, 8) if vb else None)
+    sol = solve(prob, M.Synchronous(), backend=be, saveat=3)
+    assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t, halo=False))
+
+
+def test_life_band_walk_non_binary_states(oracle, backend):
+    # set_state! may bring values a Life rule never makes; they count as not alive
+    dims = (300, 50)
+    u = rng_state(5, dims, [0, 1, 2, -1, 3])
+    rule = M.Life("B3/S23")
+    prob = CAProblem(rule, np.zeros(dims, np.int8), 4, space=M.Grid(*dims, boundary=M.Constant(2)))
+    st = backend.allocate_state(prob, M.Synchronous())
+    try:
+        backend.set_state(st, u)
+        backend.advance(st, prob, M.Synchronous(), None, 0, 4)
+        got = backend.current_state(st)
+    finally:
+        backend.free_state(st)
+    d, keep = desc_for(prob)
+    assert np.array_equal(got, oracle.advance_plain(d, u, 0, 4))
+
+
+def test_ltl_1024(oracle, backend):
+    u0 = oracle.fill_bernoulli(1, (1024, 768), 0.5)
+    prob = CAProblem(ltl_rule(), u0, 24, space=M.Torus(1024, 768))
+    sol = solve(prob, M.Synchronous(), backend=backend, saveat=8)
+    assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t))
+
+
+# ------------------------------------------------------------------ forest fire, VN(1)
+
+FOREST = [
+    ((64, 64), M.Reflecting(), 0.03, 1e-4),
+    ((1000, 300), M.Reflecting(), 0.3, 0.01),
+    ((517, 33), M.Periodic(), 0.5, 0.05),
+    ((100, 77), M.Constant(2), 0.2, 0.02),
+    ((5, 3), M.Reflecting(), 0.4, 0.1),
+    ((300, 200), M.Reflecting(), np.float32(0.25), np.float32(0.03)),
+    ((300, 200), M.Reflecting(), 0.25, np.float32(0.03)),
+]
+
+
+@pytest.mark.parametrize("i", range(len(FOREST)))
+@pytest.mark.parametrize("vb", [0, 2, 4])
+def test_forest_band_walk(oracle, i, vb):
+    dims, b, p, f = FOREST[i]
+    u = rng_state(i, dims, [0, 1, 1, 1, 2])
+    prob = forest_problem(dims, p=p, f=f, seed=7, boundary=b, u0=u, tspan=(5, 29))
+    be = HIPBackend(tune=(vb, 16) if vb else None)
+    sol = solve(prob, M.Synchronous(), backend=be, saveat=6)
+    assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t))
+
+
+def test_forest_2048_from_empty(oracle, backend):
+    dims = (2048, 1536)
+    prob = forest_problem(dims, p=0.03, f=1e-4, seed=7, u0=np.zeros(dims, np.int8), tspan=60)
+    sol = solve(prob, M.Synchronous(), backend=backend, saveat=30)
+    assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t))
+    assert (sol.u[-1] == 2).any()  # fires have started
+
+
+# ------------------------------------------------------------------ Lenia
+
+def _lenia_ref_steps(oracle, prob, u, n):
+    d, keep = desc_for(prob, field_method=0)
+    return oracle.advance_plain(d, u, 0, n)
+
+
+def test_lenia_sequential_fp64(oracle):
+    # FP64 field in tap order on the GPU: agrees with the FP64 oracle to the
+    # rounding of exp (device vs libm) and the final Float32 rounding
+    rows = np.array(__import__("helpers").golden("reference_vectors.json")["orbium"]["rows"])
+    u = np.zeros((64, 64), np.float32)
+    u[19:39, 19:39] = rows
+    prob = CAProblem(M.Lenia(), u, 20)
+    sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field="seq"), saveat=1)
+    for t in range(20):
+        ref = _lenia_ref_steps(oracle, prob, sol.u[t], 1)
+        assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() <= 2 ** -23
+
+
+def test_lenia_f32_field_per_step_error(oracle):
+    # FP32 FMAs over the dense table: per-step max-abs error vs the FP64 oracle <= 1e-4
+    rng = np.random.default_rng(2)
+    u = np.zeros((96, 80), np.float32)
+    u[30:62, 20:52] = rng.random((32, 32)).astype(np.float32)
+    prob = CAProblem(M.Lenia(), u, 10)
+    sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field="f32"), saveat=1)
+    for t in range(10):
+        ref = _lenia_ref_steps(oracle, prob, sol.u[t], 1)
+        assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() < 1e-4
+
+
+@pytest.mark.parametrize("scheme", [M.Stochastic(0.5), M.Checkerboard()])
+def test_lenia_schemes_and_edges(oracle, scheme):
+    rng = np.random.default_rng(3)
+    u = np.asfortranarray(rng.random((40, 36)).astype(np.float32))
+    space = M.Lattice((40, 36), (M.Reflecting(), M.Constant(0.25)))
+    prob = CAProblem(M.Lenia(R=5), u, 4, space=space, seed=9)
+    sol = solve(prob, scheme, backend=HIPBackend(lenia_field="seq"), saveat=1)
+    d, keep = desc_for(prob, scheme, field_method=0)
+    for t in range(4):
+        ref = oracle.advance_plain(d, sol.u[t], t, 1)
+        assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() <= 2 ** -23
+
+
+def test_lenia_float32_precision(oracle):
+    rng = np.random.default_rng(4)
+    u = np.asfortranarray(rng.random((48, 48)).astype(np.float32))
+    prob = CAProblem(M.Lenia(precision=32), u, 5)
+    for field in ("seq", "f32"):
+        sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field=field), saveat=1)
+        for t in range(5):
+            ref = _lenia_ref_steps(oracle, prob, sol.u[t], 1)
+            assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() < 1e-4
+
+
+# ------------------------------------------------------------------ interface
+
+def test_cells_and_device_copies(oracle, backend):
+    hip = ctypes.CDLL("libamdhip64.so")
+    dims = (70, 40)
+    u = rng_state(8, dims, [0, 1])
+    prob = CAProblem(M.Life(), u, 1)
+    st = backend.allocate_state(prob, M.Synchronous())
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    try:
+        assert backend.read_cell(st, (3, 5)) == u[3, 5]
+        backend.write_cell(st, (3, 5), 1 - u[3, 5])
+        assert backend.read_cell(st, (3, 5)) == 1 - u[3, 5]
+        with pytest.raises(M.ArgumentError):
+            backend.read_cell(st, (70, 0))
+        with pytest.raises(M.StateShapeError):
+            backend.set_state(st, np.zeros((10, 10), np.int8))
+        # device-to-device copies in and out of dense column-major device arrays
+        assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(u.size)) == 0
+        assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(u.size)) == 0
+        v = np.asfortranarray(rng_state(9, dims, [0, 1]))
+        assert hip.hipMemcpy(src, ctypes.c_void_p(v.ctypes.data), ctypes.c_size_t(u.size), 1) == 0
+        from cahip.backend import check
+        check(backend.lib.cah_state_set_device(st.handle, src, u.size))
+        assert np.array_equal(backend.current_state(st), v)
+        check(backend.lib.cah_state_get_device(st.handle, dst, u.size))
+        w = np.empty(dims, np.int8, order="F")
+        assert hip.hipMemcpy(ctypes.c_void_p(w.ctypes.data), dst, ctypes.c_size_t(u.size), 2) == 0
+        assert np.array_equal(w, v)
+    finally:
+        backend.free_state(st)
+        for p in (src, dst):
+            if p.value:
+                hip.hipFree(p)
+
+
+def test_resume_is_exact(oracle, backend):
+    # exact resume from (u, t, seed, p): stop at t, restore, continue
+    dims = (200, 120)
+    prob = forest_problem(dims, p=0.2, f=0.01, u0=rng_state(2, dims, [0, 1, 2]), tspan=40)
+    whole = solve(prob, backend=backend, save_everystep=False).u[-1]
+    first = solve(prob.remake(tspan=(0, 17)), backend=backend, save_everystep=False).u[-1]
+    rest = solve(prob.remake(u0=first, tspan=(17, 40)), backend=backend, save_everystep=False).u[-1]
+    assert np.array_equal(whole, rest)
+
+
+def test_batch_equals_single_runs(oracle, backend):
+    from cahip.problem import EnsembleProblem, solve_ensemble
+    base = forest_problem((96, 80), p=0.1, f=0.01, seed=0, u0=np.zeros((96, 80), np.int8), tspan=30)
+    ep = EnsembleProblem(base)
+    batched = solve_ensemble(ep, backend=backend, trajectories=5, batched=True)
+    single = solve_ensemble(ep, backend=backend, trajectories=5, batched=False)
+    for a, b in zip(batched, single):
+        assert np.array_equal(a, b)
+    assert len({a.tobytes() for a in batched}) == 5

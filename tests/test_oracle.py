@@ -1,0 +1,362 @@
+"""The CPU oracle pinned against the reference's known answers, naive
+definitions, closed forms and invariants (no GPU)."""
+import numpy as np
+import pytest
+
+from cahip import model as M
+from cahip.problem import CAProblem
+from helpers import (brute_life_torus, desc_for, forest_problem, golden, ltl_rule,
+                     naive_generations, params_for, rng_state)
+
+G = golden("reference_vectors.json")
+
+
+# ------------------------------------------------------------------ Philox and draws
+
+def test_philox_known_answers(oracle):
+    for c in G["philox4x32"]["cases"]:
+        assert oracle.philox4x32(c["ctr"], c["key"]) == tuple(c["out"])
+
+
+def _py_philox(ctr, key):
+    # an independent big-integer evaluation, pinned by the same known answers
+    c = list(ctr)
+    k = list(key)
+    for rnd in range(10):
+        p0, p1 = 0xD2511F53 * c[0], 0xCD9E8D57 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k[0]) & 0xFFFFFFFF, p1 & 0xFFFFFFFF,
+             ((p0 >> 32) ^ c[3] ^ k[1]) & 0xFFFFFFFF, p0 & 0xFFFFFFFF]
+        k = [(k[0] + 0x9E3779B9) & 0xFFFFFFFF, (k[1] + 0xBB67AE85) & 0xFFFFFFFF]
+    return tuple(c)
+
+
+def test_python_philox_is_pinned_too():
+    for c in G["philox4x32"]["cases"]:
+        assert _py_philox(c["ctr"], c["key"]) == tuple(c["out"])
+
+
+def test_uniform_closed_form(oracle):
+    # _philox_draw / _uniform / _uniform32 (src/random.jl:102-120)
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        seed = int(rng.integers(0, 2**64, dtype=np.uint64))
+        cell = int(rng.integers(1, 2**40))
+        t = int(rng.integers(0, 2**33))
+        k = int(rng.integers(0, 2**32))
+        r = _py_philox((cell & 0xFFFFFFFF, cell >> 32, t & 0xFFFFFFFF, k),
+                       (seed & 0xFFFFFFFF, seed >> 32))
+        u = (((r[0] << 32) | r[1]) >> 11) * 2.0**-53
+        assert oracle.uniform(seed, cell, t, k) == u
+        assert oracle.uniform32(seed, cell, t, k) == np.float32((r[0] >> 8) * 2.0**-24)
+
+
+def test_uniform_invariants(oracle):
+    # test/test_random.jl:18-56
+    u = oracle.uniform
+    assert u(1, 7, 3, 1) == u(1, 7, 3, 1)
+    assert u(1, 7, 3, 1) != u(1, 8, 3, 1)
+    assert u(1, 7, 3, 1) != u(1, 7, 4, 1)
+    assert u(1, 7, 3, 1) != u(2, 7, 3, 1)
+    assert u(1, 7, 3, 1) != u(1, 7, 3, 2)
+    xs = np.array([[u(2026, i, 0, k) for k in range(1, 5)] for i in range(1, 5001)])
+    assert ((xs >= 0) & (xs < 1)).all()
+    assert abs(xs.mean() - 0.5) < 0.01
+    assert abs((xs < 0.25).mean() - 0.25) < 0.01
+    a, b = xs[:, 0], xs[:, 1]
+    assert abs(np.mean((a - 0.5) * (b - 0.5)) / (1 / 12)) < 0.05
+    # the 24-bit draw is the top 24 bits of the 53-bit one
+    for i in range(1, 2000):
+        x, y = u(9, i, 2, 1), oracle.uniform32(9, i, 2, 1)
+        assert np.float32(np.floor(x * 2**24) / 2**24) == y
+    hits = sum(oracle.uniform32(5, i, 1, 1) < np.float32(0.3) for i in range(1, 20001))
+    assert abs(hits / 20000 - 0.3) < 0.015
+
+
+# ------------------------------------------------------------------ spaces, offsets
+
+def test_map_axis_vectors(oracle):
+    g = G["map_axis"]
+    assert [oracle.map_axis(0, j, 4)[0] for j in range(-1, 7)] == g["periodic_n4_j_from_-1"]
+    assert [oracle.map_axis(1, j, 4)[0] for j in range(-2, 8)] == g["reflecting_n4_j_from_-2"]
+    for c in g["constant"]:
+        assert list(oracle.map_axis(2, c["j"], c["n"])) == c["out"]
+    # the host mirror agrees
+    assert [M.map_axis(M.Reflecting(), j, 4)[0] for j in range(-2, 8)] == g["reflecting_n4_j_from_-2"]
+
+
+def test_boundary_reads(oracle):
+    # test/test_spaces.jl:94-102 through a 1-step probe: a rule-free check uses
+    # update_cell of a Life rule whose neighbourhood is one offset, so the new
+    # state reveals whether the neighbour read equals 1.
+    u = np.arange(1, 13, dtype=np.int8).reshape((3, 4), order="F")
+    for ent in G["boundary_read"]["grid_reflecting_constant_minus1"]:
+        k1, k2 = ent["K"]
+        v = _read_via_offsets(oracle, u, (M.Reflecting(), M.Constant(-1)), k1 - 1, k2 - 1)
+        assert v == ent["value"]
+    z = np.zeros((3, 3), dtype=np.int8)
+    for ent in G["boundary_read"]["grid_constant7_constant9_zeros"]:
+        k1, k2 = ent["K"]
+        v = _read_via_offsets(oracle, z, (M.Constant(7), M.Constant(9)), k1 - 1, k2 - 1)
+        assert v == ent["value"]
+
+
+def _read_via_offsets(oracle, u, bounds, k1, k2):
+    # Generations with C large enough that "c + 1" reveals the neighbour: a
+    # Life-family rule counts neighbours equal to 1 only, so instead probe every
+    # value v with a one-offset neighbourhood and a one-cell lattice trick is
+    # heavy; read the value with the forest-fire rule's "neighbour == 2" test
+    # after shifting values so that the probed value maps to 2.
+    for v in range(-128, 128):
+        shifted = u.astype(np.int16) - v + 2
+        if shifted.min() < -128 or shifted.max() > 127:
+            continue
+        cv = [b.value - v + 2 if isinstance(b, M.Constant) else 0 for b in bounds]
+        if any(not -128 <= c <= 127 for c in cv):
+            continue
+        bs = tuple(M.Constant(c) if isinstance(b, M.Constant) else b for b, c in zip(bounds, cv))
+        space = M.Lattice(u.shape, bs)
+        # probe from cell (0, 0) with the offset (k1, k2)
+        uu = shifted.astype(np.int8)
+        uu = np.asfortranarray(uu)
+        prob = CAProblem(M.ForestFire(neighbourhood=M.Offsets((k1, k2))),
+                         np.zeros(u.shape, np.int8), 1, {"p": 0.0, "f": 0.0}, space=space)
+        d, keep = desc_for(prob)
+        probe = uu.copy()
+        probe[0, 0] = 1  # a tree: becomes fire iff its neighbour == 2
+        if k1 == 0 and k2 == 0:
+            continue
+        res = oracle.update_cell(d, probe, 0, 0, 0, params_for(prob))
+        if res == 2:
+            return v
+    return None
+
+
+def test_offsets_counts_and_orders(oracle):
+    for kind, r, nd, n in G["neighbourhood_counts"]["cases"]:
+        k = 0 if kind == "Moore" else 1
+        assert len(oracle.offsets(k, nd, r)) == n
+    o = G["neighbourhood_orders"]
+    assert [list(x) for x in oracle.offsets(1, 2, 1)] == o["VonNeumann1_2d"]
+    assert [[x[0]] for x in oracle.offsets(0, 1, 2)] == o["Moore2_1d"]
+    # host mirror agrees with the oracle in order
+    for nb, k in ((M.Moore(3), 0), (M.VonNeumann(2), 1), (M.Moore(5), 0)):
+        host = M.neighbourhood_offsets(nb, M.Torus(20, 20)).offsets
+        assert list(host) == [tuple(x) for x in oracle.offsets(k, 2, nb.radius)]
+    assert M.halo(M.neighbourhood_offsets(M.Moore(3), M.Torus(9, 9)).offsets) == o["halo_Moore3"]
+
+
+def test_desc_layout_matches_ctypes(oracle):
+    import ctypes
+    from cahip import _lib as L
+    lay = oracle.desc_layout()
+    names = [f[0] for f in L.CahDesc._fields_]
+    assert lay[0] == ctypes.sizeof(L.CahDesc)
+    assert lay[1:1 + len(names)] == [getattr(L.CahDesc, n).offset for n in names]
+    pn = [f[0] for f in L.CahParams._fields_]
+    rest = lay[1 + len(names):]
+    assert rest[0] == ctypes.sizeof(L.CahParams)
+    assert rest[1:] == [getattr(L.CahParams, n).offset for n in pn]
+
+
+# ------------------------------------------------------------------ rules
+
+@pytest.mark.parametrize("rule,b,s,C", [("B3/S23", [3], [2, 3], 2), ("highlife", [3, 6], [2, 3], 2),
+                                        ("seeds", [2], [], 2), ("brians_brain", [2], [], 3),
+                                        ("star_wars", [2], [3, 4, 5], 4),
+                                        ("B3/S23/C2", [3], [2, 3], 2)])
+def test_life_generations_naive(oracle, rule, b, s, C):
+    # test/test_rules.jl:75-87
+    r = M.Generations(rule) if ("C" in rule or rule in M.GENERATIONS_PRESETS) else M.Life(rule)
+    for seed in (1, 2, 3):
+        u = rng_state(seed, (12, 15), range(C))
+        prob = CAProblem(r, u, 1)
+        d, keep = desc_for(prob)
+        got = oracle.advance_plain(d, u, 0, 1)
+        assert (got == naive_generations(b, s, C, u)).all()
+
+
+def test_classic_life_patterns(oracle):
+    # test/test_rules.jl:109-122
+    def run(u, n):
+        prob = CAProblem(M.Life(), u, n)
+        d, keep = desc_for(prob)
+        return oracle.advance_plain(d, u, 0, n)
+    blinker = np.zeros((5, 5), np.int8)
+    blinker[2, 1:4] = 1
+    assert (run(blinker, 1) == blinker.T).all()
+    assert (run(blinker, 2) == blinker).all()
+    block = np.zeros((4, 4), np.int8)
+    block[1:3, 1:3] = 1
+    assert (run(block, 1) == block).all()
+    glider = np.array([[0, 1, 0], [0, 0, 1], [1, 1, 1]], np.int8)  # "bo$2bo$3o!"
+    u = np.zeros((10, 10), np.int8)
+    u[1:4, 1:4] = glider
+    assert (run(u, 4) == np.roll(u, (1, 1), axis=(0, 1))).all()
+    assert (run(u, 40) == u).all()
+
+
+def test_larger_than_life_brute_force(oracle):
+    rule = ltl_rule()
+    assert rule.rulestring() == G["rulestrings"]["ltl"]["rulestring"]
+    for seed, dims in ((1, (23, 19)), (2, (16, 16)), (3, (11, 30))):
+        u = rng_state(seed, dims, [0, 1])
+        prob = CAProblem(rule, u, 1)
+        d, keep = desc_for(prob)
+        got = oracle.advance_plain(d, u, 0, 1)
+        assert (got == brute_life_torus(u, range(34, 46), range(34, 59), 5)).all()
+
+
+def test_forest_fire_invariants(oracle):
+    dims = (40, 30)
+    u = rng_state(4, dims, [0, 1, 2])
+    prob = forest_problem(dims, u0=u)
+    d, keep = desc_for(prob)
+    out = oracle.advance_plain(d, u, 0, 1, params_for(prob))
+    assert (out[u == 2] == 0).all()                      # fire burns out
+    assert set(np.unique(out[u == 0])) <= {0, 1}         # empty ground only grows trees
+    # a tree next to a fire catches fire
+    f = (u == 2)
+    nb = np.zeros_like(f)
+    nb[1:, :] |= f[:-1, :]
+    nb[:-1, :] |= f[1:, :]
+    nb[:, 1:] |= f[:, :-1]
+    nb[:, :-1] |= f[:, 1:]
+    assert (out[(u == 1) & nb] == 2).all()
+    # p = 0 keeps an empty lattice empty; f = 0 never ignites a fire-free forest
+    z = np.zeros(dims, np.int8)
+    p0 = forest_problem(dims, p=0.0, f=0.5, u0=z)
+    dz, k2 = desc_for(p0)
+    assert (oracle.advance_plain(dz, z, 0, 5, params_for(p0)) == 0).all()
+    trees = np.ones(dims, np.int8)
+    f0 = forest_problem(dims, p=0.5, f=0.0, u0=trees)
+    dt_, k3 = desc_for(f0)
+    assert (oracle.advance_plain(dt_, trees, 0, 5, params_for(f0)) == 1).all()
+    # empty -> tree at rate p
+    big = (200, 200)
+    pz = forest_problem(big, p=0.3, f=0.0, u0=np.zeros(big, np.int8))
+    db, k4 = desc_for(pz)
+    grown = oracle.advance_plain(db, np.zeros(big, np.int8), 3, 1, params_for(pz))
+    assert abs(grown.mean() - 0.3) < 0.01
+
+
+# ------------------------------------------------------------------ engines agree
+
+CASES = [
+    ("life", lambda: (M.Life(), M.Torus(17, 13), [0, 1])),
+    ("ltl_grid", lambda: (ltl_rule(), M.Grid(19, 23, boundary=M.Constant(1)), [0, 1])),
+    ("gen_reflect", lambda: (M.Generations("star_wars"), M.Grid(21, 9, boundary=M.Reflecting()),
+                             [0, 1, 2, 3])),
+    ("gen_mixed", lambda: (M.Generations("brians_brain"),
+                           M.Lattice((14, 11), (M.Periodic(), M.Constant(2))), [0, 1, 2])),
+    ("life_vn2", lambda: (M.Life("B2/S13", neighbourhood=M.VonNeumann(2)), M.Torus(15, 16), [0, 1])),
+    ("life_1d", lambda: (M.Life("B1/S1", neighbourhood=M.Moore(2)), M.Torus(31), [0, 1])),
+    ("forest", lambda: (M.ForestFire(), M.Grid(25, 18, boundary=M.Reflecting()), [0, 1, 2])),
+]
+SCHEMES = [M.Synchronous(), M.Stochastic(0.6), M.Stochastic(np.float32(0.4)), M.Checkerboard()]
+
+
+@pytest.mark.parametrize("case", [c[0] for c in CASES])
+@pytest.mark.parametrize("si", range(len(SCHEMES)))
+def test_halo_engine_equals_plain(oracle, case, si):
+    rule, space, vals = dict(CASES)[case]()
+    u = rng_state(11, space.dims, vals)
+    p = {"p": 0.2, "f": 0.05} if isinstance(rule, M.ForestFire) else None
+    mask = None
+    if si == 1:
+        mask = np.ones(space.dims, bool)
+        mask.flat[::7] = False
+    prob = CAProblem(rule, u, 5, p, space=space, seed=3, mask=mask)
+    d, keep = desc_for(prob, SCHEMES[si])
+    a = oracle.advance_plain(d, u, 2, 5, params_for(prob))
+    b = oracle.advance_halo(d, u, 2, 5, params_for(prob), nthreads=3)
+    assert (a == b).all()
+
+
+def test_update_cell_is_one_step(oracle):
+    # test/test_backend_interface.jl:40-68
+    dims = (12, 10)
+    u = rng_state(3, dims, [0, 1, 2])
+    mask = np.ones(dims, bool)
+    mask[:3, :] = False
+    prob = CAProblem(M.ForestFire(), u, 8, {"p": 0.5, "f": 0.2}, seed=4, mask=mask,
+                     space=M.Lattice(dims, (M.Constant(1), M.Reflecting())))
+    d, keep = desc_for(prob)
+    for t in (0, 3):
+        step = oracle.advance_plain(d, u, t, 1, params_for(prob))
+        cells = np.array([[oracle.update_cell(d, u, i, j, t, params_for(prob))
+                           for j in range(dims[1])] for i in range(dims[0])], dtype=np.int8)
+        assert (cells == step).all()
+    assert oracle.update_cell(d, u, 1, 4, 0, params_for(prob)) == u[1, 4]  # masked
+
+
+# ------------------------------------------------------------------ Lenia
+
+def test_lenia_kernel(oracle):
+    offs, w = oracle.lenia_kernel(13)
+    assert len(offs) == 516 and (0, 0) not in offs
+    assert abs(w.sum() - 1) < 1e-14 and (w > 0).all()
+    assert max(max(abs(a), abs(b)) for a, b in offs) == 12
+    wd = dict(zip(offs, w))
+    for (a, b), x in wd.items():
+        assert abs(wd[(-a, b)] - x) < 1e-16 and abs(wd[(b, a)] - x) < 1e-16
+    ho, hw = M.kernel_weights(M.LeniaKernel(13), M.Torus(64, 64))
+    assert ho == offs and np.allclose(hw, w, rtol=1e-14, atol=0)
+    o2, w2 = oracle.lenia_kernel(6, (0.5, 1.0))
+    assert abs(w2.sum() - 1) < 1e-14
+
+
+def _lenia_problem(u, **kw):
+    return CAProblem(M.Lenia(**kw), u, 1)
+
+
+def test_lenia_fft_equals_direct(oracle):
+    from oracle.lenia_fft import LeniaFFT
+    rng = np.random.default_rng(1)
+    u = np.asfortranarray(rng.random((48, 40)).astype(np.float32))
+    prob = _lenia_problem(u)
+    d, keep = desc_for(prob, field_method=0)
+    taps, w = prob.taps
+    lf = LeniaFFT(u.shape, taps, w, prob.rule.mu, prob.rule.sigma, prob.rule.dt)
+    assert np.abs(oracle.lenia_field(d, u) - lf.field(u)).max() < 1e-12
+    # 10 steps agree to rounding (test/test_extras.jl:164-166)
+    a = oracle.advance_plain(d, u, 0, 10)
+    b = u
+    for _ in range(10):
+        b = lf.step(b)
+    assert np.abs(a.astype(np.float64) - b).max() < 1e-5
+    # a constant state has a constant field equal to it (weights sum to one)
+    c = np.full((32, 32), 0.37, np.float32)
+    pc = _lenia_problem(c)
+    dc, kc = desc_for(pc, field_method=0)
+    assert np.abs(oracle.lenia_field(dc, c) - np.float32(0.37)).max() < 1e-12
+
+
+def test_orbium_glides(oracle):
+    # test/test_extras.jl:156-163: mass within 15% and a move of more than 3 cells
+    from oracle.lenia_fft import LeniaFFT
+    rows = np.array(G["orbium"]["rows"], dtype=np.float64)
+    u = np.zeros((64, 64), np.float32)
+    u[19:39, 19:39] = rows.astype(np.float32)
+    prob = _lenia_problem(u)
+    taps, w = prob.taps
+    lf = LeniaFFT(u.shape, taps, w, prob.rule.mu, prob.rule.sigma, prob.rule.dt)
+    snaps = [u]
+    x = u
+    for k in range(200):
+        x = lf.step(x)
+        if k in (99, 199):
+            snaps.append(x)
+    mass = [s.astype(np.float64).sum() for s in snaps]
+    assert all(abs(m - mass[0]) / mass[0] < 0.15 for m in mass)
+    def centre(s):
+        s = s.astype(np.float64)
+        return (np.arange(1, 65)[:, None] * s).sum() / s.sum()
+    assert abs(centre(snaps[1]) - centre(snaps[0])) > 3
+
+
+def test_generator(oracle):
+    u = oracle.fill_bernoulli(1, (512, 512), 0.5)
+    assert abs(u.mean() - 0.5) < 0.01 and set(np.unique(u)) == {0, 1}
+    f = oracle.fill_uniform_box(3, (64, 48), 10, 20, 5, 30)
+    assert (f[:10] == 0).all() and (f[30:] == 0).all() and (f[:, :5] == 0).all()
+    assert (f[10:30, 5:35] > 0).mean() > 0.99
