@@ -682,35 +682,132 @@ extern "C" int cah_state_fill_uniform_box(cah_state* s, uint64_t seed, int64_t l
 }
 
 // ---------------------------------------------------------------- batches
+//
+// A batch owns one state per trajectory (its buffers, pitch and rule) and
+// advances all of them with one launch per step on the batch's stream: the
+// trajectories' band-walk waves are concatenated into one flat wave list.
+// Trajectories that no band-walk kernel runs advance one after another.
+
+enum BatchKind { B_LOOP, B_FOREST, B_LIFE };
 
 struct cah_batch {
     int device = 0;
-    std::vector<cah_state*> members;
     hipStream_t stream = nullptr;
+    std::vector<cah_state*> members;
+    BatchKind kind = B_LOOP;
+    int vb = 1, life_R = 0;
+    bool small = true;
+    int parity = 0;
+    int32_t nwaves = 0;
+    void* items_d = nullptr;
+    int32_t* wave_item_d = nullptr;
+    int32_t* wave_base_d = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
+
+static void free_batch(cah_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    for (void* p : {b->items_d, (void*)b->wave_item_d, (void*)b->wave_base_d})
+        if (p) hipFree(p);
+    for (auto* m : b->members) free_state(m);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// Wave layout of a member: rows per wave fixed for the whole batch so that
+// every wave carries about the same work.
+constexpr int32_t BATCH_ROWS = 64;
+
+static int batch_layout(cah_batch* b) {
+    std::vector<int32_t> wave_item, wave_base;
+    std::vector<ForestArgs> fitems;
+    std::vector<LifeArgs> litems;
+    const int64_t ow = 62 * 4 * b->vb;
+    for (size_t k = 0; k < b->members.size(); ++k) {
+        cah_state* s = b->members[k];
+        const int32_t nbands = (int32_t)((s->n1 + ow - 1) / ow);
+        const int32_t rows = (int32_t)std::min<int64_t>(BATCH_ROWS, s->n2);
+        const int32_t nw = nbands * (int32_t)((s->n2 + rows - 1) / rows);
+        wave_base.push_back((int32_t)wave_item.size());
+        for (int32_t w = 0; w < nw; ++w) wave_item.push_back((int32_t)k);
+        if (b->kind == B_FOREST) {
+            ForestArgs a{};
+            a.cur = (const int8_t*)s->buf[0];
+            a.next = (int8_t*)s->buf[1];
+            a.L = lat_of(s);
+            a.cv1 = (int32_t)s->d.boundary_value[0];
+            a.cv2 = (int32_t)s->d.boundary_value[1];
+            a.nbands = nbands;
+            a.rows = rows;
+            a.nwaves = nw;
+            a.seed = s->d.seed;
+            fitems.push_back(a);
+        } else {
+            LifeArgs a = s->life;
+            a.cur = (const int8_t*)s->buf[0];
+            a.next = (int8_t*)s->buf[1];
+            a.L = lat_of(s);
+            a.cv1 = (int32_t)s->d.boundary_value[0];
+            a.cv2 = (int32_t)s->d.boundary_value[1];
+            a.nbands = nbands;
+            a.rows = rows;
+            a.nwaves = nw;
+            litems.push_back(a);
+        }
+    }
+    b->nwaves = (int32_t)wave_item.size();
+    if (b->kind == B_FOREST) {
+        if (int rc = upload((ForestArgs**)&b->items_d, fitems.data(), fitems.size())) return rc;
+    } else {
+        if (int rc = upload((LifeArgs**)&b->items_d, litems.data(), litems.size())) return rc;
+    }
+    if (int rc = upload(&b->wave_item_d, wave_item.data(), wave_item.size())) return rc;
+    if (int rc = upload(&b->wave_base_d, wave_base.data(), wave_base.size())) return rc;
+    return CAH_OK;
+}
 
 extern "C" int cah_batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, cah_batch** out) {
     if (!ctx || !descs || !out || n < 1) return fail(CAH_ERR_ARGUMENT, "cah_batch_create: bad arguments");
     cah_batch* b = new cah_batch();
     b->device = ctx->device;
+    auto bail = [&](int rc) {
+        std::string keep = g_err;
+        free_batch(b);
+        g_err = keep;
+        return rc;
+    };
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CAH_ERR_DEVICE, "cannot create the batch stream"));
     for (int32_t i = 0; i < n; ++i) {
-        if (descs[i].rule != descs[0].rule || descs[i].dtype != CAH_DTYPE_I8) {
-            for (auto* m : b->members) free_state(m);
-            delete b;
-            return fail(CAH_ERR_ARGUMENT, "a batch holds trajectories of one Int8 rule");
-        }
+        if (descs[i].rule != descs[0].rule || descs[i].dtype != CAH_DTYPE_I8)
+            return bail(fail(CAH_ERR_ARGUMENT, "a batch holds trajectories of one Int8 rule"));
         cah_state* s = nullptr;
-        int rc = cah_state_create(ctx, &descs[i], &s);
-        if (rc) {
-            std::string keep = g_err;
-            for (auto* m : b->members) free_state(m);
-            delete b;
-            g_err = keep;
-            return rc;
-        }
+        if (int rc = cah_state_create(ctx, &descs[i], &s)) return bail(rc);
         b->members.push_back(s);
     }
-    b->stream = b->members[0]->stream;
+    bool all_forest = true, all_life = true;
+    int R = b->members[0]->life_R;
+    uint64_t cells = 0;
+    for (auto* s : b->members) {
+        all_forest &= s->path == P_FOREST;
+        all_life &= s->path == P_LIFE_SWAR && s->life_R == R;
+        cells = std::max<uint64_t>(cells, (uint64_t)(s->n1 * s->n2));
+    }
+    b->small = cells < 0xFFFFFFFFull;
+    if (all_forest) {
+        b->kind = B_FOREST;
+        b->vb = 1;
+    } else if (all_life) {
+        b->kind = B_LIFE;
+        b->life_R = R;
+        b->vb = R == 5 ? 2 : 1;
+    }
+    if (b->kind != B_LOOP)
+        if (int rc = batch_layout(b)) return bail(rc);
+    for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
     *out = b;
     return CAH_OK;
 }
@@ -725,32 +822,108 @@ static int member(cah_batch* b, int32_t i, cah_state** s) {
 extern "C" int cah_batch_set(cah_batch* b, int32_t i, const void* host, size_t nbytes) {
     cah_state* s;
     if (int rc = member(b, i, &s)) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
     return cah_state_set(s, host, nbytes);
 }
 
 extern "C" int cah_batch_get(cah_batch* b, int32_t i, void* host, size_t nbytes) {
     cah_state* s;
     if (int rc = member(b, i, &s)) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
     return cah_state_get(s, host, nbytes);
 }
 
 extern "C" int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, double p) {
     cah_state* s;
     if (int rc = member(b, i, &s)) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
     return cah_state_fill_bernoulli(s, seed, p);
+}
+
+template <int R>
+static void launch_life_batch_r(cah_batch* b, bool bin, int blocks) {
+    const LifeArgs* it = (const LifeArgs*)b->items_d;
+    if constexpr (R <= 4) {
+        if (b->vb == 1) {
+            if (bin) k_life_batch<R, 1, true><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
+            else k_life_batch<R, 1, false><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
+            return;
+        }
+    }
+    if (bin) k_life_batch<R, 2, true><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
+    else k_life_batch<R, 2, false><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
+}
+
+static void launch_batch_step(cah_batch* b, int64_t t, const cah_params* p, bool bin) {
+    const int blocks = waves_to_blocks(b->nwaves);
+    if (b->kind == B_FOREST) {
+        const ForestArgs* it = (const ForestArgs*)b->items_d;
+        if (b->small)
+            k_forest_batch<1, true><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, p->ff_p, p->ff_f, p->ff_p_is_f32, p->ff_f_is_f32);
+        else
+            k_forest_batch<1, false><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, p->ff_p, p->ff_f, p->ff_p_is_f32, p->ff_f_is_f32);
+    } else {
+        switch (b->life_R) {
+        case 1: launch_life_batch_r<1>(b, bin, blocks); break;
+        case 2: launch_life_batch_r<2>(b, bin, blocks); break;
+        case 3: launch_life_batch_r<3>(b, bin, blocks); break;
+        case 4: launch_life_batch_r<4>(b, bin, blocks); break;
+        default: launch_life_batch_r<5>(b, bin, blocks); break;
+        }
+    }
+    b->parity ^= 1;
 }
 
 extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const cah_params* p,
                                  int32_t use_graph) {
     if (!b) return fail(CAH_ERR_ARGUMENT, "batch is NULL");
-    (void)use_graph;
-    for (auto* s : b->members)
-        if (int rc = cah_advance(s, t0, nsteps, p)) return rc;
+    if (nsteps < 0 || t0 < 0) return fail(CAH_ERR_ARGUMENT, "t0 and nsteps must be >= 0");
+    HIPCHK(hipSetDevice(b->device));
+    if (b->kind == B_LOOP) {
+        for (auto* s : b->members)
+            if (int rc = cah_advance(s, t0, nsteps, p)) return rc;
+        return CAH_OK;
+    }
+    if (b->kind == B_FOREST && !p)
+        return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
+    // members may have been set through their own streams
+    for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
+    bool bin = true;
+    for (auto* s : b->members) {
+        const bool e1 = s->d.boundary[0] != CAH_BOUNDARY_CONSTANT ||
+                        s->d.boundary_value[0] == 0 || s->d.boundary_value[0] == 1;
+        const bool e2 = s->d.boundary[1] != CAH_BOUNDARY_CONSTANT ||
+                        s->d.boundary_value[1] == 0 || s->d.boundary_value[1] == 1;
+        bin &= s->bin_clean && e1 && e2;
+    }
+    if (use_graph && nsteps > 1) {
+        if (b->exec) {
+            HIPCHK(hipStreamSynchronize(b->stream));
+            HIPCHK(hipGraphExecDestroy(b->exec));
+            b->exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, p, bin);
+        HIPCHK(hipStreamEndCapture(b->stream, &graph));
+        HIPCHK(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, p, bin);
+    }
+    HIPCHK(hipGetLastError());
+    for (auto* s : b->members) {
+        s->cur = b->parity;
+        if (b->kind == B_FOREST) s->bin_clean = false;
+    }
     return CAH_OK;
 }
 
 extern "C" int cah_batch_sync(cah_batch* b) {
     if (!b) return fail(CAH_ERR_ARGUMENT, "batch is NULL");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
     for (auto* s : b->members)
         if (int rc = cah_state_sync(s)) return rc;
     return CAH_OK;
@@ -763,8 +936,6 @@ extern "C" int cah_batch_stream(cah_batch* b, void** stream) {
 }
 
 extern "C" int cah_batch_destroy(cah_batch* b) {
-    if (!b) return CAH_OK;
-    for (auto* s : b->members) free_state(s);
-    delete b;
+    free_batch(b);
     return CAH_OK;
 }

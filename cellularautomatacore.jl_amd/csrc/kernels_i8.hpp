@@ -208,14 +208,13 @@ CAH_DEV void hsum(const uint32_t (&E)[VB + 2 * G], uint32_t (&W)[VB]) {
     }
 }
 
+// One wave's work: band `wid % nbands`, rows chunk `wid / nbands`.
 template <int R, int VB, bool BIN>
-__global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
+CAH_DEV void life_band(const LifeArgs& a, int wid) {
     constexpr int P = 2 * R + 1;
     constexpr int G = (R + 3) / 4;  // guard dwords needed on each side
     static_assert(G <= VB, "a lane must hold the guard dwords of its neighbours");
     constexpr int OW = 62 * 4 * VB;  // output bytes per band
-    const int wid = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    if (wid >= a.nwaves) return;  // whole waves only
     const int lane = threadIdx.x & 63;
     const int band = wid % a.nbands, chunk = wid / a.nbands;
     const int32_t ys = chunk * a.rows;
@@ -288,6 +287,13 @@ __global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
     }
 }
 
+template <int R, int VB, bool BIN>
+__global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
+    const int wid = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (wid >= a.nwaves) return;  // whole waves only
+    life_band<R, VB, BIN>(a, wid);
+}
+
 // ---------------------------------------------------------------- forest fire, VN(1)
 
 struct ForestArgs {
@@ -328,10 +334,8 @@ CAH_DEV uint32_t forest_cell(uint32_t c, uint32_t nf, uint64_t cellid, const For
 }
 
 template <int VB, bool SMALL>
-__global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
+CAH_DEV void forest_band(const ForestArgs& a, int wid) {
     constexpr int OW = 62 * 4 * VB;
-    const int wid = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    if (wid >= a.nwaves) return;
     const int lane = threadIdx.x & 63;
     const int band = wid % a.nbands, chunk = wid / a.nbands;
     const int32_t ys = chunk * a.rows;
@@ -390,6 +394,66 @@ __global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
             }
         });
     }
+}
+
+template <int VB, bool SMALL>
+__global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
+    const int wid = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (wid >= a.nwaves) return;
+    forest_band<VB, SMALL>(a, wid);
+}
+
+// ---------------------------------------------------------------- batched ensembles
+//
+// One launch advances every trajectory of a batch by one step.  The waves of all
+// trajectories form one flat list: wave w belongs to item wave_item[w] and is
+// that item's wave (w - wave_base[item]).  Each item is the trajectory's own
+// argument block (buffers for parity 0, lattice, seed, band layout); the step's
+// time, parameters and buffer parity come from the launch.
+
+CAH_DEV int wave_id() {
+    return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+}
+
+template <int VB, bool SMALL>
+__global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restrict__ items,
+                                                      const int32_t* __restrict__ wave_item,
+                                                      const int32_t* __restrict__ wave_base,
+                                                      int32_t nwaves, int32_t parity, int64_t t,
+                                                      double p, double f, int32_t p_f32,
+                                                      int32_t f_f32) {
+    const int wid = wave_id();
+    if (wid >= nwaves) return;
+    const int k = __builtin_amdgcn_readfirstlane(wave_item[wid]);
+    ForestArgs a = items[k];
+    if (parity) {
+        const int8_t* c = a.cur;
+        a.cur = a.next;
+        a.next = const_cast<int8_t*>(c);
+    }
+    a.t = t;
+    a.p = p;
+    a.f = f;
+    a.p_f32 = p_f32;
+    a.f_f32 = f_f32;
+    forest_band<VB, SMALL>(a, wid - wave_base[k]);
+}
+
+template <int R, int VB, bool BIN>
+__global__ __launch_bounds__(256) void k_life_batch(const LifeArgs* __restrict__ items,
+                                                    const int32_t* __restrict__ wave_item,
+                                                    const int32_t* __restrict__ wave_base,
+                                                    int32_t nwaves, int32_t parity) {
+    const int wid = wave_id();
+    if (wid >= nwaves) return;
+    const int k = __builtin_amdgcn_readfirstlane(wave_item[wid]);
+    LifeArgs a = items[k];
+    if (parity) {
+        const int8_t* c = a.cur;
+        a.cur = a.next;
+        a.next = const_cast<int8_t*>(c);
+    }
+    life_band<R, VB, BIN>(a, wid - wave_base[k]);
 }
 
 // ---------------------------------------------------------------- synthetic inputs
