@@ -125,13 +125,15 @@ def main():
     ev.sync()
 
     K = args.steps
-    a1, b1, a2, b2 = ev.event(), ev.event(), ev.event(), ev.event()
+    # each lattice timed on its own (the other idle), with HIP events recorded on
+    # the state's own stream around its K launches
     str_ltl, str_for = be.stream(s_ltl), be.stream(s_for)
-    ev.sync()
+    a1, b1, a2, b2 = ev.event(), ev.event(), ev.event(), ev.event()
     t_wall = time.perf_counter()
     ev.record(a1, str_ltl)
     run(s_ltl, pl, args.warmup, K)
     ev.record(b1, str_ltl)
+    ev.sync()
     ev.record(a2, str_for)
     run(s_for, pf, args.warmup, K)
     ev.record(b2, str_for)
@@ -141,7 +143,7 @@ def main():
     ms_for = ev.elapsed_ms(a2, b2) / K
 
     c_ltl, c_for = 8192 * 8192, 16384 * 16384
-    ms_step = wall * 1e3 / K
+    ms_step = ms_ltl + ms_for
     value = (c_ltl + c_for) / (ms_step * 1e-3) / 1e9
     # dominant kernel: the forest fire sweep (one launch per step)
     bytes_per_launch = 2 * c_for
@@ -154,7 +156,8 @@ def main():
     line = {
         "metric": "GCUPS + % HBM roofline (synchronous sweep)",
         "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": K,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "wall_ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int8", "data": "synthetic",
         "config": {"workload": "LtL Moore(5) B34..45/S34..58 Torus 8192^2 Bernoulli(0.5) seed 1 "
                                "+ forest fire VN(1) Reflecting 16384^2 all-empty p=0.03 f=1e-4 seed 7",

@@ -19,8 +19,34 @@
 
 #include "../../include/cahip.h"
 #include "device.hpp"
-#include "kernels_i8.hpp"
+#include "kernels_generic.hpp"
+#include "launch.hpp"
 #include "kernels_lenia.hpp"
+
+// ---------------------------------------------------------------- Life dispatch
+
+inline void launch_life(int R, int vb, bool bin, int niv, int blocks, const LifeArgs& a,
+                        hipStream_t st) {
+    switch (R) {
+    case 1: launch_life_r<1>(vb, bin, niv, blocks, a, st); break;
+    case 2: launch_life_r<2>(vb, bin, niv, blocks, a, st); break;
+    case 3: launch_life_r<3>(vb, bin, niv, blocks, a, st); break;
+    case 4: launch_life_r<4>(vb, bin, niv, blocks, a, st); break;
+    default: launch_life_r<5>(vb, bin, niv, blocks, a, st); break;
+    }
+}
+
+inline void launch_life_batch(int R, int vb, bool bin, int niv, int blocks, const LifeArgs* it,
+                              const int32_t* wi, const int32_t* wb, int32_t nw, int32_t parity,
+                              hipStream_t st) {
+    switch (R) {
+    case 1: launch_life_batch_r<1>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
+    case 2: launch_life_batch_r<2>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
+    case 3: launch_life_batch_r<3>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
+    case 4: launch_life_batch_r<4>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
+    default: launch_life_batch_r<5>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
+    }
+}
 
 // ---------------------------------------------------------------- errors
 
@@ -202,7 +228,14 @@ static bool is_vn1(const std::vector<int32_t>& o) {
 }
 
 // Runs of set bits of a count mask -> [lo, hi] intervals.
+// Runs of set bits of a count mask -> [lo, hi] intervals, as the byte-parallel
+// constants of k_life_swar; slots left over are neutral (lo = 128 matches no
+// count).  -1 when the mask has more than LIFE_MAX_IV runs.
 static int intervals(uint64_t m, uint32_t* lo, uint32_t* hi) {
+    for (int k = 0; k < LIFE_MAX_IV; ++k) {
+        lo[k] = 0;
+        hi[k] = 0;
+    }
     int k = 0;
     for (int c = 0; c < 64;) {
         if (!((m >> c) & 1u)) {
@@ -442,38 +475,27 @@ static int waves_to_blocks(int64_t nwaves) { return (int)((nwaves + 3) / 4); }
 
 // Rows per wave: enough waves to fill 256 CUs several times over, but at least
 // a few windows tall so the (2R)-row warm-up stays a small share of the loads.
+// Measured on MI355X: Larger-than-Life Moore(5) at 8192^2 is fastest at 48 rows
+// per wave (the 2R-row warm-up against the number of waves in flight); the
+// Philox-bound forest fire at 32.
 static int32_t band_rows(const cah_state* s, int64_t nbands, int R) {
     if (s->tune_rows > 0) return s->tune_rows;
-    const int64_t target_waves = 8192;
+    const int64_t target_waves = R == 1 ? 4096 : 8192;
     int64_t rows = (s->n2 * nbands + target_waves - 1) / target_waves;
-    rows = std::max<int64_t>(rows, 8 * R + 8);
-    rows = std::min<int64_t>(rows, 256);
+    rows = std::max<int64_t>(rows, R == 1 ? 4 : 8 * R + 8);
+    rows = std::min<int64_t>(rows, R == 1 ? 32 : 256);
     return (int32_t)std::max<int64_t>(1, std::min<int64_t>(rows, s->n2));
 }
 
-template <int R, int VB>
-static void launch_life_vb(LifeArgs a, bool bin, int blocks, hipStream_t st) {
-    if (bin) k_life_swar<R, VB, true><<<blocks, 256, 0, st>>>(a);
-    else k_life_swar<R, VB, false><<<blocks, 256, 0, st>>>(a);
-}
-
-template <int R>
-static int launch_life_r(cah_state* s, LifeArgs a, int vb) {
-    const int blocks = waves_to_blocks(a.nwaves);
-    if (R == 5 && vb < 2) vb = 2;
-    if (vb == 4) launch_life_vb<R, 4>(a, s->bin_clean, blocks, s->stream);
-    else if (vb == 2) launch_life_vb<R, 2>(a, s->bin_clean, blocks, s->stream);
-    else if constexpr (R <= 4) launch_life_vb<R, 1>(a, s->bin_clean, blocks, s->stream);
-    return CAH_OK;
-}
-
 static int life_vb(const cah_state* s) {
-    int vb = s->tune_vb ? s->tune_vb : (s->n1 >= 2048 ? 2 : 1);
+    int vb = s->tune_vb ? std::min(s->tune_vb, 2) : (s->n1 >= 2048 ? 2 : 1);
     if (s->life_R == 5 && vb < 2) vb = 2;
     return vb;
 }
 
-static int launch_life(cah_state* s) {
+static int life_niv(const LifeArgs& a) { return std::max(1, std::max(a.nb, a.ns)); }
+
+static int step_life(cah_state* s) {
     LifeArgs a = s->life;
     a.cur = (const int8_t*)s->buf[s->cur];
     a.next = (int8_t*)s->buf[1 - s->cur];
@@ -491,19 +513,12 @@ static int launch_life(cah_state* s) {
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
     a.rows = band_rows(s, a.nbands, s->life_R);
     a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
-    int rc;
-    switch (s->life_R) {
-    case 1: rc = launch_life_r<1>(s, a, vb); break;
-    case 2: rc = launch_life_r<2>(s, a, vb); break;
-    case 3: rc = launch_life_r<3>(s, a, vb); break;
-    case 4: rc = launch_life_r<4>(s, a, vb); break;
-    default: rc = launch_life_r<5>(s, a, vb); break;
-    }
+    launch_life(s->life_R, vb, s->bin_clean, life_niv(a), waves_to_blocks(a.nwaves), a, s->stream);
     s->bin_clean = saved;
-    return rc;
+    return CAH_OK;
 }
 
-static int launch_forest(cah_state* s, int64_t t, const cah_params* p) {
+static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
     ForestArgs a{};
     a.cur = (const int8_t*)s->buf[s->cur];
     a.next = (int8_t*)s->buf[1 - s->cur];
@@ -516,23 +531,14 @@ static int launch_forest(cah_state* s, int64_t t, const cah_params* p) {
     a.f = p->ff_f;
     a.p_f32 = p->ff_p_is_f32;
     a.f_f32 = p->ff_f_is_f32;
-    const int vb = s->tune_vb ? s->tune_vb : 1;
+    const int vb = s->tune_vb ? std::min(s->tune_vb, 2) : 2;
     const int64_t ow = 62 * 4 * vb;
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
     a.rows = band_rows(s, a.nbands, 1);
     a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
     const int blocks = waves_to_blocks(a.nwaves);
     const bool small = (uint64_t)(s->n1 * s->n2) < 0xFFFFFFFFull;
-    if (vb == 4) {
-        if (small) k_forest_vn1<4, true><<<blocks, 256, 0, s->stream>>>(a);
-        else k_forest_vn1<4, false><<<blocks, 256, 0, s->stream>>>(a);
-    } else if (vb == 2) {
-        if (small) k_forest_vn1<2, true><<<blocks, 256, 0, s->stream>>>(a);
-        else k_forest_vn1<2, false><<<blocks, 256, 0, s->stream>>>(a);
-    } else {
-        if (small) k_forest_vn1<1, true><<<blocks, 256, 0, s->stream>>>(a);
-        else k_forest_vn1<1, false><<<blocks, 256, 0, s->stream>>>(a);
-    }
+    launch_forest(vb, small, blocks, a, s->stream);
     return CAH_OK;
 }
 
@@ -638,8 +644,8 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
         if (s->path == P_LENIA_SEQ) launch_lenia_field_seq(s, t);
         for (int g = 0; g < ngates(s->d); ++g) {
             switch (s->path) {
-            case P_LIFE_SWAR: launch_life(s); break;
-            case P_FOREST: launch_forest(s, t, p); break;
+            case P_LIFE_SWAR: step_life(s); break;
+            case P_FOREST: step_forest(s, t, p); break;
             case P_LENIA_SEQ: launch_lenia_growth(s, t, g); break;
             case P_LENIA_F32COL: launch_lenia_f32col(s, t); break;
             default: launch_generic(s, t, g, p); break;
@@ -695,7 +701,7 @@ struct cah_batch {
     hipStream_t stream = nullptr;
     std::vector<cah_state*> members;
     BatchKind kind = B_LOOP;
-    int vb = 1, life_R = 0;
+    int vb = 1, life_R = 0, niv = 1;
     bool small = true;
     int parity = 0;
     int32_t nwaves = 0;
@@ -804,6 +810,7 @@ extern "C" int cah_batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, 
         b->kind = B_LIFE;
         b->life_R = R;
         b->vb = R == 5 ? 2 : 1;
+        for (auto* s : b->members) b->niv = std::max(b->niv, life_niv(s->life));
     }
     if (b->kind != B_LOOP)
         if (int rc = batch_layout(b)) return bail(rc);
@@ -840,37 +847,15 @@ extern "C" int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, 
     return cah_state_fill_bernoulli(s, seed, p);
 }
 
-template <int R>
-static void launch_life_batch_r(cah_batch* b, bool bin, int blocks) {
-    const LifeArgs* it = (const LifeArgs*)b->items_d;
-    if constexpr (R <= 4) {
-        if (b->vb == 1) {
-            if (bin) k_life_batch<R, 1, true><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
-            else k_life_batch<R, 1, false><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
-            return;
-        }
-    }
-    if (bin) k_life_batch<R, 2, true><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
-    else k_life_batch<R, 2, false><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity);
-}
-
 static void launch_batch_step(cah_batch* b, int64_t t, const cah_params* p, bool bin) {
     const int blocks = waves_to_blocks(b->nwaves);
-    if (b->kind == B_FOREST) {
-        const ForestArgs* it = (const ForestArgs*)b->items_d;
-        if (b->small)
-            k_forest_batch<1, true><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, p->ff_p, p->ff_f, p->ff_p_is_f32, p->ff_f_is_f32);
-        else
-            k_forest_batch<1, false><<<blocks, 256, 0, b->stream>>>(it, b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, p->ff_p, p->ff_f, p->ff_p_is_f32, p->ff_f_is_f32);
-    } else {
-        switch (b->life_R) {
-        case 1: launch_life_batch_r<1>(b, bin, blocks); break;
-        case 2: launch_life_batch_r<2>(b, bin, blocks); break;
-        case 3: launch_life_batch_r<3>(b, bin, blocks); break;
-        case 4: launch_life_batch_r<4>(b, bin, blocks); break;
-        default: launch_life_batch_r<5>(b, bin, blocks); break;
-        }
-    }
+    if (b->kind == B_FOREST)
+        launch_forest_batch(b->vb, b->small, blocks, (const ForestArgs*)b->items_d, b->wave_item_d,
+                            b->wave_base_d, b->nwaves, b->parity, t, p->ff_p, p->ff_f,
+                            p->ff_p_is_f32, p->ff_f_is_f32, b->stream);
+    else
+        launch_life_batch(b->life_R, b->vb, bin, b->niv, blocks, (const LifeArgs*)b->items_d,
+                          b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, b->stream);
     b->parity ^= 1;
 }
 

@@ -1,0 +1,49 @@
+// forest.hip -- instantiations and launchers of the forest-fire kernels.
+#include "launch.hpp"
+
+template <int VB, bool SMALL>
+static void forest_vs(int prec, int blocks, const ForestArgs& a, hipStream_t st) {
+    switch (prec) {
+    case 0: k_forest_vn1<VB, SMALL, 0><<<blocks, 256, 0, st>>>(a); break;
+    case 1: k_forest_vn1<VB, SMALL, 1><<<blocks, 256, 0, st>>>(a); break;
+    case 2: k_forest_vn1<VB, SMALL, 2><<<blocks, 256, 0, st>>>(a); break;
+    default: k_forest_vn1<VB, SMALL, 3><<<blocks, 256, 0, st>>>(a); break;
+    }
+}
+
+void launch_forest(int vb, bool small, int blocks, const ForestArgs& a, hipStream_t st) {
+    const int prec = (a.p_f32 ? 1 : 0) | (a.f_f32 ? 2 : 0);
+    if (vb == 2) {
+        if (small) forest_vs<2, true>(prec, blocks, a, st);
+        else forest_vs<2, false>(prec, blocks, a, st);
+    } else {
+        if (small) forest_vs<1, true>(prec, blocks, a, st);
+        else forest_vs<1, false>(prec, blocks, a, st);
+    }
+}
+
+template <bool SMALL, int PREC>
+static void forest_batch_go(int blocks, const ForestArgs* it, const int32_t* wi, const int32_t* wb,
+                            int32_t nw, int32_t parity, int64_t t, double p, double f,
+                            int32_t p_f32, int32_t f_f32, hipStream_t st) {
+    k_forest_batch<1, SMALL, PREC><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, t, p, f, p_f32,
+                                                           f_f32);
+}
+
+void launch_forest_batch(int vb, bool small, int blocks, const ForestArgs* it, const int32_t* wi,
+                         const int32_t* wb, int32_t nw, int32_t parity, int64_t t, double p,
+                         double f, int32_t p_f32, int32_t f_f32, hipStream_t st) {
+    (void)vb;
+    const int prec = (p_f32 ? 1 : 0) | (f_f32 ? 2 : 0);
+    auto go = [&](auto S) {
+        constexpr bool SM = decltype(S)::value;
+        switch (prec) {
+        case 0: forest_batch_go<SM, 0>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
+        case 1: forest_batch_go<SM, 1>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
+        case 2: forest_batch_go<SM, 2>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
+        default: forest_batch_go<SM, 3>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
+        }
+    };
+    if (small) go(std::true_type{});
+    else go(std::false_type{});
+}

@@ -1,0 +1,171 @@
+// kernels_forest.hpp -- the forest fire with VonNeumann(1), Synchronous, no mask:
+// the band walk of kernels_life.hpp with one Philox4x32-10 draw per cell.
+#pragma once
+#include "band.hpp"
+
+struct ForestArgs {
+    const int8_t* cur;
+    int8_t* next;
+    Lat L;
+    int32_t cv1, cv2;
+    int32_t nbands, rows, nwaves;
+    uint64_t seed;
+    int64_t t;
+    double p, f;
+    int32_t p_f32, f_f32;
+};
+
+// One forest-fire cell (README.md:56-60): c = state, nf = a VonNeumann neighbour is
+// on fire.  Exactly one Philox draw is formed per cell, k = 1 (growth) for an
+// empty cell and k = 2 (lightning) otherwise; it decides the cell only where the
+// reference rule draws it.  The FP64 test u < p with u = x * 2^-53 is made as
+// x < p * 2^53 (scaling by a power of two is exact), still in FP64.  PREC says
+// which of p (bit 0) and f (bit 1) are Float32 and use the 24-bit draw.  The
+// cell's result is formed with selects, not branches.
+template <bool SMALL, int PREC>
+CAH_DEV uint32_t forest_cell(uint32_t c, uint32_t nf, uint64_t cellid, int64_t t, uint64_t seed,
+                             double p53, double f53, float p32, float f32) {
+    const bool empty = c == 0;
+    uint32_t c0 = (uint32_t)cellid, c1 = SMALL ? 0u : (uint32_t)(cellid >> 32),
+             c2 = (uint32_t)(uint64_t)t, c3 = empty ? 1u : 2u;
+    philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    bool hit;
+    if constexpr (PREC == 0) {
+        const uint64_t bits = ((uint64_t)c0 << 32) | (uint64_t)c1;
+        hit = (double)(bits >> 11) < (empty ? p53 : f53);
+    } else if constexpr (PREC == 3) {
+        hit = (float)(c0 >> 8) * (1.0f / 16777216.0f) < (empty ? p32 : f32);
+    } else {
+        const uint64_t bits = ((uint64_t)c0 << 32) | (uint64_t)c1;
+        const bool hit64 = (double)(bits >> 11) < (empty ? p53 : f53);
+        const bool hit32 = (float)(c0 >> 8) * (1.0f / 16777216.0f) < (empty ? p32 : f32);
+        hit = (empty ? (PREC & 1) : (PREC & 2)) ? hit32 : hit64;
+    }
+    const uint32_t as_empty = hit ? 1u : 0u;            // growth
+    const uint32_t as_tree = (nf || hit) ? 2u : 1u;     // catches fire, or lightning
+    const uint32_t r = empty ? as_empty : as_tree;
+    return c == 2 ? 0u : r;                             // a fire burns out
+}
+
+// The walk of one wave down rows [ys, ye) of its band, as life_walk: rows in a
+// ring of 6 register slots (row r in slot (r - ys + 1) mod 6), each load writing
+// its final slot, 3 rows in flight.
+template <int VB, bool SMALL, int PREC, bool FAST>
+CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer) {
+    constexpr int S = 6;
+    const int8_t cv1 = (int8_t)a.cv1, cv2 = (int8_t)a.cv2;
+    const uint32_t fill1 = 0x01010101u * (uint8_t)cv1, fill2 = 0x01010101u * (uint8_t)cv2;
+    const LaneMap lm = lane_map<VB>(a.L, x);
+    const int32_t last = ye;  // the last row any output of the chunk reads
+    auto load = [&](int32_t y, uint32_t (&X)[VB]) {
+        if constexpr (FAST) load_mapped<VB>(a.cur, a.L, lm, fill1, fill2, y, X);
+        else load_row<VB>(a.cur, a.L, cv1, cv2, x, y, X);
+    };
+    const double p53 = a.p * 0x1.0p53, f53 = a.f * 0x1.0p53;
+    const float p32 = (float)a.p, f32 = (float)a.f;
+
+    uint32_t raw[S][VB], fire[S][VB];
+    load(ys - 1, raw[0]);
+    load(ys, raw[1]);
+    Unroll<2, S>::run([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        load(min(ys + k - 1, last), raw[k]);
+    });
+#pragma unroll
+    for (int i = 0; i < VB; ++i) {
+        fire[0][i] = bytes_eq(raw[0][i], 0x02020202u);
+        fire[1][i] = bytes_eq(raw[1][i], 0x02020202u);
+    }
+
+    int32_t y = ys;
+    auto body = [&](auto PH, bool more) {
+        constexpr int ph = decltype(PH)::value;
+        constexpr int s_dn = (ph + 2) % S, s_c = (ph + 1) % S, s_up = ph % S;
+#pragma unroll
+        for (int i = 0; i < VB; ++i) fire[s_dn][i] = bytes_eq(raw[s_dn][i], 0x02020202u);
+        uint32_t E[VB + 2];
+        E[0] = from_prev_lane(fire[s_c][VB - 1]);
+        E[VB + 1] = from_next_lane(fire[s_c][0]);
+#pragma unroll
+        for (int i = 0; i < VB; ++i) E[1 + i] = fire[s_c][i];
+        uint32_t out[VB];
+        const uint64_t rowid = (uint64_t)y * (uint64_t)a.L.n1 + (uint64_t)x + 1;
+#pragma unroll
+        for (int i = 0; i < VB; ++i) {
+            // VonNeumann(1): left, right, up, down
+            const uint32_t nf = funnel(E[1 + i], E[i], 24) | funnel(E[2 + i], E[1 + i], 8) |
+                                fire[s_up][i] | fire[s_dn][i];
+            const uint32_t c4 = raw[s_c][i];
+            uint32_t o = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t c = (c4 >> (8 * b)) & 0xFFu;
+                const uint32_t f = (nf >> (8 * b)) & 0xFFu;
+                o |= forest_cell<SMALL, PREC>(c, f, rowid + (uint64_t)(4 * i + b), a.t, a.seed,
+                                              p53, f53, p32, f32)
+                     << (8 * b);
+            }
+            out[i] = o;
+        }
+        if (writer) store_row<VB>(a.next, a.L, x, y, out);
+        // the slot of row y - 1 now takes row y + 5 (clamped past the chunk)
+        if (more) load(min(y + 5, last), raw[s_up]);
+        ++y;
+    };
+    while (y + S <= ye) Unroll<0, S>::run([&](auto PH) { body(PH, true); });
+    Unroll<0, S>::run([&](auto PH) {
+        if (y < ye) body(PH, y + 4 < ye);
+    });
+}
+
+template <int VB, bool SMALL, int PREC>
+CAH_DEV void forest_band(const ForestArgs& a, int wid) {
+    constexpr int OW = 62 * 4 * VB;
+    const int lane = threadIdx.x & 63;
+    const int band = wid % a.nbands, chunk = wid / a.nbands;
+    const int32_t ys = chunk * a.rows;
+    const int32_t ye = min(ys + a.rows, a.L.n2);
+    const int32_t x = band * OW - 4 * VB + lane * 4 * VB;
+    const bool writer = lane >= 1 && lane <= 62 && x < a.L.n1;
+    if (mapped_ok<VB>(a.L, 3)) forest_walk<VB, SMALL, PREC, true>(a, x, ys, ye, writer);
+    else forest_walk<VB, SMALL, PREC, false>(a, x, ys, ye, writer);
+}
+
+template <int VB, bool SMALL, int PREC>
+__global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
+    const int wid = wave_id();
+    if (wid >= a.nwaves) return;
+    forest_band<VB, SMALL, PREC>(a, wid);
+}
+
+// ---------------------------------------------------------------- batched ensembles
+//
+// One launch advances every trajectory of a batch by one step.  The waves of all
+// trajectories form one flat list: wave w belongs to item wave_item[w] and is
+// that item's wave (w - wave_base[item]).  Each item is the trajectory's own
+// argument block (buffers for parity 0, lattice, seed, band layout); the step's
+// time, parameters and buffer parity come from the launch.
+
+template <int VB, bool SMALL, int PREC>
+__global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restrict__ items,
+                                                      const int32_t* __restrict__ wave_item,
+                                                      const int32_t* __restrict__ wave_base,
+                                                      int32_t nwaves, int32_t parity, int64_t t,
+                                                      double p, double f, int32_t p_f32,
+                                                      int32_t f_f32) {
+    const int wid = wave_id();
+    if (wid >= nwaves) return;
+    const int k = __builtin_amdgcn_readfirstlane(wave_item[wid]);
+    ForestArgs a = items[k];
+    if (parity) {
+        const int8_t* c = a.cur;
+        a.cur = a.next;
+        a.next = const_cast<int8_t*>(c);
+    }
+    a.t = t;
+    a.p = p;
+    a.f = f;
+    a.p_f32 = p_f32;
+    a.f_f32 = f_f32;
+    forest_band<VB, SMALL, PREC>(a, wid - wave_base[k]);
+}

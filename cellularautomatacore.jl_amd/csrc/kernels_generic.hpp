@@ -1,0 +1,109 @@
+// kernels_generic.hpp -- the general Int8 sweep and the synthetic-input generators.
+//
+//   k_generic_i8  any neighbourhood (halo <= 32), any Int8 rule, every gated
+//                 scheme, masks.  An LDS tile with the boundary conditions applied
+//                 while it is loaded (no halo buffer, no separate halo launch).
+#pragma once
+#include "device.hpp"
+
+// ---------------------------------------------------------------- generic
+
+struct GenArgs {
+    const int8_t* cur;
+    int8_t* next;
+    const uint8_t* mask;
+    const int32_t* ofs;  // n_offsets pairs (o1, o2)
+    Lat L;
+    int32_t h, nofs;
+    int32_t cv1, cv2;
+    int32_t rule, C;
+    uint64_t birth, survive;
+    uint64_t seed;
+    int64_t t;
+    int32_t scheme, gate, sp_f32;
+    double sp;
+    double ffp, fff;
+    int32_t ffp_f32, fff_f32;
+};
+
+constexpr int GEN_TX = 64, GEN_TY = 16;
+
+__global__ __launch_bounds__(256) void k_generic_i8(GenArgs a) {
+    extern __shared__ int8_t tile[];
+    const int W = GEN_TX + 2 * a.h, H = GEN_TY + 2 * a.h;
+    const int x0 = blockIdx.x * GEN_TX, y0 = blockIdx.y * GEN_TY;
+    for (int i = threadIdx.x; i < W * H; i += blockDim.x) {
+        const int lx = i % W, ly = i / W;
+        tile[i] = read_mapped<int8_t>(a.cur, a.L, (int8_t)a.cv1, (int8_t)a.cv2, x0 - a.h + lx,
+                                      y0 - a.h + ly);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % GEN_TX;
+    for (int ty = threadIdx.x / GEN_TX; ty < GEN_TY; ty += blockDim.x / GEN_TX) {
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= a.L.n1 || y >= a.L.n2) continue;
+        const int64_t lin = (int64_t)y * a.L.n1 + x;  // unpadded, 0-based
+        const uint64_t cellid = (uint64_t)lin + 1;
+        const int8_t* cp = tile + (tx + a.h) + (ty + a.h) * W;
+        const int c = *cp;
+        int out = c;
+        const bool open = (a.mask == nullptr || a.mask[lin] != 0) &&
+                          gate_open(a.scheme, a.gate, a.sp, a.sp_f32, a.seed, cellid, a.t, x, y,
+                                    a.L.two_d);
+        if (open) {
+            int n1 = 0, n2 = 0;  // neighbours equal to 1, and to 2
+            for (int k = 0; k < a.nofs; ++k) {
+                const int v = cp[a.ofs[2 * k] + a.ofs[2 * k + 1] * W];
+                n1 += (v == 1);
+                n2 += (v == 2);
+            }
+            if (a.rule == CAH_RULE_LIFE) {
+                // local_rule(::Life) (src/rules.jl:385-389)
+                const uint64_t m = c == 1 ? a.survive : a.birth;
+                out = n1 < 64 ? (int)((m >> n1) & 1u) : 0;
+            } else if (a.rule == CAH_RULE_GENERATIONS) {
+                // local_rule(::Generations) (src/rules.jl:446-456)
+                if (c == 0) out = n1 < 64 ? (int)((a.birth >> n1) & 1u) : 0;
+                else if (c == 1) out = (n1 < 64 && ((a.survive >> n1) & 1u)) ? 1 : (a.C > 2 ? 2 : 0);
+                else out = c + 1 < a.C ? c + 1 : 0;
+            } else {
+                // the forest-fire LocalRule (README.md:56-60), Int8 encoding
+                if (c == 2) out = 0;
+                else if (c == 0) out = bernoulli(a.seed, cellid, a.t, 1, a.ffp, a.ffp_f32) ? 1 : 0;
+                else if (n2 > 0) out = 2;
+                else out = bernoulli(a.seed, cellid, a.t, 2, a.fff, a.fff_f32) ? 2 : 1;
+            }
+        }
+        a.next[(int64_t)y * a.L.pitch + x] = (int8_t)out;
+    }
+}
+
+// ---------------------------------------------------------------- synthetic inputs
+
+// The documented generator: ctr = (cellid lo32, cellid hi32, 0xFFFFFFFF, 0).
+CAH_DEV double gen_uniform(uint64_t seed, uint64_t cellid) {
+    uint32_t c0 = (uint32_t)cellid, c1 = (uint32_t)(cellid >> 32), c2 = 0xFFFFFFFFu, c3 = 0u;
+    philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t bits = ((uint64_t)c0 << 32) | (uint64_t)c1;
+    return (double)(bits >> 11) * 0x1.0p-53;
+}
+
+__global__ void k_fill_bernoulli(int8_t* u, Lat L, uint64_t seed, double p) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= L.n1) return;
+    for (int32_t y = blockIdx.y; y < L.n2; y += gridDim.y) {
+        const uint64_t id = (uint64_t)y * (uint64_t)L.n1 + (uint64_t)x + 1;
+        u[(int64_t)y * L.pitch + x] = gen_uniform(seed, id) < p ? 1 : 0;
+    }
+}
+
+__global__ void k_fill_uniform_box(float* u, Lat L, uint64_t seed, int32_t lo1, int32_t len1,
+                                   int32_t lo2, int32_t len2) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= L.n1) return;
+    for (int32_t y = blockIdx.y; y < L.n2; y += gridDim.y) {
+        const uint64_t id = (uint64_t)y * (uint64_t)L.n1 + (uint64_t)x + 1;
+        const bool in = x >= lo1 && x < lo1 + len1 && y >= lo2 && y < lo2 + len2;
+        u[(int64_t)y * L.pitch + x] = in ? (float)gen_uniform(seed, id) : 0.0f;
+    }
+}

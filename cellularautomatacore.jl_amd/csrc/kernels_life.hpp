@@ -1,0 +1,204 @@
+// kernels_life.hpp -- Life with Moore(R <= 5), Synchronous, no mask.
+//
+// Each lane owns 4*VB consecutive cells of a row and walks down a column band,
+// keeping the vertical (2R+1)-row running count in registers and forming the
+// horizontal (2R+1)-wide sum with byte-parallel (SWAR) adds and funnel shifts;
+// the rule is a union of count intervals tested four cells at a time.  A wave
+// covers 64 lanes and writes the middle 62 (the end lanes are guards).
+#pragma once
+#include "band.hpp"
+
+constexpr int LIFE_MAX_IV = 2;
+
+struct LifeArgs {
+    const int8_t* cur;
+    int8_t* next;
+    Lat L;
+    int32_t cv1, cv2;
+    int32_t nbands, rows, nwaves;
+    int32_t nb, ns;  // birth / survive interval counts
+    // interval [lo, hi] of counts: bytes n with n + (128 - lo) >= 128 and
+    // n + (127 - hi) < 128, replicated over the four bytes
+    uint32_t b_lo[LIFE_MAX_IV], b_hi[LIFE_MAX_IV], s_lo[LIFE_MAX_IV], s_hi[LIFE_MAX_IV];
+};
+
+// Horizontal (2R+1)-wide byte sums of the dwords E (G guard dwords on each side
+// of the lane's VB dwords): W[i] = sum over d in [-R, R] of the bytes at offset d.
+// Offsets d = 4q + rho with the same rho share one funnel shift of a range sum
+// (no byte sum reaches 256, so shifting sums equals summing shifts).
+template <int R, int VB, int G>
+CAH_DEV void hsum(const uint32_t (&E)[VB + 2 * G], uint32_t (&W)[VB]) {
+    // prefix sums over E (bytes stay below 256: at most (VB+2G)(2R+1) <= 88)
+    uint32_t P[VB + 2 * G];
+    P[0] = E[0];
+#pragma unroll
+    for (int j = 1; j < VB + 2 * G; ++j) P[j] = P[j - 1] + E[j];
+    auto range = [&](int a, int b) -> uint32_t { return a > 0 ? P[b] - P[a - 1] : P[b]; };
+#pragma unroll
+    for (int i = 0; i < VB; ++i) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int rho = 0; rho < 4; ++rho) {
+            const int qlo = -fdiv(R + rho, 4);  // ceil((-R - rho) / 4)
+            const int qhi = fdiv(R - rho, 4);
+            if (qlo > qhi) continue;
+            const uint32_t a0 = range(G + i + qlo, G + i + qhi);
+            if (rho == 0) {
+                w += a0;
+            } else {
+                const uint32_t a1 = range(G + i + 1 + qlo, G + i + 1 + qhi);
+                w += funnel(a1, a0, 8 * rho);
+            }
+        }
+        W[i] = w;
+    }
+}
+
+// The new states of VB dwords (four cells each) from the live counts n (centre
+// left out) and the centre cells c (bytes 0 or 1): bit n of survive for a live
+// cell, of birth otherwise, each a union of count intervals.
+// NIV intervals per mask are tested; unused ones are neutral (lo = 128 never
+// matches a count below 128).
+template <int VB, int NIV>
+CAH_DEV void life_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t (&c)[VB],
+                       uint32_t (&out)[VB]) {
+#pragma unroll
+    for (int i = 0; i < VB; ++i) {
+        uint32_t inb = 0, ins = 0;
+#pragma unroll
+        for (int k = 0; k < NIV; ++k) {
+            inb |= (n[i] + a.b_lo[k]) & ~(n[i] + a.b_hi[k]);
+            ins |= (n[i] + a.s_lo[k]) & ~(n[i] + a.s_hi[k]);
+        }
+        const uint32_t sel = c[i] << 7;  // 0x80 in the bytes of live cells
+        out[i] = (((sel & ins) | (~sel & inb)) & 0x80808080u) >> 7;
+    }
+}
+
+// The walk of one wave down rows [ys, ye) of its band (ys, ye wave-uniform).
+// FAST: the lattice suits load_mapped, so every row is one vector load per lane
+// with the boundary applied branch-free; otherwise rows and bytes go through
+// the general boundary map.  Rows live in a ring of 2P = 4R+2 register slots, row r in slot
+// (r - ys + R) mod 2P, and each load writes its final slot directly: the slot of
+// the row that just left the window.  P = 2R+1 rows are in flight per wave.
+template <int R, int VB, bool BIN, int NIV, bool FAST>
+CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer) {
+    constexpr int P = 2 * R + 1, S = 2 * P;
+    constexpr int G = (R + 3) / 4;  // guard dwords needed on each side
+    static_assert(G <= VB, "a lane must hold the guard dwords of its neighbours");
+    const int8_t cv1 = (int8_t)a.cv1, cv2 = (int8_t)a.cv2;
+    const uint32_t fill1 = 0x01010101u * (uint8_t)cv1, fill2 = 0x01010101u * (uint8_t)cv2;
+    const LaneMap lm = lane_map<VB>(a.L, x);
+    const int32_t last = ye + R - 1;  // the last row any output of the chunk reads
+    auto load = [&](int32_t y, uint32_t (&X)[VB]) {
+        if constexpr (FAST) load_mapped<VB>(a.cur, a.L, lm, fill1, fill2, y, X);
+        else load_row<VB>(a.cur, a.L, cv1, cv2, x, y, X);
+    };
+    auto ones = [&](uint32_t (&X)[VB]) {
+        if constexpr (!BIN) {
+#pragma unroll
+            for (int i = 0; i < VB; ++i) X[i] = bytes_eq(X[i], 0x01010101u);
+        }
+    };
+
+    uint32_t ring[S][VB];
+    // rows ys-R .. ys+R-1 counted now; rows ys+R .. ys+R+P in flight
+    Unroll<0, 2 * R>::run([&](auto K) { load(ys - R + decltype(K)::value, ring[decltype(K)::value]); });
+    Unroll<2 * R, S>::run([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        load(min(ys + k - R, last), ring[k]);
+    });
+    uint32_t V[VB];
+#pragma unroll
+    for (int i = 0; i < VB; ++i) V[i] = 0;
+    Unroll<0, 2 * R>::run([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        ones(ring[k]);
+#pragma unroll
+        for (int i = 0; i < VB; ++i) V[i] += ring[k][i];
+    });
+
+    int32_t y = ys;
+    auto body = [&](auto PH, bool more) {
+        constexpr int ph = decltype(PH)::value;
+        constexpr int s_new = (ph + 2 * R) % S;  // row y + R
+        constexpr int s_c = (ph + R) % S;        // row y
+        constexpr int s_old = ph % S;            // row y - R
+        ones(ring[s_new]);
+#pragma unroll
+        for (int i = 0; i < VB; ++i) V[i] += ring[s_new][i];
+        uint32_t E[VB + 2 * G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            E[j] = from_prev_lane(V[VB - G + j]);
+            E[G + VB + j] = from_next_lane(V[j]);
+        }
+#pragma unroll
+        for (int i = 0; i < VB; ++i) E[G + i] = V[i];
+        uint32_t Wd[VB], n[VB], out[VB];
+        hsum<R, VB, G>(E, Wd);
+#pragma unroll
+        for (int i = 0; i < VB; ++i) n[i] = Wd[i] - ring[s_c][i];  // Moore leaves the centre out
+        life_rule<VB, NIV>(a, n, ring[s_c], out);
+        if (writer) store_row<VB>(a.next, a.L, x, y, out);
+#pragma unroll
+        for (int i = 0; i < VB; ++i) V[i] -= ring[s_old][i];
+        // the slot of row y - R now takes row y - R + 2P = y + R + P + 1
+        // (clamped: rows past the chunk are never read, the clamp repeats the
+        // last row)
+        if (more) load(min(y + R + P + 1, last), ring[s_old]);
+        ++y;
+    };
+    // whole blocks of 2P rows, then the remaining (< 2P) rows
+    while (y + S <= ye) Unroll<0, S>::run([&](auto PH) { body(PH, true); });
+    Unroll<0, S>::run([&](auto PH) {
+        if (y < ye) body(PH, y + P + 1 < ye);
+    });
+}
+
+// One wave's work: band `wid % nbands`, rows chunk `wid / nbands` (wid is
+// wave-uniform, so the row bookkeeping stays in scalar registers).
+template <int R, int VB, bool BIN, int NIV>
+CAH_DEV void life_band(const LifeArgs& a, int wid) {
+    constexpr int OW = 62 * 4 * VB;  // output bytes per band
+    const int lane = threadIdx.x & 63;
+    const int band = wid % a.nbands, chunk = wid / a.nbands;
+    const int32_t ys = chunk * a.rows;
+    const int32_t ye = min(ys + a.rows, a.L.n2);
+    const int32_t x = band * OW - 4 * VB + lane * 4 * VB;
+    const bool writer = lane >= 1 && lane <= 62 && x < a.L.n1;
+    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true>(a, x, ys, ye, writer);
+    else life_walk<R, VB, BIN, NIV, false>(a, x, ys, ye, writer);
+}
+
+template <int R, int VB, bool BIN, int NIV>
+__global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
+    const int wid = wave_id();
+    if (wid >= a.nwaves) return;  // whole waves only
+    life_band<R, VB, BIN, NIV>(a, wid);
+}
+
+// ---------------------------------------------------------------- batched ensembles
+//
+// One launch advances every trajectory of a batch by one step.  The waves of all
+// trajectories form one flat list: wave w belongs to item wave_item[w] and is
+// that item's wave (w - wave_base[item]).  Each item is the trajectory's own
+// argument block (buffers for parity 0, lattice, band layout); the buffer
+// parity comes from the launch.
+
+template <int R, int VB, bool BIN, int NIV>
+__global__ __launch_bounds__(256) void k_life_batch(const LifeArgs* __restrict__ items,
+                                                    const int32_t* __restrict__ wave_item,
+                                                    const int32_t* __restrict__ wave_base,
+                                                    int32_t nwaves, int32_t parity) {
+    const int wid = wave_id();
+    if (wid >= nwaves) return;
+    const int k = __builtin_amdgcn_readfirstlane(wave_item[wid]);
+    LifeArgs a = items[k];
+    if (parity) {
+        const int8_t* c = a.cur;
+        a.cur = a.next;
+        a.next = const_cast<int8_t*>(c);
+    }
+    life_band<R, VB, BIN, NIV>(a, wid - wave_base[k]);
+}

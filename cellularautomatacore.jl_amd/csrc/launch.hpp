@@ -1,0 +1,23 @@
+// launch.hpp -- host-side launchers of the band-walk kernels.  The kernels of
+// each family (and each Life radius) are instantiated in their own translation
+// unit (life_r1.hip .. life_r5.hip, forest.hip) so that they compile in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels_forest.hpp"
+#include "kernels_life.hpp"
+
+// Life, Moore(R): vb in {1, 2} (R = 5 uses 2); niv = intervals per mask (1 or 2).
+template <int R>
+void launch_life_r(int vb, bool bin, int niv, int blocks, const LifeArgs& a, hipStream_t st);
+template <int R>
+void launch_life_batch_r(int vb, bool bin, int niv, int blocks, const LifeArgs* items,
+                         const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,
+                         int32_t parity, hipStream_t st);
+
+// forest fire, VonNeumann(1): vb in {1, 2, 4}; small = fewer than 2^32 cells.
+void launch_forest(int vb, bool small, int blocks, const ForestArgs& a, hipStream_t st);
+void launch_forest_batch(int vb, bool small, int blocks, const ForestArgs* items,
+                         const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,
+                         int32_t parity, int64_t t, double p, double f, int32_t p_f32,
+                         int32_t f_f32, hipStream_t st);
