@@ -156,8 +156,17 @@ extern "C" int cah_check_problem(const cah_desc* d) {
         if (d->lenia_precision != 32 && d->lenia_precision != 64)
             return fail(CAH_ERR_ARGUMENT, "Lenia precision must be 32 or 64");
         if (!(d->sigma > 0)) return fail(CAH_ERR_ARGUMENT, "sigma must be positive");
-        if (d->field_method != CAH_FIELD_DIRECT_SEQ && d->field_method != CAH_FIELD_DIRECT_F32)
-            return fail(CAH_ERR_UNSUPPORTED, "field method %d is not available", d->field_method);
+        if (d->field_method < CAH_FIELD_DIRECT_SEQ || d->field_method > CAH_FIELD_MFMA_F16X2)
+            return fail(CAH_ERR_ARGUMENT, "unknown field method %d", d->field_method);
+        if (d->field_method == CAH_FIELD_FFT_F32) {
+            // _resolve_field (src/environment.jl:64-66)
+            for (int a = 0; a < d->ndims; ++a)
+                if (d->boundary[a] != CAH_BOUNDARY_PERIODIC)
+                    return fail(CAH_ERR_ARGUMENT,
+                                "field :potential uses method=:fft, which needs periodic boundaries on every axis");
+            if (d->ndims != 2)
+                return fail(CAH_ERR_UNSUPPORTED, "the FFT field runs on two-dimensional lattices");
+        }
         break;
     default:
         return fail(CAH_ERR_ARGUMENT,
@@ -169,7 +178,8 @@ extern "C" int cah_check_problem(const cah_desc* d) {
 
 // ---------------------------------------------------------------- state
 
-enum Path { P_GENERIC, P_LIFE_SWAR, P_FOREST, P_LENIA_SEQ, P_LENIA_F32COL };
+enum Path { P_GENERIC, P_LIFE_SWAR, P_FOREST, P_LENIA_SEQ, P_LENIA_F32COL, P_LENIA_MFMA,
+            P_LENIA_FFT };
 
 struct cah_state {
     int device = 0;
@@ -188,6 +198,9 @@ struct cah_state {
     float* w32_d = nullptr;    // per-tap Float32 weights
     float* wtab_d = nullptr;   // dense table for k_lenia_f32col
     void* field_d = nullptr;   // field buffer for the sequential path
+    void* frag_d = nullptr;    // MFMA A fragments
+    int lenia_mode = 0;        // LM_F16 / LM_BF16 / LM_F16X2
+    LeniaFFT* fft = nullptr;
     int32_t halo = 1;
     int32_t lenia_H = 0;
     Path path = P_GENERIC;
@@ -258,8 +271,9 @@ static void free_state(cah_state* s) {
     hipSetDevice(s->device);
     if (s->stream) hipStreamSynchronize(s->stream);
     for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
-                    (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d})
+                    (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d, s->frag_d})
         if (p) hipFree(p);
+    lenia_fft_destroy(s->fft);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -315,8 +329,32 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s) {
             tab[(size_t)(s->taps[2 * k + 1] + s->lenia_H) * K + (s->taps[2 * k] + s->lenia_H)] +=
                 (float)s->weights[k];
         if (int rc = upload(&s->wtab_d, tab.data(), tab.size())) return rc;
-        const bool fused = d->field_method == CAH_FIELD_DIRECT_F32 &&
-                           d->scheme != CAH_SCHEME_CHECKERBOARD;
+        // fused field + growth kernels need the field of the step-start state for
+        // every cell they update, so Checkerboard (two half-sweeps reading the
+        // same field) runs on the separate field kernels
+        const bool cb = d->scheme == CAH_SCHEME_CHECKERBOARD;
+        const int fm = d->field_method;
+        if (fm == CAH_FIELD_FFT_F32) {
+            s->path = P_LENIA_FFT;
+            if (int rc = lenia_fft_create(&s->fft, (int)s->n1, (int)s->n2, s->pitch, s->taps.data(),
+                                          s->weights.data(), d->n_taps, s->stream))
+                return fail(rc, "cannot set up the FFT field (hipFFT)");
+            return CAH_OK;
+        }
+        if ((fm == CAH_FIELD_MFMA_F16 || fm == CAH_FIELD_MFMA_BF16 || fm == CAH_FIELD_MFMA_F16X2) &&
+            !cb && ht <= LM_HMAX) {
+            s->path = P_LENIA_MFMA;
+            s->lenia_mode = fm == CAH_FIELD_MFMA_BF16 ? LM_BF16 : fm == CAH_FIELD_MFMA_F16X2 ? LM_F16X2 : LM_F16;
+            const int Kd = 2 * ht + 1;
+            std::vector<double> wd((size_t)Kd * Kd, 0.0);
+            for (int k = 0; k < d->n_taps; ++k)
+                wd[(size_t)(s->taps[2 * k + 1] + ht) * Kd + (s->taps[2 * k] + ht)] += s->weights[k];
+            s->lenia_H = ht;
+            if (int rc = lenia_mfma_fragments(s->lenia_mode, ht, wd.data(), &s->frag_d))
+                return fail(rc, "cannot upload the MFMA fragments");
+            return CAH_OK;
+        }
+        const bool fused = fm == CAH_FIELD_DIRECT_F32 && !cb;
         s->path = fused ? P_LENIA_F32COL : P_LENIA_SEQ;
         if (!fused)
             HIPCHK(hipMalloc(&s->field_d, (size_t)(s->n1 * s->n2) * (d->lenia_precision == 32 ? 4 : 8)));
@@ -626,8 +664,36 @@ static int launch_lenia_field_seq(cah_state* s, int64_t t) {
 
 static int launch_lenia_growth(cah_state* s, int64_t t, int gate) {
     LeniaArgs a = lenia_args(s, t, gate);
+    if (s->fft) {
+        a.field = lenia_fft_buffer(s->fft);
+        a.field32 = 1;
+    }
     dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
     k_lenia_growth<<<grid, 256, 0, s->stream>>>(a);
+    return CAH_OK;
+}
+
+static int launch_lenia_mfma_step(cah_state* s, int64_t t) {
+    LeniaMfmaArgs a{};
+    a.cur = (const float*)s->buf[s->cur];
+    a.next = (float*)s->buf[1 - s->cur];
+    a.mask = s->mask_d;
+    a.afrag = s->frag_d;
+    a.L = lat_of(s);
+    a.H = s->lenia_H;
+    a.cv1 = (float)s->d.boundary_value[0];
+    a.cv2 = (float)s->d.boundary_value[1];
+    a.prec32 = s->d.lenia_precision == 32;
+    a.mu = s->d.mu;
+    a.sigma = s->d.sigma;
+    a.dt = s->d.dt;
+    a.seed = s->d.seed;
+    a.t = t;
+    a.scheme = s->d.scheme;
+    a.sp = s->d.scheme_p;
+    a.sp_f32 = s->d.scheme_p_is_f32;
+    dim3 grid((unsigned)((s->n1 + LM_BX - 1) / LM_BX), (unsigned)((s->n2 + LM_BY - 1) / LM_BY));
+    launch_lenia_mfma(s->lenia_mode, grid, a, s->stream);
     return CAH_OK;
 }
 
@@ -642,12 +708,16 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t t = t0 + k;
         if (s->path == P_LENIA_SEQ) launch_lenia_field_seq(s, t);
+        if (s->path == P_LENIA_FFT && lenia_fft_field(s->fft, (const float*)s->buf[s->cur], s->stream))
+            return fail(CAH_ERR_DEVICE, "hipFFT failed in the Lenia field");
         for (int g = 0; g < ngates(s->d); ++g) {
             switch (s->path) {
             case P_LIFE_SWAR: step_life(s); break;
             case P_FOREST: step_forest(s, t, p); break;
-            case P_LENIA_SEQ: launch_lenia_growth(s, t, g); break;
+            case P_LENIA_SEQ:
+            case P_LENIA_FFT: launch_lenia_growth(s, t, g); break;
             case P_LENIA_F32COL: launch_lenia_f32col(s, t); break;
+            case P_LENIA_MFMA: launch_lenia_mfma_step(s, t); break;
             default: launch_generic(s, t, g, p); break;
             }
             s->cur = 1 - s->cur;

@@ -10,6 +10,7 @@
 //                      CB+2H tile values in registers for CB outputs.
 #pragma once
 #include "device.hpp"
+#include "lenia_growth.hpp"
 
 struct LeniaArgs {
     const float* cur;
@@ -28,6 +29,7 @@ struct LeniaArgs {
     int64_t t;
     int32_t scheme, gate, sp_f32;
     double sp;
+    int32_t field32;      // the field buffer holds FP32 values (FFT path) for a Lenia() rule
 };
 
 constexpr int LEN_TX = 64, LEN_TY = 16;
@@ -57,25 +59,6 @@ __global__ __launch_bounds__(256) void k_lenia_field_seq(LeniaArgs a) {
     }
 }
 
-// local_rule(::Lenia): G(U) = 2 exp(-(U - mu)^2 / (2 sigma^2)) - 1 and
-// clamp(cell + dt G(U), 0, 1), in FP64 for Lenia() (then rounded to Float32) and
-// in FP32 for Lenia(; precision=Float32).
-CAH_DEV float lenia_new64(float c, double U, double mu, double sigma, double dt) {
-    const double d = U - mu;
-    const double g = 2.0 * exp(-(d * d) / (2.0 * (sigma * sigma))) - 1.0;
-    double v = (double)c + dt * g;
-    v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
-    return (float)v;
-}
-
-CAH_DEV float lenia_new32(float c, float U, float mu, float sigma, float dt) {
-    const float d = U - mu;
-    const float g = 2.0f * expf(-(d * d) / (2.0f * (sigma * sigma))) - 1.0f;
-    float v = c + dt * g;
-    v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
-    return v;
-}
-
 __global__ __launch_bounds__(256) void k_lenia_growth(LeniaArgs a) {
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= a.L.n1) return;
@@ -90,6 +73,9 @@ __global__ __launch_bounds__(256) void k_lenia_growth(LeniaArgs a) {
         if (a.prec32)
             out = lenia_new32(c, static_cast<const float*>(a.field)[lin], (float)a.mu,
                               (float)a.sigma, (float)a.dt);
+        else if (a.field32)
+            out = lenia_new64(c, (double)static_cast<const float*>(a.field)[lin], a.mu, a.sigma,
+                              a.dt);
         else
             out = lenia_new64(c, static_cast<const double*>(a.field)[lin], a.mu, a.sigma, a.dt);
     }

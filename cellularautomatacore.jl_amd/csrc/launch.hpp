@@ -21,3 +21,21 @@ void launch_forest_batch(int vb, bool small, int blocks, const ForestArgs* items
                          const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,
                          int32_t parity, int64_t t, double p, double f, int32_t p_f32,
                          int32_t f_f32, hipStream_t st);
+
+// ---------------------------------------------------------------- Lenia (lenia.hip)
+#include "kernels_lenia_mfma.hpp"
+
+// MFMA implicit GEMM: mode LM_F16 / LM_BF16 / LM_F16X2.
+void launch_lenia_mfma(int mode, dim3 grid, const LeniaMfmaArgs& a, hipStream_t st);
+// The A fragments of the implicit GEMM for a dense (2H+1)^2 FP64 weight grid
+// wd[(dy + H) * (2H + 1) + (dx + H)]: a device buffer the caller frees.
+int lenia_mfma_fragments(int mode, int H, const double* wd, void** out_dev);
+
+// FP32 FFT field on a periodic lattice (the FFTW extension's cross-correlation).
+struct LeniaFFT;
+int lenia_fft_create(LeniaFFT** out, int n1, int n2, int64_t pitch, const int32_t* taps,
+                     const double* w, int ntaps, hipStream_t st);
+// field = irfft(rfft(cur) .* conj(rfft(kpad))) into the plan's dense FP32 buffer
+int lenia_fft_field(LeniaFFT* f, const float* cur, hipStream_t st);
+float* lenia_fft_buffer(LeniaFFT* f);
+void lenia_fft_destroy(LeniaFFT* f);

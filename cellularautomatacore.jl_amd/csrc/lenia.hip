@@ -1,0 +1,152 @@
+// lenia.hip -- Lenia's MFMA implicit-GEMM field and FP32 FFT field.
+#include <hipfft/hipfft.h>
+
+#include <cstring>
+#include <vector>
+
+#include "launch.hpp"
+
+void launch_lenia_mfma(int mode, dim3 grid, const LeniaMfmaArgs& a, hipStream_t st) {
+    if (mode == LM_BF16) k_lenia_mfma<LM_BF16><<<grid, 256, 0, st>>>(a);
+    else if (mode == LM_F16X2) k_lenia_mfma<LM_F16X2><<<grid, 256, 0, st>>>(a);
+    else k_lenia_mfma<LM_F16><<<grid, 256, 0, st>>>(a);
+}
+
+// Fragment f = (dyi * 2 + c) * 64 + lane holds A_dy[m][k] for m = lane % 16 and
+// k = 32 c + 8 (lane / 16) + i, i = 0..7 (the operand order of
+// v_mfma_f32_16x16x32_*; B uses the same k order, so the sum is the same under
+// any fixed k permutation).  A_dy[m][k] = W(k - H - m, dy), zero off the band.
+template <typename E>
+static void fill_fragments(int H, const double* wd, std::vector<E>& hi, std::vector<E>& lo) {
+    const int K = 2 * H + 1;
+    const size_t nfrag = (size_t)K * 2 * 64;
+    hi.assign(nfrag * 8, (E)0.0f);
+    lo.assign(nfrag * 8, (E)0.0f);
+    for (int dyi = 0; dyi < K; ++dyi)
+        for (int c = 0; c < 2; ++c)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int i = 0; i < 8; ++i) {
+                    const int m = lane % 16, k = 32 * c + 8 * (lane / 16) + i;
+                    const int dx = k - H - m;
+                    double w = 0.0;
+                    if (dx >= -H && dx <= H) w = wd[(size_t)dyi * K + (dx + H)];
+                    const size_t at = (((size_t)dyi * 2 + c) * 64 + lane) * 8 + i;
+                    const E h = (E)(float)w;
+                    hi[at] = h;
+                    lo[at] = (E)(float)(w - (double)(float)h);
+                }
+}
+
+int lenia_mfma_fragments(int mode, int H, const double* wd, void** out_dev) {
+    const size_t nfrag = (size_t)(2 * H + 1) * 2 * 64 * 8;
+    const size_t es = 2;
+    std::vector<char> buf(nfrag * es * 2);
+    if (mode == LM_BF16) {
+        std::vector<__bf16> hi, lo;
+        fill_fragments(H, wd, hi, lo);
+        std::memcpy(buf.data(), hi.data(), nfrag * es);
+        std::memcpy(buf.data() + nfrag * es, lo.data(), nfrag * es);
+    } else {
+        std::vector<_Float16> hi, lo;
+        fill_fragments(H, wd, hi, lo);
+        std::memcpy(buf.data(), hi.data(), nfrag * es);
+        std::memcpy(buf.data() + nfrag * es, lo.data(), nfrag * es);
+    }
+    if (hipMalloc(out_dev, buf.size()) != hipSuccess) return -4;
+    if (hipMemcpy(*out_dev, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) return -7;
+    return 0;
+}
+
+// ---------------------------------------------------------------- FFT field
+
+struct LeniaFFT {
+    hipfftHandle fwd = 0, inv = 0;
+    float2* spec = nullptr;
+    float2* kspec = nullptr;
+    float* field = nullptr;
+    int n1 = 0, n2 = 0;
+    int64_t nspec = 0;
+};
+
+__global__ void k_spec_conj_scale(float2* k, int64_t n, float scale) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) k[i] = make_float2(k[i].x * scale, -k[i].y * scale);
+}
+
+__global__ void k_spec_mul(float2* s, const float2* k, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 a = s[i], b = k[i];
+        s[i] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+}
+
+void lenia_fft_destroy(LeniaFFT* f) {
+    if (!f) return;
+    if (f->fwd) hipfftDestroy(f->fwd);
+    if (f->inv) hipfftDestroy(f->inv);
+    for (void* p : {(void*)f->spec, (void*)f->kspec, (void*)f->field})
+        if (p) hipFree(p);
+    delete f;
+}
+
+int lenia_fft_create(LeniaFFT** out, int n1, int n2, int64_t pitch, const int32_t* taps,
+                     const double* w, int ntaps, hipStream_t st) {
+    LeniaFFT* f = new LeniaFFT();
+    f->n1 = n1;
+    f->n2 = n2;
+    const int nc = n1 / 2 + 1;
+    f->nspec = (int64_t)nc * n2;
+    auto bail = [&](int rc) {
+        lenia_fft_destroy(f);
+        return rc;
+    };
+    if (hipMalloc(&f->spec, f->nspec * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f->kspec, f->nspec * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&f->field, (size_t)n1 * n2 * sizeof(float)) != hipSuccess)
+        return bail(-4);
+    // column-major (n1 fastest) = row-major (n2, n1); the state rows have a pitch
+    int n[2] = {n2, n1};
+    int in_embed[2] = {n2, (int)pitch}, spec_embed[2] = {n2, nc}, dense_embed[2] = {n2, n1};
+    if (hipfftPlanMany(&f->fwd, 2, n, in_embed, 1, (int)pitch * n2, spec_embed, 1, nc * n2,
+                       HIPFFT_R2C, 1) != HIPFFT_SUCCESS)
+        return bail(-7);
+    if (hipfftPlanMany(&f->inv, 2, n, spec_embed, 1, nc * n2, dense_embed, 1, n1 * n2, HIPFFT_C2R,
+                       1) != HIPFFT_SUCCESS)
+        return bail(-7);
+    hipfftSetStream(f->fwd, st);
+    hipfftSetStream(f->inv, st);
+    // the kernel wrapped onto the lattice: kpad[mod(o, dims)] += w over the read
+    // offsets (ext/CellularAutomataCoreFFTWExt.jl:17-24), laid out with the state pitch
+    std::vector<float> kpad((size_t)pitch * n2, 0.0f);
+    std::vector<double> acc((size_t)n1 * n2, 0.0);
+    for (int k = 0; k < ntaps; ++k) {
+        const int a = ((taps[2 * k] % n1) + n1) % n1, b = ((taps[2 * k + 1] % n2) + n2) % n2;
+        acc[(size_t)b * n1 + a] += w[k];
+    }
+    for (int b = 0; b < n2; ++b)
+        for (int a = 0; a < n1; ++a) kpad[(size_t)b * pitch + a] = (float)acc[(size_t)b * n1 + a];
+    float* kdev = nullptr;
+    if (hipMalloc(&kdev, kpad.size() * sizeof(float)) != hipSuccess) return bail(-4);
+    hipMemcpy(kdev, kpad.data(), kpad.size() * sizeof(float), hipMemcpyHostToDevice);
+    hipfftResult r = hipfftExecR2C(f->fwd, kdev, (hipfftComplex*)f->kspec);
+    hipStreamSynchronize(st);
+    hipFree(kdev);
+    if (r != HIPFFT_SUCCESS) return bail(-7);
+    // conj(rfft(kpad)) / (n1 n2): the inverse transform is unnormalised
+    const float scale = 1.0f / ((float)n1 * (float)n2);
+    k_spec_conj_scale<<<(unsigned)((f->nspec + 255) / 256), 256, 0, st>>>(f->kspec, f->nspec, scale);
+    if (hipStreamSynchronize(st) != hipSuccess) return bail(-7);
+    *out = f;
+    return 0;
+}
+
+int lenia_fft_field(LeniaFFT* f, const float* cur, hipStream_t st) {
+    if (hipfftExecR2C(f->fwd, const_cast<float*>(cur), (hipfftComplex*)f->spec) != HIPFFT_SUCCESS)
+        return -7;
+    k_spec_mul<<<(unsigned)((f->nspec + 255) / 256), 256, 0, st>>>(f->spec, f->kspec, f->nspec);
+    if (hipfftExecC2R(f->inv, (hipfftComplex*)f->spec, f->field) != HIPFFT_SUCCESS) return -7;
+    return 0;
+}
+
+float* lenia_fft_buffer(LeniaFFT* f) { return f->field; }

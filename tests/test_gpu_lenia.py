@@ -1,0 +1,76 @@
+"""Lenia's field methods on the GPU against the FP64 oracle, per step.
+
+Tolerances (max-abs per step, one GPU step vs one FP64 oracle step from the
+same state): FFT-FP32 and split-FP16 MFMA 1e-5, FP16 MFMA 1e-3 (the target the
+reference's own Metal check uses, test/metal.jl:52-56), BF16 MFMA 2e-2 (8-bit
+inputs; reported, not a parity method)."""
+import numpy as np
+import pytest
+
+from cahip import model as M
+from cahip.backend import HIPBackend
+from cahip.problem import CAProblem, solve
+from helpers import desc_for, golden
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fft_f32": 1e-5, "mfma_f16x2": 1e-5, "mfma_f16": 1e-3, "mfma_bf16": 2e-2, "f32": 1e-4}
+
+
+def patch_state(dims, lo, size, seed):
+    rng = np.random.default_rng(seed)
+    u = np.zeros(dims, np.float32)
+    u[lo[0]:lo[0] + size[0], lo[1]:lo[1] + size[1]] = rng.random(size).astype(np.float32)
+    return np.asfortranarray(u)
+
+
+@pytest.mark.parametrize("field", list(TOL))
+@pytest.mark.parametrize("dims", [(128, 96), (200, 136)])
+def test_lenia_field_methods_per_step(oracle, field, dims):
+    u = patch_state(dims, (30, 20), (48, 40), 7)
+    prob = CAProblem(M.Lenia(), u, 6)
+    sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field=field), saveat=1)
+    d, keep = desc_for(prob, field_method=0)
+    worst = 0.0
+    for t in range(6):
+        ref = oracle.advance_plain(d, sol.u[t], 0, 1)
+        worst = max(worst, float(np.abs(sol.u[t + 1].astype(np.float64) - ref).max()))
+    assert worst < TOL[field], f"{field}: per-step max-abs {worst:.3g}"
+
+
+@pytest.mark.parametrize("field", ["mfma_f16", "fft_f32"])
+def test_orbium_glides_on_gpu(field):
+    rows = np.array(golden("reference_vectors.json")["orbium"]["rows"], dtype=np.float32)
+    u = np.zeros((64, 64), np.float32)
+    u[19:39, 19:39] = rows
+    prob = CAProblem(M.Lenia(), u, 200)
+    sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field=field), saveat=100)
+    mass = [s.astype(np.float64).sum() for s in sol.u]
+    assert all(abs(m - mass[0]) / mass[0] < 0.15 for m in mass)
+
+    def centre(s):
+        s = s.astype(np.float64)
+        return (np.arange(1, 65)[:, None] * s).sum() / s.sum()
+    assert abs(centre(sol.u[1]) - centre(sol.u[0])) > 3
+
+
+def test_fft_needs_periodic_edges():
+    u = np.zeros((32, 32), np.float32)
+    with pytest.raises(M.ArgumentError):
+        CAProblem(M.Lenia(method="fft"), u, 1, space=M.Grid(32, 32))
+    prob = CAProblem(M.Lenia(), u, 1, space=M.Grid(32, 32, boundary=M.Reflecting()))
+    with pytest.raises(M.ArgumentError):
+        HIPBackend(lenia_field="fft_f32").check_problem(prob, M.Synchronous())
+
+
+def test_mfma_with_edges_and_gates(oracle):
+    # Reflecting/Constant edges and a Stochastic gate through the fused MFMA kernel
+    u = patch_state((96, 80), (10, 10), (60, 50), 3)
+    space = M.Lattice((96, 80), (M.Reflecting(), M.Constant(0.25)))
+    prob = CAProblem(M.Lenia(R=9), u, 3, space=space, seed=5)
+    for scheme in (M.Synchronous(), M.Stochastic(0.5)):
+        sol = solve(prob, scheme, backend=HIPBackend(lenia_field="mfma_f16x2"), saveat=1)
+        d, keep = desc_for(prob, scheme, field_method=0)
+        for t in range(3):
+            ref = oracle.advance_plain(d, sol.u[t], t, 1)
+            assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() < 1e-5
