@@ -96,12 +96,261 @@ def cpu_baseline(nthreads, steps=2):
             "ltl_ms_per_step": t_ltl * 1e3, "forest_ms_per_step": t_for * 1e3}
 
 
+def timed(ev, stream, fn):
+    """Milliseconds between HIP events recorded on `stream` around fn()."""
+    a, b = ev.event(), ev.event()
+    ev.sync()
+    ev.record(a, stream)
+    fn()
+    ev.record(b, stream)
+    ev.sync()
+    return ev.elapsed_ms(a, b)
+
+
+def bench_life512(be, ev, steps=1000):
+    """Conway Life B3/S23, Torus 512^2, Bernoulli(0.5) seed 1, 1000 steps."""
+    from cahip import model as M
+    from cahip.backend import check, make_params
+    from cahip.problem import CAProblem
+    prob = CAProblem(M.Life("conway"), np.zeros((512, 512), np.int8), steps,
+                     space=M.Torus(512, 512), seed=1)
+    st = be.allocate_state(prob, M.Synchronous())
+    check(be.lib.cah_state_fill_bernoulli(st.handle, 1, 0.5))
+    cp = make_params(prob, None)
+    run = lambda: check(be.lib.cah_advance(st.handle, 0, steps, ctypes.byref(cp)))  # noqa: E731
+    run()
+    ms = timed(ev, be.stream(st), run)
+    be.free_state(st)
+    cells = 512 * 512
+    return {"workload": "Conway Life Moore(1) Torus 512^2 Bernoulli(0.5) seed 1, 1000 steps",
+            "us_per_step": round(ms * 1e3 / steps, 3),
+            "gcups": round(cells * steps / (ms * 1e-3) / 1e9, 2),
+            "bound": "launch latency (0.5 MiB working set stays in L2)"}
+
+
+def batch_sides(n=256, seed=2026):
+    """Square sides log-uniform in [64, 2048]: round(64 * 32^u), u from numpy's
+    default_rng(2026) (the batched workload's documented generator)."""
+    u = np.random.default_rng(seed).random(n)
+    return [int(round(64 * 32 ** x)) for x in u]
+
+
+def bench_batched(be, ev, steps=300, n=256):
+    """256 forest-fire trajectories (seeds 1..256, Reflecting, all empty,
+    p=0.03, f=1e-4), sides log-uniform in [64, 2048], 300 steps: one launch per
+    step for all of them under a captured hipGraph, against the plain
+    per-trajectory loop."""
+    from cahip import _lib as L
+    from cahip import model as M
+    from cahip.backend import check, make_desc, make_params
+    from cahip.problem import CAProblem, seed_trajectory
+    sides = batch_sides(n)
+    base = [CAProblem(M.ForestFire(), np.zeros((s, s), np.int8), steps, {"p": 0.03, "f": 1e-4},
+                      space=M.Grid(s, s, boundary=M.Reflecting()), seed=0) for s in sides]
+    probs = [seed_trajectory(p, i + 1) for i, p in enumerate(base)]
+    cells = sum(s * s for s in sides)
+    descs, keeps = zip(*[make_desc(p, M.Synchronous()) for p in probs])
+    arr = (L.CahDesc * n)(*descs)
+    cp = make_params(probs[0], probs[0].p)
+    h = ctypes.c_void_p()
+    check(be.lib.cah_batch_create(be.ctx(), n, arr, ctypes.byref(h)))
+    s = ctypes.c_void_p()
+    check(be.lib.cah_batch_stream(h, ctypes.byref(s)))
+    check(be.lib.cah_batch_advance(h, 0, 2, ctypes.byref(cp), 1))  # warm-up (graph path)
+    check(be.lib.cah_batch_sync(h))
+    for i, p in enumerate(probs):  # restart every trajectory from its empty lattice
+        check(be.lib.cah_batch_set(h, i, p.u0.ctypes.data, p.u0.nbytes))
+    ms_batch = timed(ev, s.value, lambda: check(
+        be.lib.cah_batch_advance(h, 0, steps, ctypes.byref(cp), 1)))
+    batched_final = []
+    for i, p in enumerate(probs):
+        out = np.empty(p.space.dims, np.int8, order="F")
+        check(be.lib.cah_batch_get(h, i, out.ctypes.data, out.nbytes))
+        batched_final.append(out)
+    be.lib.cah_batch_destroy(h)
+
+    def plain(sync_every_step):
+        finals = []
+        ev.sync()
+        t0 = time.perf_counter()
+        for p in probs:
+            st = be.allocate_state(p, M.Synchronous())
+            if sync_every_step:  # advance! then synchronize, as the KA GPUBackend does
+                for t in range(steps):
+                    check(be.lib.cah_advance(st.handle, t, 1, ctypes.byref(cp)))
+                    be.sync(st)
+            else:
+                check(be.lib.cah_advance(st.handle, 0, steps, ctypes.byref(cp)))
+            finals.append(be.current_state(st))
+            be.free_state(st)
+        return time.perf_counter() - t0, finals
+    t_step_sync, finals_a = plain(True)
+    t_queued, finals_b = plain(False)
+    same = all(np.array_equal(a, b) for a, b in zip(batched_final, finals_a)) and \
+        all(np.array_equal(a, b) for a, b in zip(batched_final, finals_b))
+    return {"workload": f"{n} forest-fire trajectories, seeds 1..{n}, sides round(64*32^u) "
+                        f"(u ~ default_rng(2026)), Reflecting, all empty, {steps} steps",
+            "total_cells": cells,
+            "batched_ms": round(ms_batch, 3),
+            "batched_sims_per_s": round(n / (ms_batch * 1e-3), 1),
+            "batched_gcups": round(cells * steps / (ms_batch * 1e-3) / 1e9, 2),
+            "plain_step_sync_sims_per_s": round(n / t_step_sync, 1),
+            "plain_queued_sims_per_s": round(n / t_queued, 1),
+            "speedup_vs_plain_step_sync": round(t_step_sync / (ms_batch * 1e-3), 2),
+            "speedup_vs_plain_queued": round(t_queued / (ms_batch * 1e-3), 2),
+            "identical_trajectories": bool(same)}
+
+
+def bench_lenia(be, ev, steps=500):
+    """Lenia R=13 (mu 0.15, sigma 0.015, dt 0.1), Float32 Torus 2048^2, uniform[0,1]
+    in the central 256^2 patch: ms/step per field method and the per-step max-abs
+    difference from the FP64 sequential field (the path pinned to the oracle)."""
+    from cahip import model as M
+    from cahip.backend import HIPBackend, check, make_params
+    from cahip.problem import CAProblem
+    n = 2048
+    prob = CAProblem(M.Lenia(), np.zeros((n, n), np.float32), steps, space=M.Torus(n, n), seed=1)
+    cp = make_params(prob, None)
+    ref_be = HIPBackend(0, lenia_field="seq")
+    ref = ref_be.allocate_state(prob, M.Synchronous())
+    out = {}
+    samples = (0, 100, 250, steps - 1)
+    for field in ("mfma_f16", "mfma_f16x2", "mfma_bf16", "fft_f32", "f32", "seq"):
+        fb = HIPBackend(0, lenia_field=field)
+        st = fb.allocate_state(prob, M.Synchronous())
+        check(fb.lib.cah_state_fill_uniform_box(st.handle, 1, 896, 256, 896, 256))
+        init = fb.current_state(st)
+        saved = {}
+        t = 0
+        for smp in samples:  # the trajectory, with the states at the sample times
+            check(fb.lib.cah_advance(st.handle, t, smp - t, ctypes.byref(cp)))
+            t = smp
+            saved[smp] = fb.current_state(st)
+        check(fb.lib.cah_advance(st.handle, t, steps - t, ctypes.byref(cp)))
+        final = fb.current_state(st)
+        fb.set_state(st, init)
+        ms = timed(ev, fb.stream(st), lambda: check(
+            fb.lib.cah_advance(st.handle, 0, steps, ctypes.byref(cp))))
+        err = 0.0
+        for smp, u in saved.items():  # one step of each from the same state
+            fb.set_state(st, u)
+            ref_be.set_state(ref, u)
+            check(fb.lib.cah_advance(st.handle, smp, 1, ctypes.byref(cp)))
+            check(ref_be.lib.cah_advance(ref.handle, smp, 1, ctypes.byref(cp)))
+            d = np.abs(fb.current_state(st).astype(np.float64) - ref_be.current_state(ref))
+            err = max(err, float(d.max()))
+        fb.free_state(st)
+        out[field] = {"ms_per_step": round(ms / steps, 4),
+                      "gcups": round(n * n * steps / (ms * 1e-3) / 1e9, 2),
+                      "per_step_max_abs_vs_fp64": err, "final_mass": float(final.sum())}
+    ref_be.free_state(ref)
+    seq_final = out["seq"]["final_mass"]
+    for f in out:
+        out[f]["final_mass_rel_diff_vs_fp64"] = abs(out[f]["final_mass"] - seq_final) / seq_final
+    return {"workload": "Lenia R=13 Float32 Torus 2048^2, central 256^2 uniform patch, 500 steps",
+            "methods": out}
+
+
+def serving_requests(duration_s=1.5, rate=200.0, seed=5):
+    """A Poisson stream (rate req/s over duration_s) of Life, forest-fire and Lenia
+    requests in turn; sides round(128 * 32^u) (log-uniform in [128, 4096]), tspan
+    uniform in [50, 500]; initial states: Life Bernoulli(0.5), forest all empty,
+    Lenia uniform in a central patch of side/8.  numpy default_rng(5)."""
+    from cahip import model as M
+    from cahip.problem import CAProblem
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            break
+        i = len(reqs)
+        side = int(round(128 * 32 ** rng.random()))
+        steps = int(rng.integers(50, 501))
+        kind = i % 3
+        if kind == 0:
+            u = rng.integers(0, 2, (side, side), dtype=np.int8)
+            p = CAProblem(M.Life("conway"), u, steps, seed=i)
+        elif kind == 1:
+            p = CAProblem(M.ForestFire(), np.zeros((side, side), np.int8), steps,
+                          {"p": 0.03, "f": 1e-4}, seed=1000 + i,
+                          space=M.Grid(side, side, boundary=M.Reflecting()))
+        else:
+            u = np.zeros((side, side), np.float32)
+            k = max(side // 8, 1)
+            lo = (side - k) // 2
+            u[lo:lo + k, lo:lo + k] = rng.random((k, k), dtype=np.float32)
+            p = CAProblem(M.Lenia(), u, steps, seed=i)
+        reqs.append((t, p))
+    return reqs
+
+
+def bench_serving(reqs=None):
+    """Continuous batching (cah_server_*: admission by a 200 GB budget, resident
+    lattices, 64-step chunks) against the plain FIFO loop that solves one request
+    at a time; latency = arrival to result ready."""
+    from cahip import model as M
+    from cahip.backend import HIPBackend, check, make_params
+    from cahip.server import Server
+    reqs = serving_requests() if reqs is None else reqs
+    n = len(reqs)
+    cells_steps = sum(p.u0.size * (p.tspan[1] - p.tspan[0]) for _, p in reqs)
+    srv = Server(budget_bytes=200e9, chunk=64, lenia_field="mfma_f16")
+    # warm-up: one request of each kind
+    for _, p in reqs[:3]:
+        rid = srv.submit(p)
+        while not srv.done(rid):
+            srv.tick()
+        srv.fetch(rid)
+    start = srv.clock()
+    ids, k, done = [], 0, {}
+    while len(done) < n:
+        now = srv.clock() - start
+        while k < n and reqs[k][0] <= now:
+            ids.append(srv.submit(reqs[k][1]))
+            k += 1
+        srv.tick()
+        for j, rid in enumerate(ids):
+            if j not in done and srv.done(rid):
+                srv.fetch(rid)
+                done[j] = srv.times(rid)[2] - start
+    lat = np.array([done[j] - reqs[j][0] for j in range(n)])
+    makespan = max(done.values()) - reqs[0][0]
+    srv.close()
+
+    be = HIPBackend(0, lenia_field="mfma_f16")
+    t0 = time.perf_counter()
+    fin = []
+    for ta, p in reqs:  # one request at a time, in arrival order
+        while time.perf_counter() - t0 < ta:
+            pass
+        st = be.allocate_state(p, M.Synchronous())
+        cp = make_params(p, p.p)
+        check(be.lib.cah_advance(st.handle, p.tspan[0], p.tspan[1] - p.tspan[0], ctypes.byref(cp)))
+        be.current_state(st)
+        be.free_state(st)
+        fin.append(time.perf_counter() - t0)
+    plat = np.array([f - ta for f, (ta, _) in zip(fin, reqs)])
+    pmakespan = fin[-1] - reqs[0][0]
+    return {"workload": f"{n} requests, Poisson 200 req/s over 1.5 s, Life / forest fire / Lenia "
+                        "(MFMA FP16) in turn, sides log-uniform [128, 4096], tspan U[50, 500]",
+            "requests": n, "cell_updates": int(cells_steps),
+            "served": {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                       "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                       "sims_per_s": round(n / makespan, 1)},
+            "plain_fifo": {"p50_ms": round(float(np.percentile(plat, 50)) * 1e3, 2),
+                           "p99_ms": round(float(np.percentile(plat, 99)) * 1e3, 2),
+                           "sims_per_s": round(n / pmakespan, 1)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (Life 512^2, batched, Lenia, serving)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the sweep on a single MI355X "
@@ -175,6 +424,11 @@ def main():
     }
     be.free_state(s_ltl)
     be.free_state(s_for)
+    if not args.no_configs:
+        line["configs"] = {"life_512": bench_life512(be, ev),
+                           "batched_forest": bench_batched(be, ev),
+                           "lenia_2048": bench_lenia(be, ev),
+                           "serving": bench_serving()}
     if not args.no_cpu:
         n = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(n)

@@ -20,6 +20,7 @@ _ERRORS = {
     L.CAH_ERR_STATE_SHAPE: M.StateShapeError,
     L.CAH_ERR_INVALID_STATE: M.InvalidStateError,
     L.CAH_ERR_DEVICE: M.DeviceError,
+    L.CAH_ERR_BUSY: M.OutOfMemoryError,
 }
 
 

@@ -8,12 +8,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <set>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -992,5 +995,222 @@ extern "C" int cah_batch_stream(cah_batch* b, void** stream) {
 
 extern "C" int cah_batch_destroy(cah_batch* b) {
     free_batch(b);
+    return CAH_OK;
+}
+
+// ---------------------------------------------------------------- serving
+//
+// Requests wait in a FIFO until the device-memory budget admits them.  An
+// admitted request is an ordinary cah_state on its own stream; every tick
+// queues its next chunk of steps and records an event, so requests overlap on
+// the device and a request that arrives mid-run starts at the next tick that
+// finds memory for it.  Finished results stay on the device until fetched.
+
+static double steady_now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+extern "C" double cah_server_clock(void) { return steady_now(); }
+
+struct SrvReq {
+    uint64_t id = 0;
+    cah_desc d{};
+    std::vector<int32_t> offsets, taps;
+    std::vector<double> weights;
+    std::vector<uint8_t> mask;
+    std::vector<char> u0;
+    int64_t t0 = 0, t1 = 0, tcur = 0;
+    cah_params p{};
+    cah_state* st = nullptr;
+    hipEvent_t ev = nullptr;
+    bool inflight = false;
+    int32_t status = CAH_REQ_PENDING;
+    size_t bytes = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+struct cah_server {
+    cah_ctx* ctx = nullptr;
+    size_t budget = 0, used = 0;
+    int32_t chunk = 0;
+    std::deque<SrvReq*> pending;
+    std::vector<SrvReq*> active;
+    std::unordered_map<uint64_t, SrvReq*> all;
+    uint64_t next = 1;
+};
+
+// Device bytes a request holds while resident: both buffers, the mask and the
+// Lenia field buffers.
+static size_t request_bytes(const cah_desc& d) {
+    const int64_t n1 = d.dims[0], n2 = d.ndims == 2 ? d.dims[1] : 1;
+    const bool f = d.dtype == CAH_DTYPE_F32;
+    const int64_t pitch = f ? (n1 + 15) / 16 * 16 : (n1 + 63) / 64 * 64;
+    size_t b = 2 * (size_t)pitch * (size_t)n2 * (f ? 4 : 1);
+    if (d.mask) b += (size_t)(n1 * n2);
+    if (d.rule == CAH_RULE_LENIA) b += (size_t)(n1 * n2) * 16;  // field / spectra
+    return b;
+}
+
+static void free_req(SrvReq* r) {
+    if (r->ev) hipEventDestroy(r->ev);
+    if (r->st) free_state(r->st);
+    delete r;
+}
+
+extern "C" int cah_server_create(cah_ctx* ctx, size_t budget_bytes, int32_t chunk, cah_server** out) {
+    if (!ctx || !out) return fail(CAH_ERR_ARGUMENT, "cah_server_create: NULL argument");
+    if (chunk < 0) return fail(CAH_ERR_ARGUMENT, "chunk must be >= 0");
+    cah_server* s = new cah_server();
+    s->ctx = ctx;
+    s->budget = budget_bytes;
+    s->chunk = chunk;
+    *out = s;
+    return CAH_OK;
+}
+
+extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u0, size_t nbytes,
+                                 int64_t t0, int64_t t1, const cah_params* p, uint64_t* id) {
+    if (!s || !d || !id) return fail(CAH_ERR_ARGUMENT, "cah_server_submit: NULL argument");
+    if (int rc = cah_check_problem(d)) return rc;
+    if (t0 < 0 || t1 < t0) return fail(CAH_ERR_ARGUMENT, "need 0 <= t0 <= t1");
+    if (d->rule == CAH_RULE_FOREST_FIRE && !p)
+        return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
+    SrvReq* r = new SrvReq();
+    r->d = *d;
+    r->offsets.assign(d->offsets, d->offsets + 2 * (size_t)d->n_offsets);
+    r->d.offsets = r->offsets.data();
+    if (d->rule == CAH_RULE_LENIA) {
+        r->taps.assign(d->tap_offsets, d->tap_offsets + 2 * (size_t)d->n_taps);
+        r->weights.assign(d->tap_weights, d->tap_weights + d->n_taps);
+        r->d.tap_offsets = r->taps.data();
+        r->d.tap_weights = r->weights.data();
+    }
+    const int64_t cells = d->dims[0] * (d->ndims == 2 ? d->dims[1] : 1);
+    if (d->mask) {
+        r->mask.assign(d->mask, d->mask + cells);
+        r->d.mask = r->mask.data();
+    }
+    const size_t want = (size_t)cells * (d->dtype == CAH_DTYPE_F32 ? 4 : 1);
+    if (u0) {
+        if (nbytes != want) {
+            delete r;
+            return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", want, nbytes);
+        }
+        r->u0.assign((const char*)u0, (const char*)u0 + nbytes);
+    }
+    r->t0 = r->tcur = t0;
+    r->t1 = t1;
+    if (p) r->p = *p;
+    r->bytes = request_bytes(r->d);
+    if (r->bytes > s->budget) {
+        delete r;
+        return fail(CAH_ERR_BUSY, "the request needs %zu bytes, more than the whole budget", want);
+    }
+    r->id = s->next++;
+    r->t_submit = steady_now();
+    s->pending.push_back(r);
+    s->all[r->id] = r;
+    *id = r->id;
+    return CAH_OK;
+}
+
+static int admit(cah_server* s, SrvReq* r) {
+    cah_state* st = nullptr;
+    if (int rc = cah_state_create(s->ctx, &r->d, &st)) return rc;
+    r->st = st;
+    if (!r->u0.empty()) {
+        if (int rc = cah_state_set(st, r->u0.data(), r->u0.size())) return rc;
+        std::vector<char>().swap(r->u0);
+    }
+    HIPCHK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
+    r->status = CAH_REQ_ACTIVE;
+    r->t_admit = steady_now();
+    return CAH_OK;
+}
+
+extern "C" int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
+    HIPCHK(hipSetDevice(s->ctx->device));
+    // retire finished chunks and requests
+    for (size_t k = 0; k < s->active.size();) {
+        SrvReq* r = s->active[k];
+        if (r->inflight) {
+            const hipError_t q = hipEventQuery(r->ev);
+            if (q == hipSuccess) r->inflight = false;
+            else if (q != hipErrorNotReady) return fail(CAH_ERR_DEVICE, "request %llu: %s",
+                                                       (unsigned long long)r->id, hipGetErrorString(q));
+        }
+        if (!r->inflight && r->tcur == r->t1) {
+            r->status = CAH_REQ_DONE;
+            r->t_done = steady_now();
+            s->active[k] = s->active.back();
+            s->active.pop_back();
+        } else {
+            ++k;
+        }
+    }
+    // admit in arrival order while the budget allows
+    while (!s->pending.empty() && s->used + s->pending.front()->bytes <= s->budget) {
+        SrvReq* r = s->pending.front();
+        if (int rc = admit(s, r)) return rc;
+        s->pending.pop_front();
+        s->used += r->bytes;
+        s->active.push_back(r);
+    }
+    // queue the next chunk of every idle resident request
+    for (SrvReq* r : s->active) {
+        if (r->inflight || r->tcur == r->t1) continue;
+        const int64_t n = s->chunk > 0 ? std::min<int64_t>(s->chunk, r->t1 - r->tcur) : r->t1 - r->tcur;
+        if (int rc = cah_advance(r->st, r->tcur, n, &r->p)) return rc;
+        r->tcur += n;
+        HIPCHK(hipEventRecord(r->ev, r->st->stream));
+        r->inflight = true;
+    }
+    if (active) *active = (int32_t)s->active.size();
+    if (pending) *pending = (int32_t)s->pending.size();
+    return CAH_OK;
+}
+
+static int find_req(cah_server* s, uint64_t id, SrvReq** r) {
+    auto it = s ? s->all.find(id) : decltype(s->all.end()){};
+    if (!s || it == s->all.end()) return fail(CAH_ERR_ARGUMENT, "no request %llu", (unsigned long long)id);
+    *r = it->second;
+    return CAH_OK;
+}
+
+extern "C" int cah_server_status(cah_server* s, uint64_t id, int32_t* status) {
+    SrvReq* r;
+    if (int rc = find_req(s, id, &r)) return rc;
+    if (!status) return fail(CAH_ERR_ARGUMENT, "status is NULL");
+    *status = r->status;
+    return CAH_OK;
+}
+
+extern "C" int cah_server_fetch(cah_server* s, uint64_t id, void* host, size_t nbytes) {
+    SrvReq* r;
+    if (int rc = find_req(s, id, &r)) return rc;
+    if (r->status != CAH_REQ_DONE) return fail(CAH_ERR_BUSY, "request %llu is not finished", (unsigned long long)id);
+    if (int rc = cah_state_get(r->st, host, nbytes)) return rc;
+    free_state(r->st);
+    r->st = nullptr;
+    s->used -= r->bytes;
+    r->status = CAH_REQ_FETCHED;
+    return CAH_OK;
+}
+
+extern "C" int cah_server_times(cah_server* s, uint64_t id, double* t_submit, double* t_admit,
+                                double* t_done) {
+    SrvReq* r;
+    if (int rc = find_req(s, id, &r)) return rc;
+    if (t_submit) *t_submit = r->t_submit;
+    if (t_admit) *t_admit = r->t_admit;
+    if (t_done) *t_done = r->t_done;
+    return CAH_OK;
+}
+
+extern "C" int cah_server_destroy(cah_server* s) {
+    if (!s) return CAH_OK;
+    for (auto& kv : s->all) free_req(kv.second);
+    delete s;
     return CAH_OK;
 }

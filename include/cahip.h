@@ -220,6 +220,45 @@ int cah_batch_sync(cah_batch* b);
 int cah_batch_stream(cah_batch* b, void** stream);
 int cah_batch_destroy(cah_batch* b);
 
+/* ---------------------------------------------------------------- serving
+ * No single reference counterpart: a simulation server for the traffic that
+ * ensembles and abc_rejection generate (src/calibration.jl:66-91), with
+ * continuous batching.  Requests (any rule the backend runs, any lattice, any
+ * tspan) wait in a FIFO until the device-memory budget admits them; admitted
+ * lattices stay resident in HBM, each on its own HIP stream, and advance in
+ * chunks of steps, so requests that arrive while others run start as soon as
+ * memory frees up and their work overlaps on the device.  A request's result is
+ * exactly what a cah_state with the same descriptor would give.  One host
+ * thread drives a server (calls on one server must not overlap). */
+typedef struct cah_server cah_server;
+
+#define CAH_REQ_PENDING 0   /* waiting for memory            */
+#define CAH_REQ_ACTIVE  1   /* resident, steps in flight     */
+#define CAH_REQ_DONE    2   /* finished; result on the device */
+#define CAH_REQ_FETCHED 3   /* result copied out, memory freed */
+
+/* budget_bytes: device memory the resident lattices may hold; chunk: steps
+ * queued per request per tick (0 = whole tspan at once). */
+int cah_server_create(cah_ctx* ctx, size_t budget_bytes, int32_t chunk, cah_server** out);
+/* Queue a request: the problem (copied), its initial state (host, copied; NULL =
+ * all zero), steps t0 .. t1-1 and parameters.  Returns its id. */
+int cah_server_submit(cah_server* s, const cah_desc* d, const void* u0, size_t nbytes,
+                      int64_t t0, int64_t t1, const cah_params* p, uint64_t* id);
+/* One scheduling pass: retire finished chunks, admit waiting requests that fit
+ * the budget, queue the next chunk of every resident request.  Never blocks on
+ * the GPU.  *active / *pending (may be NULL) return the counts after the pass. */
+int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending);
+/* CAH_REQ_* of a request. */
+int cah_server_status(cah_server* s, uint64_t id, int32_t* status);
+/* Copy a finished request's result out and free its device memory. */
+int cah_server_fetch(cah_server* s, uint64_t id, void* host, size_t nbytes);
+/* Host timestamps (seconds, steady clock) of submission, admission and the tick
+ * that saw the request finish. */
+int cah_server_times(cah_server* s, uint64_t id, double* t_submit, double* t_admit, double* t_done);
+/* The steady clock the timestamps use, now. */
+double cah_server_clock(void);
+int cah_server_destroy(cah_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,72 @@
+"""The serving form: requests of mixed rules, lattices and tspans arriving while
+others run give, each, exactly the result of solving that problem alone."""
+import numpy as np
+import pytest
+
+from cahip import model as M
+from cahip.backend import HIPBackend
+from cahip.problem import CAProblem, final_state, solve
+from cahip.server import Server
+from helpers import ltl_rule
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_requests(n, seed=4):
+    rng = np.random.default_rng(seed)
+    probs = []
+    for i in range(n):
+        side = int(rng.integers(40, 300))
+        side2 = int(rng.integers(40, 300))
+        steps = int(rng.integers(5, 60))
+        kind = i % 4
+        if kind == 0:
+            u = rng.integers(0, 2, (side, side2)).astype(np.int8)
+            probs.append(CAProblem(M.Life(), u, steps, seed=i))
+        elif kind == 1:
+            u = rng.choice(np.array([0, 1, 2], np.int8), (side, side2))
+            probs.append(CAProblem(M.ForestFire(), u, (3, 3 + steps), {"p": 0.05, "f": 0.01},
+                                   seed=100 + i, space=M.Grid(side, side2, boundary=M.Reflecting())))
+        elif kind == 2:
+            u = np.zeros((side, side2), np.float32)
+            u[10:30, 10:30] = rng.random((20, 20)).astype(np.float32)
+            probs.append(CAProblem(M.Lenia(), u, min(steps, 20), seed=i))
+        else:
+            u = (rng.random((side, side2)) < 0.4).astype(np.int8)
+            probs.append(CAProblem(ltl_rule(), u, min(steps, 15), seed=i))
+    return probs
+
+
+@pytest.mark.parametrize("chunk", [0, 7])
+def test_mix_equals_alone(chunk):
+    probs = mixed_requests(24)
+    # a budget that holds only a few lattices at once, so requests queue
+    srv = Server(budget_bytes=3_000_000, chunk=chunk, lenia_field="mfma_f16")
+    ids = []
+    for k, p in enumerate(probs):  # requests arrive while earlier ones run
+        ids.append(srv.submit(p))
+        if k % 3 == 2:
+            srv.tick()
+    results = {}
+    for _ in range(100000):
+        srv.tick()
+        for rid in ids:
+            if rid not in results and srv.done(rid):
+                results[rid] = srv.fetch(rid)
+        if len(results) == len(ids):
+            break
+    assert len(results) == len(ids)
+    be = HIPBackend(lenia_field="mfma_f16")
+    for rid, p in zip(ids, probs):
+        alone = final_state(solve(p, backend=be, save_everystep=False))
+        assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs"
+        t_sub, t_adm, t_done = srv.times(rid)
+        assert t_sub <= t_adm <= t_done
+    srv.close()
+
+
+def test_budget_refuses_a_request_larger_than_it():
+    srv = Server(budget_bytes=10_000, chunk=8)
+    with pytest.raises(M.OutOfMemoryError):
+        srv.submit(CAProblem(M.Life(), np.zeros((200, 200), np.int8), 5))
+    srv.close()
