@@ -251,23 +251,24 @@ def bench_lenia(be, ev, steps=500):
             "methods": out}
 
 
-def serving_requests(duration_s=1.5, rate=200.0, seed=5):
-    """A Poisson stream (rate req/s over duration_s) of Life, forest-fire and Lenia
-    requests in turn; sides round(128 * 32^u) (log-uniform in [128, 4096]), tspan
-    uniform in [50, 500]; initial states: Life Bernoulli(0.5), forest all empty,
-    Lenia uniform in a central patch of side/8.  numpy default_rng(5)."""
+SERVING_MIX = ("Life B3/S23 Torus Bernoulli(0.5); forest fire VN(1) Reflecting all-empty "
+               "p=0.03 f=1e-4; Larger-than-Life Moore(5) B34..45/S34..58 Torus Bernoulli(0.5); "
+               "Lenia R=13 Torus (MFMA FP16) uniform central side/8 patch; in turn; sides "
+               "round(64 * 16^u) (log-uniform in [64, 1024]) square; tspan (0, U{50..300})")
+
+
+def serving_problems(n, seed=5):
+    """n requests of the mix SERVING_MIX: the kind of traffic ensembles and ABC
+    rejection generate (many independent small problems).  numpy default_rng(seed)."""
     from cahip import model as M
     from cahip.problem import CAProblem
     rng = np.random.default_rng(seed)
-    t, reqs = 0.0, []
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration_s:
-            break
-        i = len(reqs)
-        side = int(round(128 * 32 ** rng.random()))
-        steps = int(rng.integers(50, 501))
-        kind = i % 3
+    ltl = M.Life(range(34, 46), range(34, 59), neighbourhood=M.Moore(5))
+    out = []
+    for i in range(n):
+        side = int(round(64 * 16 ** rng.random()))
+        steps = int(rng.integers(50, 301))
+        kind = i % 4
         if kind == 0:
             u = rng.integers(0, 2, (side, side), dtype=np.int8)
             p = CAProblem(M.Life("conway"), u, steps, seed=i)
@@ -275,72 +276,111 @@ def serving_requests(duration_s=1.5, rate=200.0, seed=5):
             p = CAProblem(M.ForestFire(), np.zeros((side, side), np.int8), steps,
                           {"p": 0.03, "f": 1e-4}, seed=1000 + i,
                           space=M.Grid(side, side, boundary=M.Reflecting()))
+        elif kind == 2:
+            u = rng.integers(0, 2, (side, side), dtype=np.int8)
+            p = CAProblem(ltl, u, steps, seed=i)
         else:
             u = np.zeros((side, side), np.float32)
             k = max(side // 8, 1)
             lo = (side - k) // 2
             u[lo:lo + k, lo:lo + k] = rng.random((k, k), dtype=np.float32)
             p = CAProblem(M.Lenia(), u, steps, seed=i)
-        reqs.append((t, p))
-    return reqs
+        out.append(p)
+    return out
 
 
-def bench_serving(reqs=None):
-    """Continuous batching (cah_server_*: admission by a 200 GB budget, resident
-    lattices, 64-step chunks) against the plain FIFO loop that solves one request
-    at a time; latency = arrival to result ready."""
-    from cahip import model as M
-    from cahip.backend import HIPBackend, check, make_params
+def poisson_arrivals(n, rate, seed=6):
+    return np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
+
+
+def serve(probs, arrivals, chunk=32):
+    """The server (cah_server_*) fed in arrival order; per request the time its
+    result reached host memory, and the results."""
     from cahip.server import Server
-    reqs = serving_requests() if reqs is None else reqs
-    n = len(reqs)
-    cells_steps = sum(p.u0.size * (p.tspan[1] - p.tspan[0]) for _, p in reqs)
-    srv = Server(budget_bytes=200e9, chunk=64, lenia_field="mfma_f16")
-    # warm-up: one request of each kind
-    for _, p in reqs[:3]:
+    srv = Server(budget_bytes=100e9, chunk=chunk, lenia_field="mfma_f16")
+    for p in probs[:4]:  # warm-up: one request of each kind
         rid = srv.submit(p)
         while not srv.done(rid):
             srv.tick()
         srv.fetch(rid)
-    start = srv.clock()
-    ids, k, done = [], 0, {}
-    while len(done) < n:
-        now = srv.clock() - start
-        while k < n and reqs[k][0] <= now:
-            ids.append(srv.submit(reqs[k][1]))
+    n = len(probs)
+    ids, k, fin, res, live = [None] * n, 0, [0.0] * n, [None] * n, set()
+    start = time.perf_counter()
+    while k < n or live:
+        now = time.perf_counter() - start
+        while k < n and arrivals[k] <= now:
+            ids[k] = srv.submit(probs[k])
+            live.add(k)
             k += 1
         srv.tick()
-        for j, rid in enumerate(ids):
-            if j not in done and srv.done(rid):
-                srv.fetch(rid)
-                done[j] = srv.times(rid)[2] - start
-    lat = np.array([done[j] - reqs[j][0] for j in range(n)])
-    makespan = max(done.values()) - reqs[0][0]
+        for j in list(live):
+            if srv.done(ids[j]):
+                res[j] = srv.fetch(ids[j])
+                fin[j] = time.perf_counter() - start
+                live.discard(j)
     srv.close()
+    return np.array(fin), res
 
+
+def plain(probs, arrivals):
+    """The plain per-problem GPU sweep: one request at a time in arrival order,
+    each allocated, advanced, read back and freed (what solve() does)."""
+    from cahip import model as M
+    from cahip.backend import HIPBackend, check, make_params
     be = HIPBackend(0, lenia_field="mfma_f16")
-    t0 = time.perf_counter()
-    fin = []
-    for ta, p in reqs:  # one request at a time, in arrival order
-        while time.perf_counter() - t0 < ta:
-            pass
+
+    def one(p):
         st = be.allocate_state(p, M.Synchronous())
         cp = make_params(p, p.p)
         check(be.lib.cah_advance(st.handle, p.tspan[0], p.tspan[1] - p.tspan[0], ctypes.byref(cp)))
-        be.current_state(st)
+        u = be.current_state(st)
         be.free_state(st)
-        fin.append(time.perf_counter() - t0)
-    plat = np.array([f - ta for f, (ta, _) in zip(fin, reqs)])
-    pmakespan = fin[-1] - reqs[0][0]
-    return {"workload": f"{n} requests, Poisson 200 req/s over 1.5 s, Life / forest fire / Lenia "
-                        "(MFMA FP16) in turn, sides log-uniform [128, 4096], tspan U[50, 500]",
-            "requests": n, "cell_updates": int(cells_steps),
-            "served": {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-                       "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-                       "sims_per_s": round(n / makespan, 1)},
-            "plain_fifo": {"p50_ms": round(float(np.percentile(plat, 50)) * 1e3, 2),
-                           "p99_ms": round(float(np.percentile(plat, 99)) * 1e3, 2),
-                           "sims_per_s": round(n / pmakespan, 1)}}
+        return u
+
+    for p in probs[:4]:
+        one(p)
+    fin, res = [], []
+    start = time.perf_counter()
+    for ta, p in zip(arrivals, probs):
+        while time.perf_counter() - start < ta:
+            pass
+        res.append(one(p))
+        fin.append(time.perf_counter() - start)
+    return np.array(fin), res
+
+
+def bench_serving():
+    """Continuous batching (cah_server_*: 100 GB budget, resident lattices, 32-step
+    ticks, one batched launch per step per kernel family) against the plain
+    per-problem sweep.  Capacity: 400 requests all present at t = 0, sims/s to the
+    last result, and whether every result equals the plain sweep's.  Latency:
+    arrival to result in host memory under Poisson arrivals, at a light load both
+    sustain and at a heavy one."""
+    probs = serving_problems(400)
+    cu = sum(p.u0.size * (p.tspan[1] - p.tspan[0]) for p in probs)
+    zero = np.zeros(len(probs))
+    fs, rs = serve(probs, zero)
+    fp, rp = plain(probs, zero)
+    same = all(np.array_equal(a, b) for a, b in zip(rs, rp))
+    out = {"workload": SERVING_MIX,
+           "capacity": {"requests": len(probs), "cell_updates": int(cu),
+                        "served_sims_per_s": round(len(probs) / fs.max(), 1),
+                        "plain_sims_per_s": round(len(probs) / fp.max(), 1),
+                        "ratio": round(fp.max() / fs.max(), 2), "identical_results": bool(same)}}
+    for name, rate, n, seed in (("light", 150.0, 300, 7), ("heavy", 1000.0, 1000, 8)):
+        probs = serving_problems(n, seed=seed)
+        arr = poisson_arrivals(n, rate, seed)
+        fs, _ = serve(probs, arr)
+        fp, _ = plain(probs, arr)
+        ls, lp = (fs - arr) * 1e3, (fp - arr) * 1e3
+        out[name] = {"rate_per_s": rate, "requests": n,
+                     "served": {"p50_ms": round(float(np.percentile(ls, 50)), 2),
+                                "p99_ms": round(float(np.percentile(ls, 99)), 2),
+                                "sims_per_s": round(n / fs.max(), 1)},
+                     "plain": {"p50_ms": round(float(np.percentile(lp, 50)), 2),
+                               "p99_ms": round(float(np.percentile(lp, 99)), 2),
+                               "sims_per_s": round(n / fp.max(), 1)}}
+    return out
 
 
 def main():

@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <mutex>
 #include <set>
 #include <string>
 #include <unordered_map>
@@ -211,6 +213,10 @@ struct cah_state {
     bool bin_clean = false;  // every cell is 0 or 1 (Life only)
     LifeArgs life{};
     int32_t tune_vb = 0, tune_rows = 0;
+    struct Arena* arena = nullptr;        // device memory source (nullptr: hipMalloc)
+    struct Arena* staging = nullptr;      // pinned host memory of a server's uploads
+    bool own_stream = true;               // false: the stream belongs to a server
+    std::vector<void*> keep;              // pinned sources of queued uploads
 };
 
 static Lat lat_of(const cah_state* s) {
@@ -269,15 +275,115 @@ static int intervals(uint64_t m, uint32_t* lo, uint32_t* hi) {
     return k;
 }
 
+// ---------------------------------------------------------------- device memory
+//
+// A state's buffers come from hipMalloc, or, for the states a server owns, from
+// the server's arena: slabs taken with hipMalloc and carved first-fit in 256-byte
+// units, so that admitting and retiring requests neither allocates nor frees
+// device memory (hipFree waits for the whole device to go idle).  A block is
+// only handed out again after every launch that used it has completed; the
+// server guarantees that by releasing a request's blocks after its result left.
+// A pinned arena does the same with page-locked host memory (hipHostMalloc), the
+// staging area of the server's host <-> device copies.
+
+struct Arena {
+    bool pinned = false;
+    struct Blk {
+        size_t size;
+        int slab;
+    };
+    size_t slab_min = (size_t)256 << 20;
+    std::vector<char*> slabs;
+    std::map<char*, Blk> free_;
+    std::unordered_map<char*, Blk> used;
+    size_t in_use = 0;
+
+    void* alloc(size_t n) {
+        n = std::max<size_t>(256, (n + 255) / 256 * 256);
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            for (auto it = free_.begin(); it != free_.end(); ++it) {
+                if (it->second.size < n) continue;
+                char* p = it->first;
+                const Blk b = it->second;
+                free_.erase(it);
+                if (b.size > n) free_[p + n] = Blk{b.size - n, b.slab};
+                used[p] = Blk{n, b.slab};
+                in_use += n;
+                return p;
+            }
+            const size_t sz = std::max(slab_min, n);
+            char* slab = nullptr;
+            const hipError_t e = pinned ? hipHostMalloc((void**)&slab, sz, hipHostMallocDefault)
+                                        : hipMalloc((void**)&slab, sz);
+            if (e != hipSuccess) return nullptr;
+            slabs.push_back(slab);
+            free_[slab] = Blk{sz, (int)slabs.size() - 1};
+        }
+        return nullptr;
+    }
+
+    void release(void* vp) {
+        char* p = (char*)vp;
+        auto u = used.find(p);
+        if (u == used.end()) return;
+        Blk b = u->second;
+        used.erase(u);
+        in_use -= b.size;
+        auto nx = free_.find(p + b.size);
+        if (nx != free_.end() && nx->second.slab == b.slab) {
+            b.size += nx->second.size;
+            free_.erase(nx);
+        }
+        auto it = free_.emplace(p, b).first;
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second.size == p && pv->second.slab == b.slab) {
+                pv->second.size += b.size;
+                free_.erase(it);
+            }
+        }
+    }
+
+    ~Arena() {
+        for (char* s : slabs) {
+            if (pinned) hipHostFree(s);
+            else hipFree(s);
+        }
+    }
+};
+
+static int dev_alloc(cah_state* s, void** p, size_t n) {
+    if (s->arena) {
+        *p = s->arena->alloc(n);
+        if (!*p) return fail(CAH_ERR_OUT_OF_MEMORY, "the device arena cannot hold %zu more bytes", n);
+        return CAH_OK;
+    }
+    HIPCHK(hipMalloc(p, std::max<size_t>(n, 1)));
+    return CAH_OK;
+}
+
+static void dev_free(cah_state* s, void* p) {
+    if (!p) return;
+    if (s->arena) s->arena->release(p);
+    else hipFree(p);
+}
+
+// The pinned sources of a server state's uploads, once those have completed.
+static void release_staging(cah_state* s) {
+    for (void* p : s->keep) s->staging->release(p);
+    s->keep.clear();
+}
+
 static void free_state(cah_state* s) {
     if (!s) return;
     hipSetDevice(s->device);
-    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->staging) release_staging(s);
+    if (s->stream && !s->arena) hipStreamSynchronize(s->stream);
     for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
                     (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d, s->frag_d})
-        if (p) hipFree(p);
+        dev_free(s, p);
     lenia_fft_destroy(s->fft);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
     delete s;
 }
 
@@ -288,10 +394,45 @@ static int upload(T** dst, const T* src, size_t n) {
     return CAH_OK;
 }
 
-static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s) {
+// A pinned copy of n host bytes that stays valid until release_staging.
+static int stage(cah_state* s, const void* src, size_t n, void** out) {
+    *out = s->staging->alloc(n);
+    if (!*out) return fail(CAH_ERR_OUT_OF_MEMORY, "cannot stage %zu bytes in pinned memory", n);
+    std::memcpy(*out, src, n);
+    s->keep.push_back(*out);
+    return CAH_OK;
+}
+
+// A state's device copy of n host values: synchronous for a state of its own,
+// queued on the state's stream from pinned memory for a server's.
+template <typename T>
+static int upload(cah_state* s, T** dst, const T* src, size_t n) {
+    if (int rc = dev_alloc(s, (void**)dst, std::max<size_t>(n, 1) * sizeof(T))) return rc;
+    if (!n) return CAH_OK;
+    if (s->staging) {
+        void* h;
+        if (int rc = stage(s, src, n * sizeof(T), &h)) return rc;
+        HIPCHK(hipMemcpyAsync(*dst, h, n * sizeof(T), hipMemcpyHostToDevice, s->stream));
+    } else {
+        HIPCHK(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    }
+    return CAH_OK;
+}
+
+// arena, staging and stream non-null: a server's state, on the server's stream
+// and memory
+static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* arena = nullptr,
+                        Arena* staging = nullptr, hipStream_t stream = nullptr) {
     s->device = ctx->device;
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->arena = arena;
+    s->staging = staging;
+    if (stream) {
+        s->stream = stream;
+        s->own_stream = false;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    }
     s->d = *d;
     s->n1 = d->dims[0];
     s->n2 = d->ndims == 2 ? d->dims[1] : 1;
@@ -308,22 +449,22 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s) {
     s->d.mask = nullptr;
     const size_t bytes = (size_t)s->pitch * (size_t)s->n2 * s->es;
     for (int b = 0; b < 2; ++b) {
-        HIPCHK(hipMalloc(&s->buf[b], bytes));
+        if (int rc = dev_alloc(s, &s->buf[b], bytes)) return rc;
         HIPCHK(hipMemsetAsync(s->buf[b], 0, bytes, s->stream));
     }
     if (d->mask) {
-        int rc = upload(&s->mask_d, d->mask, (size_t)(s->n1 * s->n2));
+        int rc = upload(s, &s->mask_d, d->mask, (size_t)(s->n1 * s->n2));
         if (rc) return rc;
     }
-    if (int rc = upload(&s->ofs_d, s->offsets.data(), s->offsets.size())) return rc;
+    if (int rc = upload(s, &s->ofs_d, s->offsets.data(), s->offsets.size())) return rc;
     s->halo = std::max(1, max_abs(s->offsets.data(), d->n_offsets));
 
     if (d->rule == CAH_RULE_LENIA) {
         s->halo = std::max(s->halo, max_abs(s->taps.data(), d->n_taps));
-        if (int rc = upload(&s->taps_d, s->taps.data(), s->taps.size())) return rc;
-        if (int rc = upload(&s->w64_d, s->weights.data(), s->weights.size())) return rc;
+        if (int rc = upload(s, &s->taps_d, s->taps.data(), s->taps.size())) return rc;
+        if (int rc = upload(s, &s->w64_d, s->weights.data(), s->weights.size())) return rc;
         std::vector<float> w32(s->weights.begin(), s->weights.end());
-        if (int rc = upload(&s->w32_d, w32.data(), w32.size())) return rc;
+        if (int rc = upload(s, &s->w32_d, w32.data(), w32.size())) return rc;
         const int ht = max_abs(s->taps.data(), d->n_taps);
         s->lenia_H = ht <= 4 ? 4 : ht <= 8 ? 8 : ht <= 12 ? 12 : 16;
         const int K = 2 * s->lenia_H + 1;
@@ -331,7 +472,7 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s) {
         for (int k = 0; k < d->n_taps; ++k)
             tab[(size_t)(s->taps[2 * k + 1] + s->lenia_H) * K + (s->taps[2 * k] + s->lenia_H)] +=
                 (float)s->weights[k];
-        if (int rc = upload(&s->wtab_d, tab.data(), tab.size())) return rc;
+        if (int rc = upload(s, &s->wtab_d, tab.data(), tab.size())) return rc;
         // fused field + growth kernels need the field of the step-start state for
         // every cell they update, so Checkerboard (two half-sweeps reading the
         // same field) runs on the separate field kernels
@@ -353,14 +494,15 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s) {
             for (int k = 0; k < d->n_taps; ++k)
                 wd[(size_t)(s->taps[2 * k + 1] + ht) * Kd + (s->taps[2 * k] + ht)] += s->weights[k];
             s->lenia_H = ht;
-            if (int rc = lenia_mfma_fragments(s->lenia_mode, ht, wd.data(), &s->frag_d))
-                return fail(rc, "cannot upload the MFMA fragments");
-            return CAH_OK;
+            std::vector<char> frag;
+            lenia_mfma_fragments(s->lenia_mode, ht, wd.data(), frag);
+            return upload(s, (char**)&s->frag_d, frag.data(), frag.size());
         }
         const bool fused = fm == CAH_FIELD_DIRECT_F32 && !cb;
         s->path = fused ? P_LENIA_F32COL : P_LENIA_SEQ;
         if (!fused)
-            HIPCHK(hipMalloc(&s->field_d, (size_t)(s->n1 * s->n2) * (d->lenia_precision == 32 ? 4 : 8)));
+            if (int rc = dev_alloc(s, &s->field_d, (size_t)(s->n1 * s->n2) * (d->lenia_precision == 32 ? 4 : 8)))
+                return rc;
         return CAH_OK;
     }
 
@@ -422,14 +564,98 @@ static bool scan_binary(const int8_t* h, size_t n) {
     return true;
 }
 
+// Rows of rb bytes from a pitched to a dense layout or back, on a stream.
+static int copy_rows(const void* src, int64_t sp, void* dst, int64_t dp, int64_t rb, int64_t rows,
+                     hipStream_t st) {
+    const uint64_t a = (uint64_t)(uintptr_t)src | (uint64_t)(uintptr_t)dst | (uint64_t)sp |
+                       (uint64_t)dp | (uint64_t)rb;
+    const int W = a % 16 == 0 ? 16 : a % 4 == 0 ? 4 : 1;
+    const int64_t n = rb / W * rows;
+    if (n == 0) return CAH_OK;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 16384);
+    const uint8_t* s8 = (const uint8_t*)src;
+    uint8_t* d8 = (uint8_t*)dst;
+    if (W == 16) k_copy_rows<16><<<blocks, 256, 0, st>>>(s8, sp, d8, dp, rb, rows);
+    else if (W == 4) k_copy_rows<4><<<blocks, 256, 0, st>>>(s8, sp, d8, dp, rb, rows);
+    else k_copy_rows<1><<<blocks, 256, 0, st>>>(s8, sp, d8, dp, rb, rows);
+    HIPCHK(hipGetLastError());
+    return CAH_OK;
+}
+
+// Host <-> device copies of a state's contents move the dense form (n1 * es
+// bytes a row) with linear copies through one pinned staging buffer, in pieces
+// of whole rows of at most STAGE_MAX bytes; the pitched form is made on the
+// device from the dense one in the state's other buffer (between steps it holds
+// nothing, and no kernel reads the pitch padding).  A pitched copy from or to
+// pageable memory is staged row by row by the runtime, and the DMA engines split
+// one whose rows are not 4-byte multiples into a transfer per row: milliseconds
+// for a lattice of a thousand rows.
+constexpr size_t STAGE_MAX = (size_t)64 << 20;
+static std::mutex g_stage_mu;
+static char* g_stage = nullptr;
+static size_t g_stage_cap = 0;
+
+static int stage_rows(const cah_state* s, int64_t* rows_per) {
+    const size_t row = (size_t)s->n1 * s->es;
+    *rows_per = std::max<int64_t>(1, (int64_t)(STAGE_MAX / row));
+    const size_t need = row * (size_t)std::min<int64_t>(*rows_per, s->n2);
+    if (need > g_stage_cap) {
+        if (g_stage) hipHostFree(g_stage);
+        g_stage = nullptr;
+        g_stage_cap = 0;
+        HIPCHK(hipHostMalloc((void**)&g_stage, need, hipHostMallocDefault));
+        g_stage_cap = need;
+    }
+    return CAH_OK;
+}
+
+static int copy_in(cah_state* s, const void* host) {
+    const size_t row = (size_t)s->n1 * s->es, pitchb = (size_t)s->pitch * s->es;
+    char* dense = (char*)(row == pitchb ? s->buf[s->cur] : s->buf[1 - s->cur]);
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    int64_t per;
+    if (int rc = stage_rows(s, &per)) return rc;
+    for (int64_t y = 0; y < s->n2; y += per) {
+        const size_t nb = (size_t)std::min<int64_t>(per, s->n2 - y) * row;
+        std::memcpy(g_stage, (const char*)host + (size_t)y * row, nb);
+        HIPCHK(hipMemcpyAsync(dense + (size_t)y * row, g_stage, nb, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    if (row != pitchb)
+        if (int rc = copy_rows(dense, (int64_t)row, s->buf[s->cur], (int64_t)pitchb, (int64_t)row, s->n2,
+                               s->stream))
+            return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+static int copy_out(cah_state* s, void* host) {
+    const size_t row = (size_t)s->n1 * s->es, pitchb = (size_t)s->pitch * s->es;
+    const char* dense = (const char*)s->buf[s->cur];
+    if (row != pitchb) {
+        dense = (const char*)s->buf[1 - s->cur];
+        if (int rc = copy_rows(s->buf[s->cur], (int64_t)pitchb, s->buf[1 - s->cur], (int64_t)row,
+                               (int64_t)row, s->n2, s->stream))
+            return rc;
+    }
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    int64_t per;
+    if (int rc = stage_rows(s, &per)) return rc;
+    for (int64_t y = 0; y < s->n2; y += per) {
+        const size_t nb = (size_t)std::min<int64_t>(per, s->n2 - y) * row;
+        HIPCHK(hipMemcpyAsync(g_stage, dense + (size_t)y * row, nb, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+        std::memcpy((char*)host + (size_t)y * row, g_stage, nb);
+    }
+    return CAH_OK;
+}
+
 extern "C" int cah_state_set(cah_state* s, const void* host, size_t nbytes) {
     if (!s || !host) return fail(CAH_ERR_ARGUMENT, "cah_state_set: NULL argument");
     if (nbytes != state_bytes(s))
         return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpy2DAsync(s->buf[s->cur], (size_t)s->pitch * s->es, host, (size_t)s->n1 * s->es,
-                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
+    if (int rc = copy_in(s, host)) return rc;
     if (s->es == 1) s->bin_clean = scan_binary((const int8_t*)host, nbytes);
     return CAH_OK;
 }
@@ -439,10 +665,7 @@ extern "C" int cah_state_get(cah_state* s, void* host, size_t nbytes) {
     if (nbytes != state_bytes(s))
         return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpy2DAsync(host, (size_t)s->n1 * s->es, s->buf[s->cur], (size_t)s->pitch * s->es,
-                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return CAH_OK;
+    return copy_out(s, host);
 }
 
 extern "C" int cah_state_set_device(cah_state* s, const void* dptr, size_t nbytes) {
@@ -450,8 +673,9 @@ extern "C" int cah_state_set_device(cah_state* s, const void* dptr, size_t nbyte
     if (nbytes != state_bytes(s))
         return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpy2DAsync(s->buf[s->cur], (size_t)s->pitch * s->es, dptr, (size_t)s->n1 * s->es,
-                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyDeviceToDevice, s->stream));
+    if (int rc = copy_rows(dptr, (int64_t)(s->n1 * s->es), s->buf[s->cur], (int64_t)(s->pitch * s->es),
+                           (int64_t)(s->n1 * s->es), s->n2, s->stream))
+        return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->es == 1) s->bin_clean = false;  // not scanned
     return CAH_OK;
@@ -462,8 +686,9 @@ extern "C" int cah_state_get_device(cah_state* s, void* dptr, size_t nbytes) {
     if (nbytes != state_bytes(s))
         return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
     HIPCHK(hipSetDevice(s->device));
-    HIPCHK(hipMemcpy2DAsync(dptr, (size_t)s->n1 * s->es, s->buf[s->cur], (size_t)s->pitch * s->es,
-                            (size_t)s->n1 * s->es, (size_t)s->n2, hipMemcpyDeviceToDevice, s->stream));
+    if (int rc = copy_rows(s->buf[s->cur], (int64_t)(s->pitch * s->es), dptr, (int64_t)(s->n1 * s->es),
+                           (int64_t)(s->n1 * s->es), s->n2, s->stream))
+        return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     return CAH_OK;
 }
@@ -782,6 +1007,9 @@ struct cah_batch {
     int32_t* wave_item_d = nullptr;
     int32_t* wave_base_d = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<ForestArgs> fitems;  // host copy of the forest items (p and f change)
+    bool fp_set = false;
+    cah_params fp{};
 };
 
 static void free_batch(cah_batch* b) {
@@ -839,6 +1067,7 @@ static int batch_layout(cah_batch* b) {
     }
     b->nwaves = (int32_t)wave_item.size();
     if (b->kind == B_FOREST) {
+        b->fitems = fitems;
         if (int rc = upload((ForestArgs**)&b->items_d, fitems.data(), fitems.size())) return rc;
     } else {
         if (int rc = upload((LifeArgs**)&b->items_d, litems.data(), litems.size())) return rc;
@@ -920,12 +1149,13 @@ extern "C" int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, 
     return cah_state_fill_bernoulli(s, seed, p);
 }
 
-static void launch_batch_step(cah_batch* b, int64_t t, const cah_params* p, bool bin) {
+static int forest_prec(const cah_params& p) { return (p.ff_p_is_f32 ? 1 : 0) | (p.ff_f_is_f32 ? 2 : 0); }
+
+static void launch_batch_step(cah_batch* b, int64_t t, bool bin) {
     const int blocks = waves_to_blocks(b->nwaves);
     if (b->kind == B_FOREST)
-        launch_forest_batch(b->vb, b->small, blocks, (const ForestArgs*)b->items_d, b->wave_item_d,
-                            b->wave_base_d, b->nwaves, b->parity, t, p->ff_p, p->ff_f,
-                            p->ff_p_is_f32, p->ff_f_is_f32, b->stream);
+        launch_forest_batch(b->small, forest_prec(b->fp), blocks, (const ForestArgs*)b->items_d,
+                            b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, b->stream);
     else
         launch_life_batch(b->life_R, b->vb, bin, b->niv, blocks, (const LifeArgs*)b->items_d,
                           b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, b->stream);
@@ -946,6 +1176,23 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
         return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
     // members may have been set through their own streams
     for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
+    if (b->kind == B_FOREST &&
+        (!b->fp_set || b->fp.ff_p != p->ff_p || b->fp.ff_f != p->ff_f ||
+         b->fp.ff_p_is_f32 != p->ff_p_is_f32 || b->fp.ff_f_is_f32 != p->ff_f_is_f32)) {
+        // the items carry p and f; the host copy is rewritten only once no
+        // queued copy still reads it
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (ForestArgs& a : b->fitems) {
+            a.p = p->ff_p;
+            a.f = p->ff_f;
+            a.p_f32 = p->ff_p_is_f32;
+            a.f_f32 = p->ff_f_is_f32;
+        }
+        HIPCHK(hipMemcpyAsync(b->items_d, b->fitems.data(), b->fitems.size() * sizeof(ForestArgs),
+                              hipMemcpyHostToDevice, b->stream));
+        b->fp = *p;
+        b->fp_set = true;
+    }
     bool bin = true;
     for (auto* s : b->members) {
         const bool e1 = s->d.boundary[0] != CAH_BOUNDARY_CONSTANT ||
@@ -962,13 +1209,13 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
         }
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, p, bin);
+        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, bin);
         HIPCHK(hipStreamEndCapture(b->stream, &graph));
         HIPCHK(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(graph));
         HIPCHK(hipGraphLaunch(b->exec, b->stream));
     } else {
-        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, p, bin);
+        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, bin);
     }
     HIPCHK(hipGetLastError());
     for (auto* s : b->members) {
@@ -1000,17 +1247,30 @@ extern "C" int cah_batch_destroy(cah_batch* b) {
 
 // ---------------------------------------------------------------- serving
 //
-// Requests wait in a FIFO until the device-memory budget admits them.  An
-// admitted request is an ordinary cah_state on its own stream; every tick
-// queues its next chunk of steps and records an event, so requests overlap on
-// the device and a request that arrives mid-run starts at the next tick that
-// finds memory for it.  Finished results stay on the device until fetched.
+// Continuous batching.  Requests wait in a FIFO until the device-memory budget
+// admits them; an admitted request is a state whose buffers come from the
+// server's arena.  Every tick queues the next chunk of steps of every resident
+// request: requests of one kernel family (Larger-than-Life of one radius and
+// interval count, the forest fire of one precision, MFMA Lenia of one mode)
+// advance together, one batched launch per step over all their lattices, on the
+// family's own stream, so the families overlap on the device; the others run
+// their own launches on a fourth stream.  A request that arrives mid-run joins
+// the next tick.  At most two ticks are in flight, so the device never waits for
+// the host between them and a new request waits at most about one tick.  The
+// per-tick item and wave tables are written to pinned host memory and copied in
+// one transfer on the upload stream, which also carries the admitted states'
+// initial contents; every family stream waits for that copy.  Results stay on the
+// device until fetched, through a separate copy stream.
 
 static double steady_now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 extern "C" double cah_server_clock(void) { return steady_now(); }
+
+enum SrvClass { SC_LIFE = 0, SC_FOREST = 1, SC_LENIA = 2, SC_LOOP = 3 };
+constexpr int SRV_STREAMS = 4;
+constexpr int SRV_SLOTS = 2;
 
 struct SrvReq {
     uint64_t id = 0;
@@ -1019,51 +1279,115 @@ struct SrvReq {
     std::vector<double> weights;
     std::vector<uint8_t> mask;
     std::vector<char> u0;
-    int64_t t0 = 0, t1 = 0, tcur = 0;
+    int64_t t0 = 0, t1 = 0, queued = 0;  // queued: the time its queued steps reach
     cah_params p{};
     cah_state* st = nullptr;
-    hipEvent_t ev = nullptr;
-    bool inflight = false;
+    int cls = SC_LOOP, key = 0;          // family and kernel variant
+    bool tail_queued = false;            // its last step is in a queued tick
     int32_t status = CAH_REQ_PENDING;
     size_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
+    int32_t steps = 0;                   // steps in the tick being built
+};
+
+struct SrvSlot {
+    hipEvent_t up = nullptr;
+    hipEvent_t done[SRV_STREAMS] = {};
+    char* host = nullptr;  // pinned
+    size_t host_cap = 0;
+    char* dev = nullptr;
+    size_t dev_cap = 0;
+    std::vector<SrvReq*> finishing;
+    bool busy = false;
 };
 
 struct cah_server {
     cah_ctx* ctx = nullptr;
     size_t budget = 0, used = 0;
     int32_t chunk = 0;
+    Arena arena;   // device memory of the resident lattices and tick tables
+    Arena pinned;  // staging of initial states and results
+    hipStream_t up = nullptr, copy = nullptr;
+    hipStream_t st[SRV_STREAMS] = {};
+    SrvSlot slot[SRV_SLOTS];
+    int oldest = 0, nbusy = 0;
     std::deque<SrvReq*> pending;
-    std::vector<SrvReq*> active;
+    std::vector<SrvReq*> active;  // admitted and not yet done
     std::unordered_map<uint64_t, SrvReq*> all;
     uint64_t next = 1;
 };
 
-// Device bytes a request holds while resident: both buffers, the mask and the
-// Lenia field buffers.
+// Device bytes a request holds while resident: both buffers, the mask, and the
+// Lenia field or spectra of the paths that keep one.
 static size_t request_bytes(const cah_desc& d) {
     const int64_t n1 = d.dims[0], n2 = d.ndims == 2 ? d.dims[1] : 1;
     const bool f = d.dtype == CAH_DTYPE_F32;
     const int64_t pitch = f ? (n1 + 15) / 16 * 16 : (n1 + 63) / 64 * 64;
     size_t b = 2 * (size_t)pitch * (size_t)n2 * (f ? 4 : 1);
     if (d.mask) b += (size_t)(n1 * n2);
-    if (d.rule == CAH_RULE_LENIA) b += (size_t)(n1 * n2) * 16;  // field / spectra
+    if (d.rule == CAH_RULE_LENIA) {
+        const int fm = d.field_method;
+        const bool mfma = fm == CAH_FIELD_MFMA_F16 || fm == CAH_FIELD_MFMA_BF16 || fm == CAH_FIELD_MFMA_F16X2;
+        b += (size_t)(n1 * n2) * (fm == CAH_FIELD_FFT_F32 ? 16 : mfma ? 0 : 8);
+        b += (size_t)1 << 20;  // taps, weights, tables, fragments
+    }
     return b;
 }
 
-static void free_req(SrvReq* r) {
-    if (r->ev) hipEventDestroy(r->ev);
-    if (r->st) free_state(r->st);
-    delete r;
+static void free_slot(SrvSlot& sl) {
+    if (sl.up) hipEventDestroy(sl.up);
+    for (hipEvent_t e : sl.done)
+        if (e) hipEventDestroy(e);
+    if (sl.host) hipHostFree(sl.host);
+}
+
+extern "C" int cah_server_destroy(cah_server* s) {
+    if (!s) return CAH_OK;
+    hipSetDevice(s->ctx->device);
+    for (hipStream_t q : s->st)
+        if (q) hipStreamSynchronize(q);
+    if (s->up) hipStreamSynchronize(s->up);
+    if (s->copy) hipStreamSynchronize(s->copy);
+    for (auto& kv : s->all) {
+        if (kv.second->st) free_state(kv.second->st);
+        delete kv.second;
+    }
+    for (SrvSlot& sl : s->slot) free_slot(sl);
+    for (hipStream_t q : s->st)
+        if (q) hipStreamDestroy(q);
+    if (s->up) hipStreamDestroy(s->up);
+    if (s->copy) hipStreamDestroy(s->copy);
+    delete s;  // the arena's slabs go with it
+    return CAH_OK;
 }
 
 extern "C" int cah_server_create(cah_ctx* ctx, size_t budget_bytes, int32_t chunk, cah_server** out) {
     if (!ctx || !out) return fail(CAH_ERR_ARGUMENT, "cah_server_create: NULL argument");
     if (chunk < 0) return fail(CAH_ERR_ARGUMENT, "chunk must be >= 0");
+    HIPCHK(hipSetDevice(ctx->device));
     cah_server* s = new cah_server();
     s->ctx = ctx;
     s->budget = budget_bytes;
     s->chunk = chunk;
+    s->arena.slab_min = std::min<size_t>(s->arena.slab_min, std::max<size_t>(budget_bytes, 1 << 20));
+    s->pinned.pinned = true;
+    s->pinned.slab_min = (size_t)64 << 20;
+    auto make = [&]() -> int {
+        for (hipStream_t& q : s->st) HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking));
+        for (SrvSlot& sl : s->slot) {
+            HIPCHK(hipEventCreateWithFlags(&sl.up, hipEventDisableTiming));
+            for (hipEvent_t& e : sl.done) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        return CAH_OK;
+    };
+    if (int rc = make()) {
+        std::string keep = g_err;
+        cah_server_destroy(s);
+        g_err = keep;
+        return rc;
+    }
     *out = s;
     return CAH_OK;
 }
@@ -1075,6 +1399,12 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
     if (t0 < 0 || t1 < t0) return fail(CAH_ERR_ARGUMENT, "need 0 <= t0 <= t1");
     if (d->rule == CAH_RULE_FOREST_FIRE && !p)
         return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
+    const int64_t cells = d->dims[0] * (d->ndims == 2 ? d->dims[1] : 1);
+    const size_t want = (size_t)cells * (d->dtype == CAH_DTYPE_F32 ? 4 : 1);
+    if (u0 && nbytes != want) return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", want, nbytes);
+    const size_t bytes = request_bytes(*d);
+    if (bytes > s->budget)
+        return fail(CAH_ERR_BUSY, "the request needs %zu bytes, more than the whole budget", bytes);
     SrvReq* r = new SrvReq();
     r->d = *d;
     r->offsets.assign(d->offsets, d->offsets + 2 * (size_t)d->n_offsets);
@@ -1085,27 +1415,15 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
         r->d.tap_offsets = r->taps.data();
         r->d.tap_weights = r->weights.data();
     }
-    const int64_t cells = d->dims[0] * (d->ndims == 2 ? d->dims[1] : 1);
     if (d->mask) {
         r->mask.assign(d->mask, d->mask + cells);
         r->d.mask = r->mask.data();
     }
-    const size_t want = (size_t)cells * (d->dtype == CAH_DTYPE_F32 ? 4 : 1);
-    if (u0) {
-        if (nbytes != want) {
-            delete r;
-            return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", want, nbytes);
-        }
-        r->u0.assign((const char*)u0, (const char*)u0 + nbytes);
-    }
-    r->t0 = r->tcur = t0;
+    if (u0) r->u0.assign((const char*)u0, (const char*)u0 + nbytes);
+    r->t0 = r->queued = t0;
     r->t1 = t1;
     if (p) r->p = *p;
-    r->bytes = request_bytes(r->d);
-    if (r->bytes > s->budget) {
-        delete r;
-        return fail(CAH_ERR_BUSY, "the request needs %zu bytes, more than the whole budget", want);
-    }
+    r->bytes = bytes;
     r->id = s->next++;
     r->t_submit = steady_now();
     s->pending.push_back(r);
@@ -1114,41 +1432,281 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
     return CAH_OK;
 }
 
+static bool edges01(const cah_state* st) {
+    const cah_desc& d = st->d;
+    return (d.boundary[0] != CAH_BOUNDARY_CONSTANT || d.boundary_value[0] == 0 || d.boundary_value[0] == 1) &&
+           (d.boundary[1] != CAH_BOUNDARY_CONSTANT || d.boundary_value[1] == 0 || d.boundary_value[1] == 1);
+}
+
+// Create the request's state on the upload stream, queue its initial contents
+// and pick its family.
 static int admit(cah_server* s, SrvReq* r) {
-    cah_state* st = nullptr;
-    if (int rc = cah_state_create(s->ctx, &r->d, &st)) return rc;
+    cah_state* st = new cah_state();
+    if (int rc = create_state(s->ctx, &r->d, st, &s->arena, &s->pinned, s->up)) {
+        std::string keep = g_err;
+        free_state(st);
+        g_err = keep;
+        return rc;
+    }
     r->st = st;
     if (!r->u0.empty()) {
-        if (int rc = cah_state_set(st, r->u0.data(), r->u0.size())) return rc;
+        void* h;
+        if (int rc = stage(st, r->u0.data(), r->u0.size(), &h)) return rc;
+        const size_t row = (size_t)st->n1 * st->es, pitchb = (size_t)st->pitch * st->es;
+        void* dense = row == pitchb ? st->buf[st->cur] : st->buf[1 - st->cur];
+        HIPCHK(hipMemcpyAsync(dense, h, r->u0.size(), hipMemcpyHostToDevice, s->up));
+        if (row != pitchb)
+            if (int rc = copy_rows(dense, (int64_t)row, st->buf[st->cur], (int64_t)pitchb, (int64_t)row,
+                                   st->n2, s->up))
+                return rc;
+        if (st->es == 1) st->bin_clean = scan_binary((const int8_t*)r->u0.data(), r->u0.size());
         std::vector<char>().swap(r->u0);
     }
-    HIPCHK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
+    switch (st->path) {
+    case P_LIFE_SWAR:
+        r->cls = SC_LIFE;
+        r->key = st->life_R * 4 + (life_niv(st->life) - 1) * 2 + (st->bin_clean && edges01(st) ? 1 : 0);
+        break;
+    case P_FOREST:
+        r->cls = SC_FOREST;
+        r->key = forest_prec(r->p) * 2 + ((uint64_t)(st->n1 * st->n2) < 0xFFFFFFFFull ? 1 : 0);
+        break;
+    case P_LENIA_MFMA:
+        r->cls = SC_LENIA;
+        r->key = st->lenia_mode;
+        break;
+    default:
+        r->cls = SC_LOOP;
+        r->key = 0;
+    }
+    // launches of its own (the fourth family) go on the loop stream
+    st->stream = s->st[SC_LOOP];
     r->status = CAH_REQ_ACTIVE;
     r->t_admit = steady_now();
+    return CAH_OK;
+}
+
+// Retire the oldest in-flight ticks that have completed.
+static int retire(cah_server* s) {
+    while (s->nbusy > 0) {
+        SrvSlot& sl = s->slot[s->oldest];
+        for (hipEvent_t e : sl.done) {
+            const hipError_t q = hipEventQuery(e);
+            if (q == hipErrorNotReady) return CAH_OK;
+            if (q != hipSuccess) return fail(CAH_ERR_DEVICE, "server tick: %s", hipGetErrorString(q));
+        }
+        const double now = steady_now();
+        for (SrvReq* r : sl.finishing) {
+            r->status = CAH_REQ_DONE;
+            r->t_done = now;
+            release_staging(r->st);
+            s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+        }
+        sl.finishing.clear();
+        sl.busy = false;
+        s->oldest = (s->oldest + 1) % SRV_SLOTS;
+        --s->nbusy;
+    }
+    return CAH_OK;
+}
+
+// One family's requests of one kernel variant, ordered by steps in this tick
+// (most first): the items the launch of step k advances are a prefix.
+struct SrvGroup {
+    int cls = 0, key = 0;
+    std::vector<SrvReq*> reqs;
+    size_t off_items = 0, off_map = 0, off_base = 0;  // in the tick's table buffer
+    std::vector<int32_t> base;                         // first wave/block of each item, + total
+};
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+static int build_tick(cah_server* s) {
+    std::vector<SrvReq*> run;
+    for (SrvReq* r : s->active) {
+        if (r->tail_queued) continue;
+        const int64_t left = r->t1 - r->queued;
+        r->steps = (int32_t)(s->chunk > 0 ? std::min<int64_t>(s->chunk, left) : left);
+        run.push_back(r);
+    }
+    if (run.empty()) return CAH_OK;
+    SrvSlot& sl = s->slot[(s->oldest + s->nbusy) % SRV_SLOTS];
+
+    // group by family and variant
+    std::map<std::pair<int, int>, SrvGroup> groups;
+    for (SrvReq* r : run) {
+        if (r->cls == SC_LOOP || r->steps == 0) continue;
+        SrvGroup& g = groups[{r->cls, r->key}];
+        g.cls = r->cls;
+        g.key = r->key;
+        g.reqs.push_back(r);
+    }
+    // table layout: per group items, wave (block) -> item map, item bases
+    size_t off = 0;
+    std::vector<std::vector<int32_t>> maps;
+    for (auto& kv : groups) {
+        SrvGroup& g = kv.second;
+        std::stable_sort(g.reqs.begin(), g.reqs.end(),
+                         [](const SrvReq* a, const SrvReq* b) { return a->steps > b->steps; });
+        std::vector<int32_t> map;
+        for (size_t k = 0; k < g.reqs.size(); ++k) {
+            const cah_state* st = g.reqs[k]->st;
+            int32_t n;
+            if (g.cls == SC_LENIA) {
+                n = (int32_t)(((st->n1 + LM_BX - 1) / LM_BX) * ((st->n2 + LM_BY - 1) / LM_BY));
+            } else {
+                const int vb = g.cls == SC_LIFE && st->life_R == 5 ? 2 : 1;
+                const int64_t ow = 62 * 4 * vb;
+                const int64_t rows = std::min<int64_t>(BATCH_ROWS, st->n2);
+                n = (int32_t)(((st->n1 + ow - 1) / ow) * ((st->n2 + rows - 1) / rows));
+            }
+            g.base.push_back((int32_t)map.size());
+            map.insert(map.end(), (size_t)n, (int32_t)k);
+        }
+        g.base.push_back((int32_t)map.size());
+        const size_t isz = g.cls == SC_LIFE ? sizeof(LifeArgs) : g.cls == SC_FOREST ? sizeof(ForestArgs)
+                                                                                    : sizeof(LeniaMfmaArgs);
+        g.off_items = off;
+        off = align256(off + isz * g.reqs.size());
+        g.off_map = off;
+        off = align256(off + 4 * map.size());
+        g.off_base = off;
+        off = align256(off + 4 * g.base.size());
+        maps.push_back(std::move(map));
+    }
+    if (off > sl.host_cap) {
+        if (sl.host) HIPCHK(hipHostFree(sl.host));
+        sl.host = nullptr;
+        const size_t cap = std::max<size_t>(off * 2, 1 << 16);
+        HIPCHK(hipHostMalloc((void**)&sl.host, cap, hipHostMallocDefault));
+        sl.host_cap = cap;
+    }
+    if (off > sl.dev_cap) {
+        if (sl.dev) s->arena.release(sl.dev);
+        const size_t cap = std::max<size_t>(off * 2, 1 << 16);
+        sl.dev = (char*)s->arena.alloc(cap);
+        if (!sl.dev) return fail(CAH_ERR_OUT_OF_MEMORY, "no device memory for the tick tables");
+        sl.dev_cap = cap;
+    }
+    size_t gi = 0;
+    for (auto& kv : groups) {
+        SrvGroup& g = kv.second;
+        for (size_t k = 0; k < g.reqs.size(); ++k) {
+            const SrvReq* r = g.reqs[k];
+            const cah_state* st = r->st;
+            const int32_t n = g.base[k + 1] - g.base[k];
+            if (g.cls == SC_LIFE) {
+                LifeArgs a = st->life;
+                const int vb = st->life_R == 5 ? 2 : 1;
+                a.cur = (const int8_t*)st->buf[st->cur];
+                a.next = (int8_t*)st->buf[1 - st->cur];
+                a.L = lat_of(st);
+                a.cv1 = (int32_t)st->d.boundary_value[0];
+                a.cv2 = (int32_t)st->d.boundary_value[1];
+                a.nbands = (int32_t)((st->n1 + 62 * 4 * vb - 1) / (62 * 4 * vb));
+                a.rows = (int32_t)std::min<int64_t>(BATCH_ROWS, st->n2);
+                a.nwaves = n;
+                std::memcpy(sl.host + g.off_items + k * sizeof a, &a, sizeof a);
+            } else if (g.cls == SC_FOREST) {
+                ForestArgs a{};
+                a.cur = (const int8_t*)st->buf[st->cur];
+                a.next = (int8_t*)st->buf[1 - st->cur];
+                a.L = lat_of(st);
+                a.cv1 = (int32_t)st->d.boundary_value[0];
+                a.cv2 = (int32_t)st->d.boundary_value[1];
+                a.nbands = (int32_t)((st->n1 + 247) / 248);
+                a.rows = (int32_t)std::min<int64_t>(BATCH_ROWS, st->n2);
+                a.nwaves = n;
+                a.seed = st->d.seed;
+                a.t = r->queued;
+                a.p = r->p.ff_p;
+                a.f = r->p.ff_f;
+                a.p_f32 = r->p.ff_p_is_f32;
+                a.f_f32 = r->p.ff_f_is_f32;
+                std::memcpy(sl.host + g.off_items + k * sizeof a, &a, sizeof a);
+            } else {
+                LeniaMfmaArgs a{};
+                a.cur = (const float*)st->buf[st->cur];
+                a.next = (float*)st->buf[1 - st->cur];
+                a.mask = st->mask_d;
+                a.afrag = st->frag_d;
+                a.L = lat_of(st);
+                a.H = st->lenia_H;
+                a.cv1 = (float)st->d.boundary_value[0];
+                a.cv2 = (float)st->d.boundary_value[1];
+                a.prec32 = st->d.lenia_precision == 32;
+                a.mu = st->d.mu;
+                a.sigma = st->d.sigma;
+                a.dt = st->d.dt;
+                a.seed = st->d.seed;
+                a.t = r->queued;
+                a.scheme = st->d.scheme;
+                a.sp = st->d.scheme_p;
+                a.sp_f32 = st->d.scheme_p_is_f32;
+                a.nbx = (int32_t)((st->n1 + LM_BX - 1) / LM_BX);
+                std::memcpy(sl.host + g.off_items + k * sizeof a, &a, sizeof a);
+            }
+        }
+        std::memcpy(sl.host + g.off_map, maps[gi].data(), 4 * maps[gi].size());
+        std::memcpy(sl.host + g.off_base, g.base.data(), 4 * g.base.size());
+        ++gi;
+    }
+    if (off) HIPCHK(hipMemcpyAsync(sl.dev, sl.host, off, hipMemcpyHostToDevice, s->up));
+    HIPCHK(hipEventRecord(sl.up, s->up));
+    for (hipStream_t q : s->st) HIPCHK(hipStreamWaitEvent(q, sl.up, 0));
+
+    // the batched families: step k advances the items with more than k steps
+    int32_t kmax = 0;
+    for (auto& kv : groups) kmax = std::max(kmax, kv.second.reqs.front()->steps);
+    for (int32_t k = 0; k < kmax; ++k) {
+        for (auto& kv : groups) {
+            SrvGroup& g = kv.second;
+            size_t na = 0;
+            while (na < g.reqs.size() && g.reqs[na]->steps > k) ++na;
+            const int32_t nw = g.base[na];
+            if (nw == 0) continue;
+            const int32_t* map = (const int32_t*)(sl.dev + g.off_map);
+            const int32_t* base = (const int32_t*)(sl.dev + g.off_base);
+            const void* items = sl.dev + g.off_items;
+            if (g.cls == SC_LIFE) {
+                const int R = g.key / 4, niv = (g.key / 2) % 2 + 1;
+                launch_life_batch(R, R == 5 ? 2 : 1, g.key % 2, niv, waves_to_blocks(nw),
+                                  (const LifeArgs*)items, map, base, nw, k & 1, s->st[SC_LIFE]);
+            } else if (g.cls == SC_FOREST) {
+                launch_forest_batch(g.key % 2, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
+                                    map, base, nw, k & 1, k, s->st[SC_FOREST]);
+            } else {
+                launch_lenia_mfma_batch(g.key, nw, (const LeniaMfmaArgs*)items, map, base, k & 1, k,
+                                        s->st[SC_LENIA]);
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    // the rest: launches of their own
+    for (SrvReq* r : run)
+        if (r->cls == SC_LOOP && r->steps > 0)
+            if (int rc = cah_advance(r->st, r->queued, r->steps, &r->p)) return rc;
+    for (SrvReq* r : run) {
+        if (r->cls != SC_LOOP) {
+            r->st->cur ^= r->steps & 1;
+            if (r->cls == SC_FOREST && r->steps > 0) r->st->bin_clean = false;
+        }
+        r->queued += r->steps;
+        if (r->queued == r->t1) {
+            r->tail_queued = true;
+            sl.finishing.push_back(r);
+        }
+    }
+    for (int c = 0; c < SRV_STREAMS; ++c) HIPCHK(hipEventRecord(sl.done[c], s->st[c]));
+    sl.busy = true;
+    ++s->nbusy;
     return CAH_OK;
 }
 
 extern "C" int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending) {
     if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
     HIPCHK(hipSetDevice(s->ctx->device));
-    // retire finished chunks and requests
-    for (size_t k = 0; k < s->active.size();) {
-        SrvReq* r = s->active[k];
-        if (r->inflight) {
-            const hipError_t q = hipEventQuery(r->ev);
-            if (q == hipSuccess) r->inflight = false;
-            else if (q != hipErrorNotReady) return fail(CAH_ERR_DEVICE, "request %llu: %s",
-                                                       (unsigned long long)r->id, hipGetErrorString(q));
-        }
-        if (!r->inflight && r->tcur == r->t1) {
-            r->status = CAH_REQ_DONE;
-            r->t_done = steady_now();
-            s->active[k] = s->active.back();
-            s->active.pop_back();
-        } else {
-            ++k;
-        }
-    }
+    if (int rc = retire(s)) return rc;
     // admit in arrival order while the budget allows
     while (!s->pending.empty() && s->used + s->pending.front()->bytes <= s->budget) {
         SrvReq* r = s->pending.front();
@@ -1157,15 +1715,8 @@ extern "C" int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending)
         s->used += r->bytes;
         s->active.push_back(r);
     }
-    // queue the next chunk of every idle resident request
-    for (SrvReq* r : s->active) {
-        if (r->inflight || r->tcur == r->t1) continue;
-        const int64_t n = s->chunk > 0 ? std::min<int64_t>(s->chunk, r->t1 - r->tcur) : r->t1 - r->tcur;
-        if (int rc = cah_advance(r->st, r->tcur, n, &r->p)) return rc;
-        r->tcur += n;
-        HIPCHK(hipEventRecord(r->ev, r->st->stream));
-        r->inflight = true;
-    }
+    if (s->nbusy < SRV_SLOTS)
+        if (int rc = build_tick(s)) return rc;
     if (active) *active = (int32_t)s->active.size();
     if (pending) *pending = (int32_t)s->pending.size();
     return CAH_OK;
@@ -1190,8 +1741,28 @@ extern "C" int cah_server_fetch(cah_server* s, uint64_t id, void* host, size_t n
     SrvReq* r;
     if (int rc = find_req(s, id, &r)) return rc;
     if (r->status != CAH_REQ_DONE) return fail(CAH_ERR_BUSY, "request %llu is not finished", (unsigned long long)id);
-    if (int rc = cah_state_get(r->st, host, nbytes)) return rc;
-    free_state(r->st);
+    if (!host) return fail(CAH_ERR_ARGUMENT, "cah_server_fetch: NULL argument");
+    cah_state* st = r->st;
+    if (nbytes != state_bytes(st))
+        return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(st), nbytes);
+    HIPCHK(hipSetDevice(s->ctx->device));
+    const size_t row = (size_t)st->n1 * st->es, pitchb = (size_t)st->pitch * st->es;
+    const void* dense = st->buf[st->cur];
+    if (row != pitchb) {
+        dense = st->buf[1 - st->cur];
+        if (int rc = copy_rows(st->buf[st->cur], (int64_t)pitchb, st->buf[1 - st->cur], (int64_t)row,
+                               (int64_t)row, st->n2, s->copy))
+            return rc;
+    }
+    char* h = (char*)s->pinned.alloc(nbytes);
+    if (!h) return fail(CAH_ERR_OUT_OF_MEMORY, "cannot stage %zu bytes in pinned memory", nbytes);
+    const hipError_t e = hipMemcpyAsync(h, dense, nbytes, hipMemcpyDeviceToHost, s->copy);
+    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(s->copy) : e;
+    if (e2 == hipSuccess) std::memcpy(host, h, nbytes);
+    s->pinned.release(h);
+    HIPCHK(e2);
+    // every launch that used its blocks has completed: back to the arena
+    free_state(st);
     r->st = nullptr;
     s->used -= r->bytes;
     r->status = CAH_REQ_FETCHED;
@@ -1205,12 +1776,5 @@ extern "C" int cah_server_times(cah_server* s, uint64_t id, double* t_submit, do
     if (t_submit) *t_submit = r->t_submit;
     if (t_admit) *t_admit = r->t_admit;
     if (t_done) *t_done = r->t_done;
-    return CAH_OK;
-}
-
-extern "C" int cah_server_destroy(cah_server* s) {
-    if (!s) return CAH_OK;
-    for (auto& kv : s->all) free_req(kv.second);
-    delete s;
     return CAH_OK;
 }

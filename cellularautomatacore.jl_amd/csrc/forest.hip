@@ -24,24 +24,19 @@ void launch_forest(int vb, bool small, int blocks, const ForestArgs& a, hipStrea
 
 template <bool SMALL, int PREC>
 static void forest_batch_go(int blocks, const ForestArgs* it, const int32_t* wi, const int32_t* wb,
-                            int32_t nw, int32_t parity, int64_t t, double p, double f,
-                            int32_t p_f32, int32_t f_f32, hipStream_t st) {
-    k_forest_batch<1, SMALL, PREC><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, t, p, f, p_f32,
-                                                           f_f32);
+                            int32_t nw, int32_t parity, int64_t dt, hipStream_t st) {
+    k_forest_batch<1, SMALL, PREC><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, dt);
 }
 
-void launch_forest_batch(int vb, bool small, int blocks, const ForestArgs* it, const int32_t* wi,
-                         const int32_t* wb, int32_t nw, int32_t parity, int64_t t, double p,
-                         double f, int32_t p_f32, int32_t f_f32, hipStream_t st) {
-    (void)vb;
-    const int prec = (p_f32 ? 1 : 0) | (f_f32 ? 2 : 0);
+void launch_forest_batch(bool small, int prec, int blocks, const ForestArgs* it, const int32_t* wi,
+                         const int32_t* wb, int32_t nw, int32_t parity, int64_t dt, hipStream_t st) {
     auto go = [&](auto S) {
         constexpr bool SM = decltype(S)::value;
         switch (prec) {
-        case 0: forest_batch_go<SM, 0>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
-        case 1: forest_batch_go<SM, 1>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
-        case 2: forest_batch_go<SM, 2>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
-        default: forest_batch_go<SM, 3>(blocks, it, wi, wb, nw, parity, t, p, f, p_f32, f_f32, st); break;
+        case 0: forest_batch_go<SM, 0>(blocks, it, wi, wb, nw, parity, dt, st); break;
+        case 1: forest_batch_go<SM, 1>(blocks, it, wi, wb, nw, parity, dt, st); break;
+        case 2: forest_batch_go<SM, 2>(blocks, it, wi, wb, nw, parity, dt, st); break;
+        default: forest_batch_go<SM, 3>(blocks, it, wi, wb, nw, parity, dt, st); break;
         }
     };
     if (small) go(std::true_type{});

@@ -143,16 +143,17 @@ __global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
 // One launch advances every trajectory of a batch by one step.  The waves of all
 // trajectories form one flat list: wave w belongs to item wave_item[w] and is
 // that item's wave (w - wave_base[item]).  Each item is the trajectory's own
-// argument block (buffers for parity 0, lattice, seed, band layout); the step's
-// time, parameters and buffer parity come from the launch.
+// argument block (buffers for parity 0, lattice, seed, band layout, p and f, and
+// its time base); the launch adds the step offset dt to the item's t and swaps
+// the buffers when parity is set.  A launch over the first nwaves waves advances
+// only the items those waves belong to, so items ordered by steps left can stop
+// at different steps of one sequence of launches.
 
 template <int VB, bool SMALL, int PREC>
 __global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restrict__ items,
                                                       const int32_t* __restrict__ wave_item,
                                                       const int32_t* __restrict__ wave_base,
-                                                      int32_t nwaves, int32_t parity, int64_t t,
-                                                      double p, double f, int32_t p_f32,
-                                                      int32_t f_f32) {
+                                                      int32_t nwaves, int32_t parity, int64_t dt) {
     const int wid = wave_id();
     if (wid >= nwaves) return;
     const int k = __builtin_amdgcn_readfirstlane(wave_item[wid]);
@@ -162,10 +163,6 @@ __global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restri
         a.cur = a.next;
         a.next = const_cast<int8_t*>(c);
     }
-    a.t = t;
-    a.p = p;
-    a.f = f;
-    a.p_f32 = p_f32;
-    a.f_f32 = f_f32;
+    a.t += dt;
     forest_band<VB, SMALL, PREC>(a, wid - wave_base[k]);
 }

@@ -4,6 +4,8 @@
 //                 scheme, masks.  An LDS tile with the boundary conditions applied
 //                 while it is loaded (no halo buffer, no separate halo launch).
 #pragma once
+#include <type_traits>
+
 #include "device.hpp"
 
 // ---------------------------------------------------------------- generic
@@ -105,5 +107,25 @@ __global__ void k_fill_uniform_box(float* u, Lat L, uint64_t seed, int32_t lo1, 
         const uint64_t id = (uint64_t)y * (uint64_t)L.n1 + (uint64_t)x + 1;
         const bool in = x >= lo1 && x < lo1 + len1 && y >= lo2 && y < lo2 + len2;
         u[(int64_t)y * L.pitch + x] = in ? (float)gen_uniform(seed, id) : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------- row layout copies
+//
+// Rows of rb bytes between a pitched and a dense layout (pitches sp and dp in
+// bytes), W bytes per thread; W divides rb, sp, dp and both base addresses.
+// Host <-> device transfers of a state move the dense form with one linear
+// copy: the DMA engines split a pitched copy whose rows are not 4-byte multiples
+// into one transfer per row.
+template <int W>
+__global__ __launch_bounds__(256) void k_copy_rows(const uint8_t* __restrict__ src, int64_t sp,
+                                                   uint8_t* __restrict__ dst, int64_t dp, int64_t rb,
+                                                   int64_t rows) {
+    using V = std::conditional_t<W == 16, uint4, std::conditional_t<W == 4, uint32_t, uint8_t>>;
+    const int64_t per = rb / W, n = per * rows;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t y = i / per, x = i - y * per;
+        *reinterpret_cast<V*>(dst + y * dp + x * W) = *reinterpret_cast<const V*>(src + y * sp + x * W);
     }
 }

@@ -44,6 +44,7 @@ struct LeniaMfmaArgs {
     int64_t t;
     int32_t scheme, sp_f32;
     double sp;
+    int32_t nbx;  // output blocks along the first axis (batched launches)
 };
 
 template <int MODE>
@@ -56,8 +57,9 @@ struct LmT {
     }
 };
 
+// One 64 x 64 output block (bx, by) of one lattice; 256 threads.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_lenia_mfma(LeniaMfmaArgs a) {
+CAH_DEV void lenia_mfma_block(const LeniaMfmaArgs& a, int bx, int by) {
     using T = LmT<MODE>;
     using E = typename T::E;
     using V8 = typename T::V8;
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(256) void k_lenia_mfma(LeniaMfmaArgs a) {
     __shared__ __attribute__((aligned(16))) E tile[NP][LM_ROWS * LM_STRIDE];
 
     const int H = a.H;
-    const int x0 = blockIdx.x * LM_BX, y0 = blockIdx.y * LM_BY;
+    const int x0 = bx * LM_BX, y0 = by * LM_BY;
     const int rows = LM_BY + 2 * H;
     for (int i = threadIdx.x; i < rows * LM_COLS; i += blockDim.x) {
         const int lx = i % LM_COLS, ly = i / LM_COLS;
@@ -128,4 +130,28 @@ __global__ __launch_bounds__(256) void k_lenia_mfma(LeniaMfmaArgs a) {
             a.next[(int64_t)y * a.L.pitch + x] = out;
         }
     }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_lenia_mfma(LeniaMfmaArgs a) {
+    lenia_mfma_block<MODE>(a, blockIdx.x, blockIdx.y);
+}
+
+// Batched: block b advances block (b - block_base[i]) of item i = block_item[b],
+// with the item's buffers swapped when parity is set and its time base + dt.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_lenia_mfma_batch(const LeniaMfmaArgs* __restrict__ items,
+                                                          const int32_t* __restrict__ block_item,
+                                                          const int32_t* __restrict__ block_base,
+                                                          int32_t parity, int64_t dt) {
+    const int i = block_item[blockIdx.x];
+    LeniaMfmaArgs a = items[i];
+    if (parity) {
+        const float* c = a.cur;
+        a.cur = a.next;
+        a.next = const_cast<float*>(c);
+    }
+    a.t += dt;
+    const int local = (int)blockIdx.x - block_base[i];
+    lenia_mfma_block<MODE>(a, local % a.nbx, local / a.nbx);
 }

@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "kernels_forest.hpp"
 #include "kernels_life.hpp"
 
@@ -17,19 +19,24 @@ void launch_life_batch_r(int vb, bool bin, int niv, int blocks, const LifeArgs* 
 
 // forest fire, VonNeumann(1): vb in {1, 2, 4}; small = fewer than 2^32 cells.
 void launch_forest(int vb, bool small, int blocks, const ForestArgs& a, hipStream_t st);
-void launch_forest_batch(int vb, bool small, int blocks, const ForestArgs* items,
+// The batch items carry p, f and their time base; prec = p_f32 | f_f32 << 1.
+void launch_forest_batch(bool small, int prec, int blocks, const ForestArgs* items,
                          const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,
-                         int32_t parity, int64_t t, double p, double f, int32_t p_f32,
-                         int32_t f_f32, hipStream_t st);
+                         int32_t parity, int64_t dt, hipStream_t st);
 
 // ---------------------------------------------------------------- Lenia (lenia.hip)
 #include "kernels_lenia_mfma.hpp"
 
 // MFMA implicit GEMM: mode LM_F16 / LM_BF16 / LM_F16X2.
 void launch_lenia_mfma(int mode, dim3 grid, const LeniaMfmaArgs& a, hipStream_t st);
+// One step of several lattices: nblocks 64 x 64 output blocks over the items,
+// block b of item block_item[b] being its block (b - block_base[item]).
+void launch_lenia_mfma_batch(int mode, int nblocks, const LeniaMfmaArgs* items,
+                             const int32_t* block_item, const int32_t* block_base,
+                             int32_t parity, int64_t dt, hipStream_t st);
 // The A fragments of the implicit GEMM for a dense (2H+1)^2 FP64 weight grid
-// wd[(dy + H) * (2H + 1) + (dx + H)]: a device buffer the caller frees.
-int lenia_mfma_fragments(int mode, int H, const double* wd, void** out_dev);
+// wd[(dy + H) * (2H + 1) + (dx + H)], as the bytes of the device buffer.
+void lenia_mfma_fragments(int mode, int H, const double* wd, std::vector<char>& out);
 
 // FP32 FFT field on a periodic lattice (the FFTW extension's cross-correlation).
 struct LeniaFFT;

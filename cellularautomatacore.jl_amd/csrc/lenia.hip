@@ -12,6 +12,17 @@ void launch_lenia_mfma(int mode, dim3 grid, const LeniaMfmaArgs& a, hipStream_t 
     else k_lenia_mfma<LM_F16><<<grid, 256, 0, st>>>(a);
 }
 
+void launch_lenia_mfma_batch(int mode, int nblocks, const LeniaMfmaArgs* items,
+                             const int32_t* block_item, const int32_t* block_base,
+                             int32_t parity, int64_t dt, hipStream_t st) {
+    if (mode == LM_BF16)
+        k_lenia_mfma_batch<LM_BF16><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, dt);
+    else if (mode == LM_F16X2)
+        k_lenia_mfma_batch<LM_F16X2><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, dt);
+    else
+        k_lenia_mfma_batch<LM_F16><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, dt);
+}
+
 // Fragment f = (dyi * 2 + c) * 64 + lane holds A_dy[m][k] for m = lane % 16 and
 // k = 32 c + 8 (lane / 16) + i, i = 0..7 (the operand order of
 // v_mfma_f32_16x16x32_*; B uses the same k order, so the sum is the same under
@@ -37,10 +48,10 @@ static void fill_fragments(int H, const double* wd, std::vector<E>& hi, std::vec
                 }
 }
 
-int lenia_mfma_fragments(int mode, int H, const double* wd, void** out_dev) {
+void lenia_mfma_fragments(int mode, int H, const double* wd, std::vector<char>& buf) {
     const size_t nfrag = (size_t)(2 * H + 1) * 2 * 64 * 8;
     const size_t es = 2;
-    std::vector<char> buf(nfrag * es * 2);
+    buf.assign(nfrag * es * 2, 0);
     if (mode == LM_BF16) {
         std::vector<__bf16> hi, lo;
         fill_fragments(H, wd, hi, lo);
@@ -52,9 +63,6 @@ int lenia_mfma_fragments(int mode, int H, const double* wd, void** out_dev) {
         std::memcpy(buf.data(), hi.data(), nfrag * es);
         std::memcpy(buf.data() + nfrag * es, lo.data(), nfrag * es);
     }
-    if (hipMalloc(out_dev, buf.size()) != hipSuccess) return -4;
-    if (hipMemcpy(*out_dev, buf.data(), buf.size(), hipMemcpyHostToDevice) != hipSuccess) return -7;
-    return 0;
 }
 
 // ---------------------------------------------------------------- FFT field
@@ -142,6 +150,9 @@ int lenia_fft_create(LeniaFFT** out, int n1, int n2, int64_t pitch, const int32_
 }
 
 int lenia_fft_field(LeniaFFT* f, const float* cur, hipStream_t st) {
+    // the state's stream may change after the plan was made (a server's states)
+    if (hipfftSetStream(f->fwd, st) != HIPFFT_SUCCESS || hipfftSetStream(f->inv, st) != HIPFFT_SUCCESS)
+        return -7;
     if (hipfftExecR2C(f->fwd, const_cast<float*>(cur), (hipfftComplex*)f->spec) != HIPFFT_SUCCESS)
         return -7;
     k_spec_mul<<<(unsigned)((f->nspec + 255) / 256), 256, 0, st>>>(f->spec, f->kspec, f->nspec);

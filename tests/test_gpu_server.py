@@ -19,7 +19,7 @@ def mixed_requests(n, seed=4):
         side = int(rng.integers(40, 300))
         side2 = int(rng.integers(40, 300))
         steps = int(rng.integers(5, 60))
-        kind = i % 4
+        kind = i % 6
         if kind == 0:
             u = rng.integers(0, 2, (side, side2)).astype(np.int8)
             probs.append(CAProblem(M.Life(), u, steps, seed=i))
@@ -31,15 +31,23 @@ def mixed_requests(n, seed=4):
             u = np.zeros((side, side2), np.float32)
             u[10:30, 10:30] = rng.random((20, 20)).astype(np.float32)
             probs.append(CAProblem(M.Lenia(), u, min(steps, 20), seed=i))
-        else:
+        elif kind == 3:
             u = (rng.random((side, side2)) < 0.4).astype(np.int8)
             probs.append(CAProblem(ltl_rule(), u, min(steps, 15), seed=i))
+        elif kind == 4:  # no batched kernel: launches of its own
+            u = rng.integers(0, 3, (side, side2)).astype(np.int8)
+            probs.append(CAProblem(M.Generations("B2/S/C3"), u, steps, seed=i,
+                                   space=M.Grid(side, side2, boundary=M.Constant(1))))
+        else:  # Larger-than-Life of radius 2 with a Constant(1) edge: its own batch
+            u = rng.integers(0, 2, (side, side2)).astype(np.int8)
+            probs.append(CAProblem(M.Life(range(5, 8), range(4, 9), neighbourhood=M.Moore(2)), u, steps, seed=i,
+                                   space=M.Grid(side, side2, boundary=M.Constant(1))))
     return probs
 
 
 @pytest.mark.parametrize("chunk", [0, 7])
 def test_mix_equals_alone(chunk):
-    probs = mixed_requests(24)
+    probs = mixed_requests(30)
     # a budget that holds only a few lattices at once, so requests queue
     srv = Server(budget_bytes=3_000_000, chunk=chunk, lenia_field="mfma_f16")
     ids = []
@@ -62,6 +70,35 @@ def test_mix_equals_alone(chunk):
         assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs"
         t_sub, t_adm, t_done = srv.times(rid)
         assert t_sub <= t_adm <= t_done
+    srv.close()
+
+
+def test_fft_and_stochastic_requests():
+    """Requests on the FFT field and the stochastic scheme (the loop family)."""
+    rng = np.random.default_rng(9)
+    probs, schemes = [], []
+    for i in range(4):
+        u = np.zeros((96, 80), np.float32)
+        u[20:50, 20:50] = rng.random((30, 30)).astype(np.float32)
+        probs.append(CAProblem(M.Lenia(method="fft"), u, 12, seed=i))
+        schemes.append(M.Synchronous())
+        v = rng.integers(0, 2, (70, 90)).astype(np.int8)
+        probs.append(CAProblem(M.Life(), v, 25, seed=50 + i))
+        schemes.append(M.Stochastic(0.5))
+    srv = Server(budget_bytes=50_000_000, chunk=5, lenia_field="mfma_f16")
+    ids = [srv.submit(p, sc) for p, sc in zip(probs, schemes)]
+    results = {}
+    for _ in range(100000):
+        srv.tick()
+        for rid in ids:
+            if rid not in results and srv.done(rid):
+                results[rid] = srv.fetch(rid)
+        if len(results) == len(ids):
+            break
+    be = HIPBackend(lenia_field="mfma_f16")
+    for rid, p, sc in zip(ids, probs, schemes):
+        alone = final_state(solve(p, sc, backend=be, save_everystep=False))
+        assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs"
     srv.close()
 
 
