@@ -24,6 +24,66 @@ sys.path.insert(0, os.path.join(ROOT, "cellularautomatacore.jl_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_ltl8192_forest16384.json")
+COPY_N = 16384 * 16384  # bytes of the calibration copy (one 16384^2 Int8 lattice)
+KERNELS = {"ltl": "k_life_swar<5, 2, true, 1>", "forest": "k_forest_vn1<2, true, 0>",
+           "copy": "k_copy_rows<16>"}
+ALGO_BYTES = {"ltl": 2 * 8192 * 8192, "forest": 2 * 16384 * 16384, "copy": 2 * COPY_N}
+
+
+def copy_calibration(be, st, reps=3):
+    """A plain device copy of known size that scales the HBM counters: the forest
+    lattice (16384^2 Int8, dense rows) copied to a device buffer by k_copy_rows<16>,
+    COPY_N bytes read and COPY_N written per launch."""
+    import torch
+    from cahip.backend import check
+    dst = torch.empty(COPY_N, dtype=torch.int8, device="cuda")
+    for _ in range(reps):
+        check(be.lib.cah_state_get_device(st.handle, ctypes.c_void_p(dst.data_ptr()), COPY_N))
+    torch.cuda.synchronize()
+
+
+def pmc_summary(fetch_csv, write_csv, cmd):
+    """HBM bytes per launch of the swept kernels from two rocprofv3 --pmc runs of
+    bench.py --copy-calibration (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch),
+    each scaled by what the counter reads for the calibration copy of known size.
+    Writes TRAFFIC_JSON and a Markdown table beside it."""
+    import csv
+
+    def means(path, counter):
+        acc = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    continue
+                for key, name in KERNELS.items():
+                    if name in r["Kernel_Name"]:
+                        acc.setdefault(key, []).append(float(r["Counter_Value"]))
+        return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+    fetch, write = means(fetch_csv, "FETCH_SIZE"), means(write_csv, "WRITE_SIZE")
+    sf = COPY_N / 1024 / fetch["copy"][0]
+    sw = COPY_N / 1024 / write["copy"][0]
+    out = {"command": cmd, "scale_fetch": sf, "scale_write": sw}
+    rows = ["| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes/launch (scaled) "
+            "| algorithmic bytes | ratio |", "|---|---|---|---|---|---|---|"]
+    for key in ("ltl", "forest", "copy"):
+        f, nf = fetch[key]
+        w, nw = write[key]
+        b = (f * sf + w * sw) * 1024
+        out[key] = {"kernel": KERNELS[key], "fetch_kib": f, "write_kib": w,
+                    "bytes_per_launch": round(b), "algorithmic_bytes": ALGO_BYTES[key]}
+        rows.append(f"| {KERNELS[key]} | {nf}/{nw} | {f:.0f} | {w:.0f} | {b:.4g} | "
+                    f"{ALGO_BYTES[key]:.4g} | {b / ALGO_BYTES[key]:.3f} |")
+    with open(TRAFFIC_JSON, "w") as fh:
+        json.dump(out, fh, indent=1)
+    md = TRAFFIC_JSON[:-5] + ".md"
+    with open(md, "w") as fh:
+        fh.write("# HBM traffic per launch (rocprofv3 --pmc)\n\n"
+                 f"Commands (one counter per run, then this summary):\n\n```\n{cmd}\n```\n\n"
+                 f"Scale factors from the calibration copy ({COPY_N} bytes read and written "
+                 f"per launch): FETCH x{sf:.4f}, WRITE x{sw:.4f}.\n\n" + "\n".join(rows) + "\n")
+    return out
 
 
 class HipEvents:
@@ -252,23 +312,26 @@ def bench_lenia(be, ev, steps=500):
 
 
 SERVING_MIX = ("Life B3/S23 Torus Bernoulli(0.5); forest fire VN(1) Reflecting all-empty "
-               "p=0.03 f=1e-4; Larger-than-Life Moore(5) B34..45/S34..58 Torus Bernoulli(0.5); "
-               "Lenia R=13 Torus (MFMA FP16) uniform central side/8 patch; in turn; sides "
-               "round(64 * 16^u) (log-uniform in [64, 1024]) square; tspan (0, U{50..300})")
+               "p=0.03 f=1e-4; Lenia R=13 Torus (MFMA FP16) uniform central side/8 patch{ltl}; "
+               "in turn; sides round({lo} * {r}^u) (log-uniform in [{lo}, {hi}]) square; "
+               "tspan (0, U{{50..{tmax}}})")
 
 
-def serving_problems(n, seed=5):
-    """n requests of the mix SERVING_MIX: the kind of traffic ensembles and ABC
-    rejection generate (many independent small problems).  numpy default_rng(seed)."""
+def serving_problems(n, seed=5, lo=64, hi=1024, tmax=300, ltl=True):
+    """n requests of the mix SERVING_MIX, in turn.  With lo=128, hi=4096, tmax=500 and
+    no Larger-than-Life requests it is the serving configuration's stream; with the
+    defaults it is the traffic ensembles and ABC rejection generate (many
+    independent small problems).  numpy default_rng(seed)."""
     from cahip import model as M
     from cahip.problem import CAProblem
     rng = np.random.default_rng(seed)
-    ltl = M.Life(range(34, 46), range(34, 59), neighbourhood=M.Moore(5))
+    rule_ltl = M.Life(range(34, 46), range(34, 59), neighbourhood=M.Moore(5))
+    kinds = (0, 1, 2, 3) if ltl else (0, 1, 3)
     out = []
     for i in range(n):
-        side = int(round(64 * 16 ** rng.random()))
-        steps = int(rng.integers(50, 301))
-        kind = i % 4
+        side = int(round(lo * (hi / lo) ** rng.random()))
+        steps = int(rng.integers(50, tmax + 1))
+        kind = kinds[i % len(kinds)]
         if kind == 0:
             u = rng.integers(0, 2, (side, side), dtype=np.int8)
             p = CAProblem(M.Life("conway"), u, steps, seed=i)
@@ -278,7 +341,7 @@ def serving_problems(n, seed=5):
                           space=M.Grid(side, side, boundary=M.Reflecting()))
         elif kind == 2:
             u = rng.integers(0, 2, (side, side), dtype=np.int8)
-            p = CAProblem(ltl, u, steps, seed=i)
+            p = CAProblem(rule_ltl, u, steps, seed=i)
         else:
             u = np.zeros((side, side), np.float32)
             k = max(side // 8, 1)
@@ -293,31 +356,34 @@ def poisson_arrivals(n, rate, seed=6):
     return np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
 
 
-def serve(probs, arrivals, chunk=32):
+def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None):
     """The server (cah_server_*) fed in arrival order; per request the time its
     result reached host memory, and the results."""
     from cahip.server import Server
-    srv = Server(budget_bytes=100e9, chunk=chunk, lenia_field="mfma_f16")
+    srv = Server(budget_bytes=budget, chunk=chunk, lenia_field="mfma_f16")
+    if tick_us:
+        srv.tune(chunk, tick_us)
     for p in probs[:4]:  # warm-up: one request of each kind
         rid = srv.submit(p)
         while not srv.done(rid):
             srv.tick()
         srv.fetch(rid)
+    srv.poll()
     n = len(probs)
-    ids, k, fin, res, live = [None] * n, 0, [0.0] * n, [None] * n, set()
+    k, fin, res, index = 0, [0.0] * n, [None] * n, {}
+    left = n
     start = time.perf_counter()
-    while k < n or live:
+    while left:
         now = time.perf_counter() - start
         while k < n and arrivals[k] <= now:
-            ids[k] = srv.submit(probs[k])
-            live.add(k)
+            index[srv.submit(probs[k])] = k
             k += 1
         srv.tick()
-        for j in list(live):
-            if srv.done(ids[j]):
-                res[j] = srv.fetch(ids[j])
-                fin[j] = time.perf_counter() - start
-                live.discard(j)
+        for rid in srv.poll():
+            j = index[rid]
+            res[j] = srv.fetch(rid)
+            fin[j] = time.perf_counter() - start
+            left -= 1
     srv.close()
     return np.array(fin), res
 
@@ -349,37 +415,58 @@ def plain(probs, arrivals):
     return np.array(fin), res
 
 
+def latency_pair(probs, arr, budget=200e9):
+    """Served and plain runs of one arrival stream: p50/p99 of arrival -> result in
+    host memory, sims/s to the last result, and whether the results agree."""
+    fs, rs = serve(probs, arr, budget=budget)
+    fp, rp = plain(probs, arr)
+    ls, lp = (fs - arr) * 1e3, (fp - arr) * 1e3
+    n = len(probs)
+    return {"served": {"p50_ms": round(float(np.percentile(ls, 50)), 2),
+                       "p99_ms": round(float(np.percentile(ls, 99)), 2),
+                       "sims_per_s": round(n / fs.max(), 1)},
+            "plain": {"p50_ms": round(float(np.percentile(lp, 50)), 2),
+                      "p99_ms": round(float(np.percentile(lp, 99)), 2),
+                      "sims_per_s": round(n / fp.max(), 1)},
+            "identical_results": bool(all(np.array_equal(a, b) for a, b in zip(rs, rp)))}
+
+
 def bench_serving():
-    """Continuous batching (cah_server_*: 100 GB budget, resident lattices, 32-step
-    ticks, one batched launch per step per kernel family) against the plain
-    per-problem sweep.  Capacity: 400 requests all present at t = 0, sims/s to the
-    last result, and whether every result equals the plain sweep's.  Latency:
-    arrival to result in host memory under Poisson arrivals, at a light load both
-    sustain and at a heavy one."""
+    """The serving configuration: a Poisson stream at 200 req/s (400 requests) of
+    Life, forest-fire and Lenia requests in turn, sides log-uniform in [128, 4096],
+    tspan U[50, 500], served under a 200 GB budget with 32-step ticks, against the
+    plain per-problem sweep taking the same stream."""
+    n, rate = 400, 200.0
+    probs = serving_problems(n, seed=5, lo=128, hi=4096, tmax=500, ltl=False)
+    arr = poisson_arrivals(n, rate, seed=6)
+    out = {"workload": SERVING_MIX.format(ltl="", lo=128, r=32, hi=4096, tmax=500) +
+           f"; Poisson {rate:g} req/s, {n} requests; budget 200 GB",
+           "requests": n,
+           "cell_updates": int(sum(p.u0.size * (p.tspan[1] - p.tspan[0]) for p in probs))}
+    out.update(latency_pair(probs, arr))
+    return out
+
+
+def bench_serving_ensembles():
+    """Ensemble-style traffic (many small problems, Larger-than-Life added):
+    capacity with 400 requests present at t = 0, and latency under Poisson
+    arrivals at a light load both paths sustain and at a heavy one."""
     probs = serving_problems(400)
-    cu = sum(p.u0.size * (p.tspan[1] - p.tspan[0]) for p in probs)
     zero = np.zeros(len(probs))
-    fs, rs = serve(probs, zero)
-    fp, rp = plain(probs, zero)
-    same = all(np.array_equal(a, b) for a, b in zip(rs, rp))
-    out = {"workload": SERVING_MIX,
-           "capacity": {"requests": len(probs), "cell_updates": int(cu),
-                        "served_sims_per_s": round(len(probs) / fs.max(), 1),
-                        "plain_sims_per_s": round(len(probs) / fp.max(), 1),
-                        "ratio": round(fp.max() / fs.max(), 2), "identical_results": bool(same)}}
+    cap = latency_pair(probs, zero)
+    out = {"workload": SERVING_MIX.format(ltl="; Larger-than-Life Moore(5) B34..45/S34..58 Torus "
+                                              "Bernoulli(0.5)", lo=64, r=16, hi=1024, tmax=300),
+           "capacity": {"requests": len(probs),
+                        "cell_updates": int(sum(p.u0.size * (p.tspan[1] - p.tspan[0]) for p in probs)),
+                        "served_sims_per_s": cap["served"]["sims_per_s"],
+                        "plain_sims_per_s": cap["plain"]["sims_per_s"],
+                        "ratio": round(cap["served"]["sims_per_s"] / cap["plain"]["sims_per_s"], 2),
+                        "identical_results": cap["identical_results"]}}
     for name, rate, n, seed in (("light", 150.0, 300, 7), ("heavy", 1000.0, 1000, 8)):
         probs = serving_problems(n, seed=seed)
         arr = poisson_arrivals(n, rate, seed)
-        fs, _ = serve(probs, arr)
-        fp, _ = plain(probs, arr)
-        ls, lp = (fs - arr) * 1e3, (fp - arr) * 1e3
-        out[name] = {"rate_per_s": rate, "requests": n,
-                     "served": {"p50_ms": round(float(np.percentile(ls, 50)), 2),
-                                "p99_ms": round(float(np.percentile(ls, 99)), 2),
-                                "sims_per_s": round(n / fs.max(), 1)},
-                     "plain": {"p50_ms": round(float(np.percentile(lp, 50)), 2),
-                               "p99_ms": round(float(np.percentile(lp, 99)), 2),
-                               "sims_per_s": round(n / fp.max(), 1)}}
+        out[name] = {"rate_per_s": rate, "requests": n}
+        out[name].update(latency_pair(probs, arr))
     return out
 
 
@@ -391,7 +478,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (Life 512^2, batched, Lenia, serving)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run the plain copy that scales the HBM counters")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
+                    help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary)))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the sweep on a single MI355X "
               "(the multi-GPU slab decomposition is design only)", file=sys.stderr)
@@ -434,14 +528,16 @@ def main():
     c_ltl, c_for = 8192 * 8192, 16384 * 16384
     ms_step = ms_ltl + ms_for
     value = (c_ltl + c_for) / (ms_step * 1e-3) / 1e9
-    # dominant kernel: the forest fire sweep (one launch per step)
-    bytes_per_launch = 2 * c_for
-    achieved = bytes_per_launch / (ms_for * 1e-3) / 1e9
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "forest_vn1_16384_traffic.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    # dominant kernel: the forest fire sweep (one launch per step); algorithmic
+    # bytes = one Int8 read and one Int8 write per cell
+    achieved = ALGO_BYTES["forest"] / (ms_for * 1e-3) / 1e9
+    achieved_ltl = ALGO_BYTES["ltl"] / (ms_ltl * 1e-3) / 1e9
+    traffic = {}
+    if os.path.exists(TRAFFIC_JSON):
+        with open(TRAFFIC_JSON) as f:
+            traffic = json.load(f)
+    if args.copy_calibration:
+        copy_calibration(be, s_for)
     line = {
         "metric": "GCUPS + % HBM roofline (synchronous sweep)",
         "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": K,
@@ -458,9 +554,15 @@ def main():
                              "ms_per_step": round(ms_for, 5),
                              "hbm_gbs_algorithmic": round(2 * c_for / ms_for / 1e6, 1)},
         },
-        "roofline": {"bound": "hbm", "kernel": "k_forest_vn1", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": KERNELS["forest"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic.get("forest", {}).get("bytes_per_launch"),
+                     "note": "one Philox4x32-10 draw per cell (20 v_mad_u64_u32): bound by "
+                             "the VALU, not by HBM"},
+        "roofline_ltl": {"bound": "hbm", "kernel": KERNELS["ltl"], "achieved": round(achieved_ltl, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved_ltl / HBM_PEAK_GBS, 4),
+                         "traffic": traffic.get("ltl", {}).get("bytes_per_launch")},
     }
     be.free_state(s_ltl)
     be.free_state(s_for)
@@ -468,7 +570,8 @@ def main():
         line["configs"] = {"life_512": bench_life512(be, ev),
                            "batched_forest": bench_batched(be, ev),
                            "lenia_2048": bench_lenia(be, ev),
-                           "serving": bench_serving()}
+                           "serving": bench_serving(),
+                           "serving_ensembles": bench_serving_ensembles()}
     if not args.no_cpu:
         n = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(n)

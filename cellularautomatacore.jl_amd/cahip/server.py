@@ -42,10 +42,26 @@ class Server:
         self.meta[rid.value] = (prob.space.dims, u0.dtype)
         return rid.value
 
+    def tune(self, chunk, tick_us):
+        """Steps per request per tick at most, and the estimated device time of a
+        tick in microseconds."""
+        check(self.lib.cah_server_tune(self.h, int(chunk), float(tick_us)))
+
     def tick(self):
         a, p = ctypes.c_int32(), ctypes.c_int32()
         check(self.lib.cah_server_tick(self.h, ctypes.byref(a), ctypes.byref(p)))
         return a.value, p.value
+
+    def poll(self, cap=1024):
+        """Ids of the requests that finished since the last poll."""
+        ids = (ctypes.c_uint64 * cap)()
+        n = ctypes.c_int32()
+        out = []
+        while True:
+            check(self.lib.cah_server_poll(self.h, ids, cap, ctypes.byref(n)))
+            out.extend(ids[:n.value])
+            if n.value < cap:
+                return out
 
     def status(self, rid):
         s = ctypes.c_int32()

@@ -64,6 +64,10 @@ CAH_DEV void load_row(const int8_t* __restrict__ u, const Lat& L, int8_t cv1, in
     }
 }
 
+// A lane's VB output dwords of row y.  Ordinary stores: the next step reads
+// them, and a lattice that fits the 256 MB Infinity Cache is then read from it
+// (streaming stores, which skip it, measured 30% slower on Larger-than-Life
+// 8192^2).
 template <int VB>
 CAH_DEV void store_row(int8_t* __restrict__ u, const Lat& L, int32_t x, int32_t y,
                        const uint32_t (&X)[VB]) {
@@ -103,6 +107,17 @@ CAH_DEV void load_vec(const int8_t* p, uint32_t (&X)[VB]) {
 // The wave's index in the grid, the same in every lane.
 CAH_DEV int wave_id() {
     return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+}
+
+// The wave index in an XCD-aware block order.  Workgroups are dealt to the eight
+// XCDs round robin (block b to XCD b % 8), and each XCD has its own L2.
+// Numbering the blocks so that XCD x runs one contiguous range of them puts
+// neighbouring row chunks, which read each other's edge rows, on one L2.
+CAH_DEV int wave_id_xcd() {
+    const int n = gridDim.x, b = blockIdx.x;
+    const int q = n >> 3, r = n & 7, xcd = b & 7;
+    const int lb = xcd * q + min(xcd, r) + (b >> 3);
+    return __builtin_amdgcn_readfirstlane(lb * (blockDim.x / 64) + threadIdx.x / 64);
 }
 
 

@@ -1284,6 +1284,7 @@ struct SrvReq {
     cah_state* st = nullptr;
     int cls = SC_LOOP, key = 0;          // family and kernel variant
     bool tail_queued = false;            // its last step is in a queued tick
+    bool in_tick = false;                // some queued tick follows its uploads
     int32_t status = CAH_REQ_PENDING;
     size_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -1298,6 +1299,7 @@ struct SrvSlot {
     char* dev = nullptr;
     size_t dev_cap = 0;
     std::vector<SrvReq*> finishing;
+    std::vector<SrvReq*> staged;  // requests whose initial uploads precede this tick
     bool busy = false;
 };
 
@@ -1314,7 +1316,9 @@ struct cah_server {
     std::deque<SrvReq*> pending;
     std::vector<SrvReq*> active;  // admitted and not yet done
     std::unordered_map<uint64_t, SrvReq*> all;
+    std::deque<uint64_t> finished;  // done since the last poll
     uint64_t next = 1;
+    double tick_ps = 0;             // estimated device work per tick (0: no limit)
 };
 
 // Device bytes a request holds while resident: both buffers, the mask, and the
@@ -1496,10 +1500,13 @@ static int retire(cah_server* s) {
             if (q != hipSuccess) return fail(CAH_ERR_DEVICE, "server tick: %s", hipGetErrorString(q));
         }
         const double now = steady_now();
+        // the uploads before the tick's table copy have completed
+        for (SrvReq* r : sl.staged) release_staging(r->st);
+        sl.staged.clear();
         for (SrvReq* r : sl.finishing) {
             r->status = CAH_REQ_DONE;
             r->t_done = now;
-            release_staging(r->st);
+            s->finished.push_back(r->id);
             s->active.erase(std::find(s->active.begin(), s->active.end(), r));
         }
         sl.finishing.clear();
@@ -1521,12 +1528,43 @@ struct SrvGroup {
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
+// Estimated device time of one cell update in picoseconds, per family, from the
+// batched kernels' rates on MI355X (Life and Larger-than-Life about 2400 GCUPS,
+// the forest fire about 600, MFMA Lenia with a 27-row kernel about 95); only the
+// size of a tick depends on it.
+static double cell_ps(const SrvReq* r) {
+    switch (r->cls) {
+    case SC_LIFE: return 0.5;
+    case SC_FOREST: return 1.7;
+    case SC_LENIA: return 11.0 * (2 * r->st->lenia_H + 1) / 27.0;
+    default: return 10.0;
+    }
+}
+
 static int build_tick(cah_server* s) {
+    // Steps go to the resident requests in arrival order: each gets up to chunk
+    // steps while the tick's work budget (if one is set) lasts, the oldest at
+    // least one, and a younger request only the steps that still fit.  Without a
+    // budget every resident request advances chunk steps a tick; measured on
+    // MI355X that serves both large-lattice and many-small-lattice traffic best,
+    // since the batched launches fill the device with the small ones at little
+    // cost to the large.
     std::vector<SrvReq*> run;
+    double budget = s->tick_ps > 0 ? s->tick_ps : 1e30;
+    bool oldest = true;
     for (SrvReq* r : s->active) {
         if (r->tail_queued) continue;
         const int64_t left = r->t1 - r->queued;
-        r->steps = (int32_t)(s->chunk > 0 ? std::min<int64_t>(s->chunk, left) : left);
+        int64_t steps = s->chunk > 0 ? std::min<int64_t>(s->chunk, left) : left;
+        if (left > 0) {
+            const double per = (double)(r->st->n1 * r->st->n2) * cell_ps(r);
+            const int64_t fit = (int64_t)std::min(budget / per, 1e18);
+            if (fit < steps) steps = oldest ? std::max<int64_t>(1, fit) : fit;
+            if (steps == 0) continue;
+            budget -= (double)steps * per;
+            oldest = false;
+        }
+        r->steps = (int32_t)steps;
         run.push_back(r);
     }
     if (run.empty()) return CAH_OK;
@@ -1687,6 +1725,8 @@ static int build_tick(cah_server* s) {
         if (r->cls == SC_LOOP && r->steps > 0)
             if (int rc = cah_advance(r->st, r->queued, r->steps, &r->p)) return rc;
     for (SrvReq* r : run) {
+        if (!r->in_tick) sl.staged.push_back(r);
+        r->in_tick = true;
         if (r->cls != SC_LOOP) {
             r->st->cur ^= r->steps & 1;
             if (r->cls == SC_FOREST && r->steps > 0) r->st->bin_clean = false;
@@ -1719,6 +1759,26 @@ extern "C" int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending)
         if (int rc = build_tick(s)) return rc;
     if (active) *active = (int32_t)s->active.size();
     if (pending) *pending = (int32_t)s->pending.size();
+    return CAH_OK;
+}
+
+extern "C" int cah_server_tune(cah_server* s, int32_t chunk, double tick_us) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
+    if (chunk < 0) return fail(CAH_ERR_ARGUMENT, "chunk must be >= 0");
+    if (!(tick_us >= 0)) return fail(CAH_ERR_ARGUMENT, "tick_us must be >= 0");
+    s->chunk = chunk;
+    s->tick_ps = tick_us * 1e6;
+    return CAH_OK;
+}
+
+extern "C" int cah_server_poll(cah_server* s, uint64_t* ids, int32_t cap, int32_t* n) {
+    if (!s || !n || (cap > 0 && !ids)) return fail(CAH_ERR_ARGUMENT, "cah_server_poll: NULL argument");
+    int32_t k = 0;
+    while (k < cap && !s->finished.empty()) {
+        ids[k++] = s->finished.front();
+        s->finished.pop_front();
+    }
+    *n = k;
     return CAH_OK;
 }
 

@@ -50,6 +50,27 @@ CAH_DEV float uniform32(uint64_t seed, uint64_t cellid, int64_t t, uint32_t k) {
     return (float)(c0 >> 8) * (1.0f / 16777216.0f);
 }
 
+// The 53-bit draw of _uniform, x = (r0 << 32 | r1) >> 11, so u = x * 2^-53.
+CAH_DEV uint64_t draw53(uint32_t r0, uint32_t r1) { return (((uint64_t)r0 << 32) | r1) >> 11; }
+
+// u < p for u = x * 2^-53 as an integer test x < thr53(p): p * 2^53 is exact
+// (a power-of-two scaling), and an integer x is below a real P exactly when it is
+// below ceil(P).  NaN and p <= 0 give 0 (never), p >= 1 gives 2^53 (always).
+CAH_DEV uint64_t thr53(double p) {
+    const double P = p * 0x1.0p53;
+    if (!(P > 0.0)) return 0;
+    if (P >= 0x1.0p53) return 1ull << 53;
+    return (uint64_t)ceil(P);
+}
+
+// The same for the Float32 draw u = x24 * 2^-24 (_uniform32) against a Float32 p.
+CAH_DEV uint32_t thr24(float p) {
+    const float P = p * 16777216.0f;
+    if (!(P > 0.0f)) return 0;
+    if (P >= 16777216.0f) return 1u << 24;
+    return (uint32_t)ceilf(P);
+}
+
 // bernoulli(ctx, p, k) (src/random.jl:129-130): the comparison stays in FP64 for a
 // Float64 p, and uses the 24-bit draw for a Float32 p.
 CAH_DEV bool bernoulli(uint64_t seed, uint64_t cellid, int64_t t, uint32_t k, double p,

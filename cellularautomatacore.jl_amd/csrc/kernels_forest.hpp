@@ -18,27 +18,26 @@ struct ForestArgs {
 // One forest-fire cell (README.md:56-60): c = state, nf = a VonNeumann neighbour is
 // on fire.  Exactly one Philox draw is formed per cell, k = 1 (growth) for an
 // empty cell and k = 2 (lightning) otherwise; it decides the cell only where the
-// reference rule draws it.  The FP64 test u < p with u = x * 2^-53 is made as
-// x < p * 2^53 (scaling by a power of two is exact), still in FP64.  PREC says
-// which of p (bit 0) and f (bit 1) are Float32 and use the 24-bit draw.  The
-// cell's result is formed with selects, not branches.
+// reference rule draws it.  The FP64 test u < p with u = x * 2^-53 (x the 53-bit
+// draw) is made exactly as the integer test x < thr53(p), the Float32 one as
+// x24 < thr24(p) (device.hpp).  PREC says which of p (bit 0) and f (bit 1) are
+// Float32 and use the 24-bit draw.  The cell's result is formed with selects,
+// not branches.
 template <bool SMALL, int PREC>
 CAH_DEV uint32_t forest_cell(uint32_t c, uint32_t nf, uint64_t cellid, int64_t t, uint64_t seed,
-                             double p53, double f53, float p32, float f32) {
+                             uint64_t tp53, uint64_t tf53, uint32_t tp24, uint32_t tf24) {
     const bool empty = c == 0;
     uint32_t c0 = (uint32_t)cellid, c1 = SMALL ? 0u : (uint32_t)(cellid >> 32),
              c2 = (uint32_t)(uint64_t)t, c3 = empty ? 1u : 2u;
     philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
     bool hit;
     if constexpr (PREC == 0) {
-        const uint64_t bits = ((uint64_t)c0 << 32) | (uint64_t)c1;
-        hit = (double)(bits >> 11) < (empty ? p53 : f53);
+        hit = draw53(c0, c1) < (empty ? tp53 : tf53);
     } else if constexpr (PREC == 3) {
-        hit = (float)(c0 >> 8) * (1.0f / 16777216.0f) < (empty ? p32 : f32);
+        hit = (c0 >> 8) < (empty ? tp24 : tf24);
     } else {
-        const uint64_t bits = ((uint64_t)c0 << 32) | (uint64_t)c1;
-        const bool hit64 = (double)(bits >> 11) < (empty ? p53 : f53);
-        const bool hit32 = (float)(c0 >> 8) * (1.0f / 16777216.0f) < (empty ? p32 : f32);
+        const bool hit64 = draw53(c0, c1) < (empty ? tp53 : tf53);
+        const bool hit32 = (c0 >> 8) < (empty ? tp24 : tf24);
         hit = (empty ? (PREC & 1) : (PREC & 2)) ? hit32 : hit64;
     }
     const uint32_t as_empty = hit ? 1u : 0u;            // growth
@@ -61,8 +60,8 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
         if constexpr (FAST) load_mapped<VB>(a.cur, a.L, lm, fill1, fill2, y, X);
         else load_row<VB>(a.cur, a.L, cv1, cv2, x, y, X);
     };
-    const double p53 = a.p * 0x1.0p53, f53 = a.f * 0x1.0p53;
-    const float p32 = (float)a.p, f32 = (float)a.f;
+    const uint64_t tp53 = thr53(a.p), tf53 = thr53(a.f);
+    const uint32_t tp24 = thr24((float)a.p), tf24 = thr24((float)a.f);
 
     uint32_t raw[S][VB], fire[S][VB];
     load(ys - 1, raw[0]);
@@ -102,7 +101,7 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
                 const uint32_t c = (c4 >> (8 * b)) & 0xFFu;
                 const uint32_t f = (nf >> (8 * b)) & 0xFFu;
                 o |= forest_cell<SMALL, PREC>(c, f, rowid + (uint64_t)(4 * i + b), a.t, a.seed,
-                                              p53, f53, p32, f32)
+                                              tp53, tf53, tp24, tf24)
                      << (8 * b);
             }
             out[i] = o;

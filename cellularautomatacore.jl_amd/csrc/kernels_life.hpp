@@ -75,14 +75,16 @@ CAH_DEV void life_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_
     }
 }
 
-// The walk of one wave down rows [ys, ye) of its band (ys, ye wave-uniform).
-// FAST: the lattice suits load_mapped, so every row is one vector load per lane
-// with the boundary applied branch-free; otherwise rows and bytes go through
-// the general boundary map.  Rows live in a ring of 2P = 4R+2 register slots, row r in slot
-// (r - ys + R) mod 2P, and each load writes its final slot directly: the slot of
-// the row that just left the window.  P = 2R+1 rows are in flight per wave.
+// The walk of one wave down rows [ys, ye) of its band (ys, ye wave-uniform), or
+// up them when `up`: the walk runs over rows y' = ys + ye - 1 - y (the Moore sums
+// are symmetric in the row offset).  FAST: the lattice suits load_mapped, so every
+// row is one vector load per lane with the boundary applied branch-free;
+// otherwise rows and bytes go through the general boundary map.  Rows live in a
+// ring of 2P = 4R+2 register slots, row r in slot (r - ys + R) mod 2P, and each
+// load writes its final slot directly: the slot of the row that just left the
+// window.  P = 2R+1 rows are in flight per wave.
 template <int R, int VB, bool BIN, int NIV, bool FAST>
-CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer) {
+CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer, bool up) {
     constexpr int P = 2 * R + 1, S = 2 * P;
     constexpr int G = (R + 3) / 4;  // guard dwords needed on each side
     static_assert(G <= VB, "a lane must hold the guard dwords of its neighbours");
@@ -90,9 +92,11 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
     const uint32_t fill1 = 0x01010101u * (uint8_t)cv1, fill2 = 0x01010101u * (uint8_t)cv2;
     const LaneMap lm = lane_map<VB>(a.L, x);
     const int32_t last = ye + R - 1;  // the last row any output of the chunk reads
+    const int32_t mirror = ys + ye - 1;
     auto load = [&](int32_t y, uint32_t (&X)[VB]) {
-        if constexpr (FAST) load_mapped<VB>(a.cur, a.L, lm, fill1, fill2, y, X);
-        else load_row<VB>(a.cur, a.L, cv1, cv2, x, y, X);
+        const int32_t ya = up ? mirror - y : y;
+        if constexpr (FAST) load_mapped<VB>(a.cur, a.L, lm, fill1, fill2, ya, X);
+        else load_row<VB>(a.cur, a.L, cv1, cv2, x, ya, X);
     };
     auto ones = [&](uint32_t (&X)[VB]) {
         if constexpr (!BIN) {
@@ -140,7 +144,7 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 #pragma unroll
         for (int i = 0; i < VB; ++i) n[i] = Wd[i] - ring[s_c][i];  // Moore leaves the centre out
         life_rule<VB, NIV>(a, n, ring[s_c], out);
-        if (writer) store_row<VB>(a.next, a.L, x, y, out);
+        if (writer) store_row<VB>(a.next, a.L, x, up ? mirror - y : y, out);
 #pragma unroll
         for (int i = 0; i < VB; ++i) V[i] -= ring[s_old][i];
         // the slot of row y - R now takes row y - R + 2P = y + R + P + 1
@@ -157,7 +161,9 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 }
 
 // One wave's work: band `wid % nbands`, rows chunk `wid / nbands` (wid is
-// wave-uniform, so the row bookkeeping stays in scalar registers).
+// wave-uniform, so the row bookkeeping stays in scalar registers).  Odd chunks
+// walk up: the two chunks beside a chunk boundary then read the rows around it
+// at the same time, so the second read finds them in L2.
 template <int R, int VB, bool BIN, int NIV>
 CAH_DEV void life_band(const LifeArgs& a, int wid) {
     constexpr int OW = 62 * 4 * VB;  // output bytes per band
@@ -167,13 +173,14 @@ CAH_DEV void life_band(const LifeArgs& a, int wid) {
     const int32_t ye = min(ys + a.rows, a.L.n2);
     const int32_t x = band * OW - 4 * VB + lane * 4 * VB;
     const bool writer = lane >= 1 && lane <= 62 && x < a.L.n1;
-    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true>(a, x, ys, ye, writer);
-    else life_walk<R, VB, BIN, NIV, false>(a, x, ys, ye, writer);
+    const bool up = chunk & 1;
+    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true>(a, x, ys, ye, writer, up);
+    else life_walk<R, VB, BIN, NIV, false>(a, x, ys, ye, writer, up);
 }
 
 template <int R, int VB, bool BIN, int NIV>
 __global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
-    const int wid = wave_id();
+    const int wid = wave_id_xcd();
     if (wid >= a.nwaves) return;  // whole waves only
     life_band<R, VB, BIN, NIV>(a, wid);
 }

@@ -225,11 +225,13 @@ int cah_batch_destroy(cah_batch* b);
  * ensembles and abc_rejection generate (src/calibration.jl:66-91), with
  * continuous batching.  Requests (any rule the backend runs, any lattice, any
  * tspan) wait in a FIFO until the device-memory budget admits them; admitted
- * lattices stay resident in HBM, each on its own HIP stream, and advance in
- * chunks of steps, so requests that arrive while others run start as soon as
- * memory frees up and their work overlaps on the device.  A request's result is
- * exactly what a cah_state with the same descriptor would give.  One host
- * thread drives a server (calls on one server must not overlap). */
+ * lattices stay resident in HBM.  Each tick advances the resident requests by up
+ * to `chunk` steps, in arrival order while the tick's work budget lasts, and the
+ * requests of one kernel family advance together, one batched launch per step
+ * over all their lattices; so requests that arrive while others run start as
+ * soon as memory frees up and their work overlaps on the device.  A request's
+ * result is exactly what a cah_state with the same descriptor would give.  One
+ * host thread drives a server (calls on one server must not overlap). */
 typedef struct cah_server cah_server;
 
 #define CAH_REQ_PENDING 0   /* waiting for memory            */
@@ -238,16 +240,25 @@ typedef struct cah_server cah_server;
 #define CAH_REQ_FETCHED 3   /* result copied out, memory freed */
 
 /* budget_bytes: device memory the resident lattices may hold; chunk: steps
- * queued per request per tick (0 = whole tspan at once). */
+ * queued per request per tick at most (0 = no limit but the tick's work). */
 int cah_server_create(cah_ctx* ctx, size_t budget_bytes, int32_t chunk, cah_server** out);
+/* Steps per request per tick at most (as in create) and the estimated device
+ * time one tick may hold, in microseconds (0, the default: no limit, every
+ * resident request advances `chunk` steps a tick; a limit gives the steps to the
+ * oldest requests first). */
+int cah_server_tune(cah_server* s, int32_t chunk, double tick_us);
 /* Queue a request: the problem (copied), its initial state (host, copied; NULL =
  * all zero), steps t0 .. t1-1 and parameters.  Returns its id. */
 int cah_server_submit(cah_server* s, const cah_desc* d, const void* u0, size_t nbytes,
                       int64_t t0, int64_t t1, const cah_params* p, uint64_t* id);
-/* One scheduling pass: retire finished chunks, admit waiting requests that fit
- * the budget, queue the next chunk of every resident request.  Never blocks on
- * the GPU.  *active / *pending (may be NULL) return the counts after the pass. */
+/* One scheduling pass: retire finished ticks, admit waiting requests that fit
+ * the budget, queue the next tick.  Never blocks on the GPU.  *active / *pending
+ * (may be NULL) return the counts after the pass. */
 int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending);
+/* The ids of requests that finished since the last poll, in the order their
+ * ticks finished: up to cap of them into ids, their number into *n (the rest
+ * wait for the next poll). */
+int cah_server_poll(cah_server* s, uint64_t* ids, int32_t cap, int32_t* n);
 /* CAH_REQ_* of a request. */
 int cah_server_status(cah_server* s, uint64_t id, int32_t* status);
 /* Copy a finished request's result out and free its device memory. */

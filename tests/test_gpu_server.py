@@ -45,25 +45,31 @@ def mixed_requests(n, seed=4):
     return probs
 
 
-@pytest.mark.parametrize("chunk", [0, 7])
-def test_mix_equals_alone(chunk):
+@pytest.mark.parametrize("chunk,tick_us", [(0, 0), (7, 0), (7, 400)])
+def test_mix_equals_alone(chunk, tick_us):
     probs = mixed_requests(30)
     # a budget that holds only a few lattices at once, so requests queue
     srv = Server(budget_bytes=3_000_000, chunk=chunk, lenia_field="mfma_f16")
+    srv.tune(chunk, tick_us)
     ids = []
     for k, p in enumerate(probs):  # requests arrive while earlier ones run
         ids.append(srv.submit(p))
         if k % 3 == 2:
             srv.tick()
-    results = {}
+    results, order = {}, []
     for _ in range(100000):
         srv.tick()
-        for rid in ids:
-            if rid not in results and srv.done(rid):
-                results[rid] = srv.fetch(rid)
+        for rid in srv.poll():
+            assert srv.done(rid) and rid not in results
+            results[rid] = srv.fetch(rid)
+            order.append(rid)
         if len(results) == len(ids):
             break
     assert len(results) == len(ids)
+    assert srv.poll() == []
+    # poll reports requests in the order their ticks finished
+    t_done = [srv.times(rid)[2] for rid in order]
+    assert t_done == sorted(t_done)
     be = HIPBackend(lenia_field="mfma_f16")
     for rid, p in zip(ids, probs):
         alone = final_state(solve(p, backend=be, save_everystep=False))
