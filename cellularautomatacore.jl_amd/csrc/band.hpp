@@ -71,24 +71,27 @@ CAH_DEV void load_row(const int8_t* __restrict__ u, const Lat& L, int8_t cv1, in
 template <int VB>
 CAH_DEV void store_row(int8_t* __restrict__ u, const Lat& L, int32_t x, int32_t y,
                        const uint32_t (&X)[VB]) {
-    int8_t* row = u + (int64_t)y * L.pitch;
+    // x >= 0 for every writer lane: an unsigned offset from the wave-uniform row
+    int8_t* row = u + (int64_t)y * L.pitch + (uint32_t)x;
     if constexpr (VB == 4) {
-        *reinterpret_cast<uint4*>(row + x) = make_uint4(X[0], X[1], X[2], X[3]);
+        *reinterpret_cast<uint4*>(row) = make_uint4(X[0], X[1], X[2], X[3]);
     } else if constexpr (VB == 2) {
-        *reinterpret_cast<uint2*>(row + x) = make_uint2(X[0], X[1]);
+        *reinterpret_cast<uint2*>(row) = make_uint2(X[0], X[1]);
     } else {
 #pragma unroll
-        for (int i = 0; i < VB; ++i) reinterpret_cast<uint32_t*>(row + x)[i] = X[i];
+        for (int i = 0; i < VB; ++i) reinterpret_cast<uint32_t*>(row)[i] = X[i];
     }
 }
 
 // Neighbour lanes' values within a wave (DPP wave shifts: lane i receives lane
 // i-1, or lane i+1).  Lanes at the wave's ends receive 0; they are guard lanes.
+// bound_ctrl: the lane without a source reads 0, so the move can fold into the
+// instruction that uses it as a DPP operand.
 CAH_DEV uint32_t from_prev_lane(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);
 }
 CAH_DEV uint32_t from_next_lane(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);
 }
 
 template <int VB>
@@ -166,12 +169,50 @@ CAH_DEV int32_t row_map(const Lat& L, int32_t y, bool& fill) {
 
 CAH_DEV uint32_t bswap32(uint32_t v) { return __builtin_amdgcn_perm(v, v, 0x00010203u); }
 
+// Buffer addressing for the mapped rows: one resource per buffer (its base in
+// scalar registers, gfx9 descriptor word 3 = 0x00020000, no range clamp: every
+// address is inside the lattice by construction), the wave-uniform row offset
+// in a scalar register and the lane's byte offset in a vector register, so a
+// row load or store costs no vector instructions for its address.  Offsets fit
+// 32 bits: mapped_ok requires pitch * n2 < 2^31.
+CAH_DEV __amdgpu_buffer_rsrc_t row_rsrc(const int8_t* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(p), 0, 0xFFFFFFFF, 0x00020000);
+}
+
 template <int VB>
-CAH_DEV void load_mapped(const int8_t* __restrict__ u, const Lat& L, const LaneMap& m,
+CAH_DEV void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint32_t (&X)[VB]) {
+    if constexpr (VB == 4) {
+        typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+        const v4 v = __builtin_bit_cast(v4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+        X[0] = v.x; X[1] = v.y; X[2] = v.z; X[3] = v.w;
+    } else if constexpr (VB == 2) {
+        typedef uint32_t v2 __attribute__((ext_vector_type(2)));
+        const v2 v = __builtin_bit_cast(v2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+        X[0] = v.x; X[1] = v.y;
+    } else {
+        X[0] = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    }
+}
+
+template <int VB>
+CAH_DEV void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, const uint32_t (&X)[VB]) {
+    if constexpr (VB == 4) {
+        typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+        __builtin_amdgcn_raw_buffer_store_b128(v4{X[0], X[1], X[2], X[3]}, r, voff, soff, 0);
+    } else if constexpr (VB == 2) {
+        typedef uint32_t v2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(v2{X[0], X[1]}, r, voff, soff, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(X[0], r, voff, soff, 0);
+    }
+}
+
+template <int VB>
+CAH_DEV void load_mapped(__amdgpu_buffer_rsrc_t u, const Lat& L, const LaneMap& m,
                          uint32_t fill1, uint32_t fill2, int32_t y, uint32_t (&X)[VB]) {
     bool frow;
     const int32_t my = row_map(L, y, frow);
-    load_vec<VB>(u + (int64_t)my * L.pitch + m.src, X);
+    buf_load<VB>(u, (uint32_t)m.src, (uint32_t)my * (uint32_t)L.pitch, X);
     uint32_t R[VB];
 #pragma unroll
     for (int i = 0; i < VB; ++i) R[i] = bswap32(X[VB - 1 - i]);
@@ -183,8 +224,16 @@ CAH_DEV void load_mapped(const int8_t* __restrict__ u, const Lat& L, const LaneM
     }
 }
 
+// A writer lane's row store in the mapped walk (x >= 0).
+template <int VB>
+CAH_DEV void store_mapped(__amdgpu_buffer_rsrc_t u, const Lat& L, int32_t x, int32_t y,
+                          const uint32_t (&X)[VB]) {
+    buf_store<VB>(u, (uint32_t)x, (uint32_t)y * (uint32_t)L.pitch, X);
+}
+
 // Whether a lattice's rows and lanes can all use load_mapped.
 template <int VB>
 CAH_DEV bool mapped_ok(const Lat& L, int32_t halo_rows) {
-    return L.n1 % (4 * VB) == 0 && L.n1 >= 64 * 4 * VB && L.n2 > halo_rows;
+    return L.n1 % (4 * VB) == 0 && L.n1 >= 64 * 4 * VB && L.n2 > halo_rows &&
+           L.pitch * (int64_t)L.n2 < (int64_t(1) << 31);
 }

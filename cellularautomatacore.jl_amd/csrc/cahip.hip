@@ -739,17 +739,38 @@ extern "C" int cah_state_stream(cah_state* s, void** stream) {
 
 static int waves_to_blocks(int64_t nwaves) { return (int)((nwaves + 3) / 4); }
 
-// Rows per wave: enough waves to fill 256 CUs several times over, but at least
-// a few windows tall so the (2R)-row warm-up stays a small share of the loads.
-// Measured on MI355X: Larger-than-Life Moore(5) at 8192^2 is fastest at 48 rows
-// per wave (the 2R-row warm-up against the number of waves in flight); the
-// Philox-bound forest fire at 32.
+// Compute units of a device (256 on MI355X), asked once.
+static int device_cus(int device) {
+    static int cus[64] = {};
+    if (device < 0 || device >= 64) return 256;
+    if (!cus[device]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0)
+            n = 256;
+        cus[device] = n;
+    }
+    return cus[device];
+}
+
+// Rows per wave.  The forest fire and Life (R = 1) are bound by the VALU and
+// like many short chunks: enough waves to fill the device several times over,
+// 32 rows at most.  Larger-than-Life (R >= 2) is bound by HBM and the VALU
+// together, and all its waves are resident at once, so the grid is packed to
+// a whole number of four-wave blocks per CU: measured on MI355X (Moore(5),
+// 8192^2, 17 bands), the step time follows rows x the most blocks on any CU,
+// and four blocks per CU is fastest (35 rows: 23.4-24.5 us; 46 rows, three
+// blocks: 24.8-25.5 us; 45 rows, a few CUs with four: 28.5 us).
 static int32_t band_rows(const cah_state* s, int64_t nbands, int R) {
     if (s->tune_rows > 0) return s->tune_rows;
-    const int64_t target_waves = R == 1 ? 4096 : 8192;
-    int64_t rows = (s->n2 * nbands + target_waves - 1) / target_waves;
-    rows = std::max<int64_t>(rows, R == 1 ? 4 : 8 * R + 8);
-    rows = std::min<int64_t>(rows, R == 1 ? 32 : 256);
+    int64_t rows;
+    if (R == 1) {
+        rows = (s->n2 * nbands + 4095) / 4096;
+        rows = std::min<int64_t>(std::max<int64_t>(rows, 4), 32);
+    } else {
+        const int64_t chunks = std::max<int64_t>(1, (int64_t)16 * device_cus(s->device) / nbands);
+        rows = (s->n2 + chunks - 1) / chunks;
+        rows = std::min<int64_t>(std::max<int64_t>(rows, 2 * R + 2), 256);
+    }
     return (int32_t)std::max<int64_t>(1, std::min<int64_t>(rows, s->n2));
 }
 

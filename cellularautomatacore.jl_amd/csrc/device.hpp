@@ -16,21 +16,34 @@ CAH_DEV uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
+constexpr uint32_t PHILOX_M0 = 0xD2511F53u, PHILOX_M1 = 0xCD9E8D57u;
+constexpr uint32_t PHILOX_W0 = 0x9E3779B9u, PHILOX_W1 = 0xBB67AE85u;
+
+// One Philox4x32 round given the round's first product p0 = M0 * c0.
+CAH_DEV void philox_round(uint64_t p0, uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                          uint32_t& k0, uint32_t& k1) {
+    const uint64_t p1 = (uint64_t)PHILOX_M1 * (uint64_t)c2;
+    const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
+    const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+    k0 += PHILOX_W0;
+    k1 += PHILOX_W1;
+}
+
+// Rounds R0 .. 9 of Philox4x32-10.
+template <int R0 = 0>
+CAH_DEV void philox_rounds(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
+                           uint32_t k1) {
+#pragma unroll
+    for (int r = R0; r < 10; ++r) philox_round((uint64_t)PHILOX_M0 * (uint64_t)c0, c0, c1, c2, c3, k0, k1);
+}
+
 CAH_DEV void philox10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
                       uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
-        const uint32_t n0 = xor3((uint32_t)(p1 >> 32), c1, k0);
-        const uint32_t n2 = xor3((uint32_t)(p0 >> 32), c3, k1);
-        c1 = (uint32_t)p1;
-        c3 = (uint32_t)p0;
-        c0 = n0;
-        c2 = n2;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
+    philox_rounds<0>(c0, c1, c2, c3, k0, k1);
 }
 
 // _philox_draw + _uniform (src/random.jl:102-120): FP64 in [0, 1).

@@ -15,35 +15,52 @@ struct ForestArgs {
     int32_t p_f32, f_f32;
 };
 
-// One forest-fire cell (README.md:56-60): c = state, nf = a VonNeumann neighbour is
-// on fire.  Exactly one Philox draw is formed per cell, k = 1 (growth) for an
-// empty cell and k = 2 (lightning) otherwise; it decides the cell only where the
-// reference rule draws it.  The FP64 test u < p with u = x * 2^-53 (x the 53-bit
-// draw) is made exactly as the integer test x < thr53(p), the Float32 one as
-// x24 < thr24(p) (device.hpp).  PREC says which of p (bit 0) and f (bit 1) are
-// Float32 and use the 24-bit draw.  The cell's result is formed with selects,
-// not branches.
-template <bool SMALL, int PREC>
-CAH_DEV uint32_t forest_cell(uint32_t c, uint32_t nf, uint64_t cellid, int64_t t, uint64_t seed,
-                             uint64_t tp53, uint64_t tf53, uint32_t tp24, uint32_t tf24) {
-    const bool empty = c == 0;
-    uint32_t c0 = (uint32_t)cellid, c1 = SMALL ? 0u : (uint32_t)(cellid >> 32),
-             c2 = (uint32_t)(uint64_t)t, c3 = empty ? 1u : 2u;
-    philox10(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
-    bool hit;
+// The forest fire (README.md:56-60) on four cells at once.  Each cell forms
+// exactly one Philox draw, k = 1 (growth) for an empty cell and k = 2 (lightning)
+// otherwise; it decides the cell only where the reference rule draws it.  The
+// FP64 test u < p with u = x * 2^-53 (x the 53-bit draw) is made exactly as the
+// integer test x < thr53(p), the Float32 one as x24 < thr24(p) (device.hpp).
+// PREC says which of p (bit 0) and f (bit 1) are Float32 and use the 24-bit draw.
+// The cell is rowid + J, and k (1 or 2) comes in as c3.  With fewer than 2^32
+// cells (SMALL) the cell id is a 32-bit word without carries, so the first
+// round's product M0 * (rowid + J) is one multiply-add, M0 * rowid + M0 * J.
+template <bool SMALL, int PREC, int J>
+CAH_DEV bool forest_draw(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, uint64_t tp53,
+                         uint64_t tf53, uint32_t tp24, uint32_t tf24) {
+    const bool empty = c3 == 1u;
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint32_t c0, c1, c2 = (uint32_t)(uint64_t)t;
+    if constexpr (SMALL) {
+        c0 = (uint32_t)rowid;
+        c1 = 0;
+        const uint64_t p0 = (uint64_t)PHILOX_M0 * c0 + (uint64_t)PHILOX_M0 * (uint32_t)J;
+        philox_round(p0, c0, c1, c2, c3, k0, k1);
+        philox_rounds<1>(c0, c1, c2, c3, k0, k1);
+    } else {
+        const uint64_t cellid = rowid + (uint64_t)J;
+        c0 = (uint32_t)cellid;
+        c1 = (uint32_t)(cellid >> 32);
+        philox10(c0, c1, c2, c3, k0, k1);
+    }
     if constexpr (PREC == 0) {
-        hit = draw53(c0, c1) < (empty ? tp53 : tf53);
+        return draw53(c0, c1) < (empty ? tp53 : tf53);
     } else if constexpr (PREC == 3) {
-        hit = (c0 >> 8) < (empty ? tp24 : tf24);
+        return (c0 >> 8) < (empty ? tp24 : tf24);
     } else {
         const bool hit64 = draw53(c0, c1) < (empty ? tp53 : tf53);
         const bool hit32 = (c0 >> 8) < (empty ? tp24 : tf24);
-        hit = (empty ? (PREC & 1) : (PREC & 2)) ? hit32 : hit64;
+        return (empty ? (PREC & 1) : (PREC & 2)) ? hit32 : hit64;
     }
-    const uint32_t as_empty = hit ? 1u : 0u;            // growth
-    const uint32_t as_tree = (nf || hit) ? 2u : 1u;     // catches fire, or lightning
-    const uint32_t r = empty ? as_empty : as_tree;
-    return c == 2 ? 0u : r;                             // a fire burns out
+}
+
+// The rule's result for four cells (one byte each), from 0/1 bytes: e4 empty,
+// f4 burning, nf4 a VonNeumann neighbour burning, hit4 the cell's draw hit.  An
+// empty cell grows a tree on a hit; a burning cell burns out; any other cell is a
+// tree, which catches fire (2) beside a fire or on a lightning hit, else stays 1.
+CAH_DEV uint32_t forest_bytes(uint32_t e4, uint32_t f4, uint32_t nf4, uint32_t hit4) {
+    const uint32_t em = (e4 << 8) - e4, fm = (f4 << 8) - f4;  // 0xFF bytes where set
+    const uint32_t tree = 0x01010101u + (nf4 | hit4);
+    return ((em & hit4) | (~em & tree)) & ~fm;
 }
 
 // The walk of one wave down rows [ys, ye) of its band, as life_walk: rows in a
@@ -55,9 +72,10 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
     const int8_t cv1 = (int8_t)a.cv1, cv2 = (int8_t)a.cv2;
     const uint32_t fill1 = 0x01010101u * (uint8_t)cv1, fill2 = 0x01010101u * (uint8_t)cv2;
     const LaneMap lm = lane_map<VB>(a.L, x);
+    const __amdgpu_buffer_rsrc_t rcur = row_rsrc(a.cur), rnext = row_rsrc(a.next);
     const int32_t last = ye;  // the last row any output of the chunk reads
     auto load = [&](int32_t y, uint32_t (&X)[VB]) {
-        if constexpr (FAST) load_mapped<VB>(a.cur, a.L, lm, fill1, fill2, y, X);
+        if constexpr (FAST) load_mapped<VB>(rcur, a.L, lm, fill1, fill2, y, X);
         else load_row<VB>(a.cur, a.L, cv1, cv2, x, y, X);
     };
     const uint64_t tp53 = thr53(a.p), tf53 = thr53(a.f);
@@ -89,24 +107,26 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
         for (int i = 0; i < VB; ++i) E[1 + i] = fire[s_c][i];
         uint32_t out[VB];
         const uint64_t rowid = (uint64_t)y * (uint64_t)a.L.n1 + (uint64_t)x + 1;
-#pragma unroll
-        for (int i = 0; i < VB; ++i) {
+        Unroll<0, VB>::run([&](auto I) {
+            constexpr int i = decltype(I)::value;
             // VonNeumann(1): left, right, up, down
             const uint32_t nf = funnel(E[1 + i], E[i], 24) | funnel(E[2 + i], E[1 + i], 8) |
                                 fire[s_up][i] | fire[s_dn][i];
-            const uint32_t c4 = raw[s_c][i];
-            uint32_t o = 0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t c = (c4 >> (8 * b)) & 0xFFu;
-                const uint32_t f = (nf >> (8 * b)) & 0xFFu;
-                o |= forest_cell<SMALL, PREC>(c, f, rowid + (uint64_t)(4 * i + b), a.t, a.seed,
-                                              tp53, tf53, tp24, tf24)
-                     << (8 * b);
-            }
-            out[i] = o;
+            const uint32_t e4 = bytes_eq(raw[s_c][i], 0u);
+            const uint32_t k4 = 0x02020202u - e4;  // the draw's k: 1 empty, 2 otherwise
+            uint32_t hit4 = 0;
+            Unroll<0, 4>::run([&](auto B) {
+                constexpr int b = decltype(B)::value;
+                const bool hit = forest_draw<SMALL, PREC, 4 * i + b>(
+                    (k4 >> (8 * b)) & 0xFFu, rowid, a.t, a.seed, tp53, tf53, tp24, tf24);
+                hit4 |= (hit ? 1u : 0u) << (8 * b);
+            });
+            out[i] = forest_bytes(e4, fire[s_c][i], nf, hit4);
+        });
+        if (writer) {
+            if constexpr (FAST) store_mapped<VB>(rnext, a.L, x, y, out);
+            else store_row<VB>(a.next, a.L, x, y, out);
         }
-        if (writer) store_row<VB>(a.next, a.L, x, y, out);
         // the slot of row y - 1 now takes row y + 5 (clamped past the chunk)
         if (more) load(min(y + 5, last), raw[s_up]);
         ++y;

@@ -91,11 +91,12 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
     const int8_t cv1 = (int8_t)a.cv1, cv2 = (int8_t)a.cv2;
     const uint32_t fill1 = 0x01010101u * (uint8_t)cv1, fill2 = 0x01010101u * (uint8_t)cv2;
     const LaneMap lm = lane_map<VB>(a.L, x);
+    const __amdgpu_buffer_rsrc_t rcur = row_rsrc(a.cur), rnext = row_rsrc(a.next);
     const int32_t last = ye + R - 1;  // the last row any output of the chunk reads
     const int32_t mirror = ys + ye - 1;
     auto load = [&](int32_t y, uint32_t (&X)[VB]) {
         const int32_t ya = up ? mirror - y : y;
-        if constexpr (FAST) load_mapped<VB>(a.cur, a.L, lm, fill1, fill2, ya, X);
+        if constexpr (FAST) load_mapped<VB>(rcur, a.L, lm, fill1, fill2, ya, X);
         else load_row<VB>(a.cur, a.L, cv1, cv2, x, ya, X);
     };
     auto ones = [&](uint32_t (&X)[VB]) {
@@ -144,7 +145,10 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 #pragma unroll
         for (int i = 0; i < VB; ++i) n[i] = Wd[i] - ring[s_c][i];  // Moore leaves the centre out
         life_rule<VB, NIV>(a, n, ring[s_c], out);
-        if (writer) store_row<VB>(a.next, a.L, x, up ? mirror - y : y, out);
+        if (writer) {
+            if constexpr (FAST) store_mapped<VB>(rnext, a.L, x, (up ? mirror - y : y), out);
+            else store_row<VB>(a.next, a.L, x, (up ? mirror - y : y), out);
+        }
 #pragma unroll
         for (int i = 0; i < VB; ++i) V[i] -= ring[s_old][i];
         // the slot of row y - R now takes row y - R + 2P = y + R + P + 1
