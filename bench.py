@@ -345,8 +345,8 @@ def serving_problems(n, seed=5, lo=64, hi=1024, tmax=300, ltl=True):
         else:
             u = np.zeros((side, side), np.float32)
             k = max(side // 8, 1)
-            lo = (side - k) // 2
-            u[lo:lo + k, lo:lo + k] = rng.random((k, k), dtype=np.float32)
+            a = (side - k) // 2
+            u[a:a + k, a:a + k] = rng.random((k, k), dtype=np.float32)
             p = CAProblem(M.Lenia(), u, steps, seed=i)
         out.append(p)
     return out
@@ -356,9 +356,14 @@ def poisson_arrivals(n, rate, seed=6):
     return np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
 
 
-def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None):
+def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inline_bytes=4 << 20):
     """The server (cah_server_*) fed in arrival order; per request the time its
-    result reached host memory, and the results."""
+    result reached host memory, and the results.  The serving thread submits,
+    ticks and polls; results of `inline_bytes` and more are copied out by
+    `clients` threads, as a server's connection threads would, while the serving
+    thread goes on (a smaller one it fetches itself: its copy takes less than
+    handing it to another Python thread)."""
+    import concurrent.futures as cf
     from cahip.server import Server
     srv = Server(budget_bytes=budget, chunk=chunk, lenia_field="mfma_f16")
     if tick_us:
@@ -372,6 +377,13 @@ def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None):
     n = len(probs)
     k, fin, res, index = 0, [0.0] * n, [None] * n, {}
     left = n
+    pool = cf.ThreadPoolExecutor(max_workers=clients) if clients else None
+    futures = []
+
+    def take(rid, j):
+        res[j] = srv.fetch(rid)
+        fin[j] = time.perf_counter() - start
+
     start = time.perf_counter()
     while left:
         now = time.perf_counter() - start
@@ -381,9 +393,15 @@ def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None):
         srv.tick()
         for rid in srv.poll():
             j = index[rid]
-            res[j] = srv.fetch(rid)
-            fin[j] = time.perf_counter() - start
+            if pool and probs[j].u0.nbytes >= inline_bytes:
+                futures.append(pool.submit(take, rid, j))
+            else:
+                take(rid, j)
             left -= 1
+    for f in futures:
+        f.result()
+    if pool:
+        pool.shutdown()
     srv.close()
     return np.array(fin), res
 

@@ -24,6 +24,7 @@
 
 #include "../../include/cahip.h"
 #include "device.hpp"
+#include "host_copy.hpp"
 #include "kernels_generic.hpp"
 #include "launch.hpp"
 #include "kernels_lenia.hpp"
@@ -398,7 +399,7 @@ static int upload(T** dst, const T* src, size_t n) {
 static int stage(cah_state* s, const void* src, size_t n, void** out) {
     *out = s->staging->alloc(n);
     if (!*out) return fail(CAH_ERR_OUT_OF_MEMORY, "cannot stage %zu bytes in pinned memory", n);
-    std::memcpy(*out, src, n);
+    hostcopy::copy(*out, src, n);
     s->keep.push_back(*out);
     return CAH_OK;
 }
@@ -617,7 +618,7 @@ static int copy_in(cah_state* s, const void* host) {
     if (int rc = stage_rows(s, &per)) return rc;
     for (int64_t y = 0; y < s->n2; y += per) {
         const size_t nb = (size_t)std::min<int64_t>(per, s->n2 - y) * row;
-        std::memcpy(g_stage, (const char*)host + (size_t)y * row, nb);
+        hostcopy::copy(g_stage, (const char*)host + (size_t)y * row, nb);
         HIPCHK(hipMemcpyAsync(dense + (size_t)y * row, g_stage, nb, hipMemcpyHostToDevice, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
     }
@@ -645,7 +646,7 @@ static int copy_out(cah_state* s, void* host) {
         const size_t nb = (size_t)std::min<int64_t>(per, s->n2 - y) * row;
         HIPCHK(hipMemcpyAsync(g_stage, dense + (size_t)y * row, nb, hipMemcpyDeviceToHost, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
-        std::memcpy((char*)host + (size_t)y * row, g_stage, nb);
+        hostcopy::copy((char*)host + (size_t)y * row, g_stage, nb);
     }
     return CAH_OK;
 }
@@ -759,7 +760,9 @@ static int device_cus(int device) {
 // a whole number of four-wave blocks per CU: measured on MI355X (Moore(5),
 // 8192^2, 17 bands), the step time follows rows x the most blocks on any CU,
 // and four blocks per CU is fastest (35 rows: 23.4-24.5 us; 46 rows, three
-// blocks: 24.8-25.5 us; 45 rows, a few CUs with four: 28.5 us).
+// blocks: 24.8-25.5 us; 45 rows, a few CUs with four: 28.5 us).  A small
+// lattice gets short chunks (4 rows at least): its waves are few, and more of
+// them finish sooner despite the 2R-row warm-up each.
 static int32_t band_rows(const cah_state* s, int64_t nbands, int R) {
     if (s->tune_rows > 0) return s->tune_rows;
     int64_t rows;
@@ -769,7 +772,7 @@ static int32_t band_rows(const cah_state* s, int64_t nbands, int R) {
     } else {
         const int64_t chunks = std::max<int64_t>(1, (int64_t)16 * device_cus(s->device) / nbands);
         rows = (s->n2 + chunks - 1) / chunks;
-        rows = std::min<int64_t>(std::max<int64_t>(rows, 2 * R + 2), 256);
+        rows = std::min<int64_t>(std::max<int64_t>(rows, 4), 256);
     }
     return (int32_t)std::max<int64_t>(1, std::min<int64_t>(rows, s->n2));
 }
@@ -818,7 +821,8 @@ static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
     a.f = p->ff_f;
     a.p_f32 = p->ff_p_is_f32;
     a.f_f32 = p->ff_f_is_f32;
-    const int vb = s->tune_vb ? std::min(s->tune_vb, 2) : 2;
+    // 4 bytes a lane below 2048 columns: twice the bands, for lattices with few waves
+    const int vb = s->tune_vb ? std::min(s->tune_vb, 2) : (s->n1 >= 2048 ? 2 : 1);
     const int64_t ow = 62 * 4 * vb;
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
     a.rows = band_rows(s, a.nbands, 1);
@@ -1045,19 +1049,26 @@ static void free_batch(cah_batch* b) {
     delete b;
 }
 
-// Wave layout of a member: rows per wave fixed for the whole batch so that
-// every wave carries about the same work.
-constexpr int32_t BATCH_ROWS = 64;
+// Rows per wave of a batched launch, one value for all its lattices so that every
+// wave carries about the same work: `work` = the sum over lattices of bands x
+// rows; about 4096 waves, 4 to 64 rows each (a lone small lattice gets short
+// chunks, a large batch long ones).
+static int64_t batch_rows(int64_t work) {
+    return std::min<int64_t>(64, std::max<int64_t>(4, (work + 4095) / 4096));
+}
 
 static int batch_layout(cah_batch* b) {
     std::vector<int32_t> wave_item, wave_base;
     std::vector<ForestArgs> fitems;
     std::vector<LifeArgs> litems;
     const int64_t ow = 62 * 4 * b->vb;
+    int64_t work = 0;
+    for (cah_state* s : b->members) work += (s->n1 + ow - 1) / ow * s->n2;
+    const int64_t brows = batch_rows(work);
     for (size_t k = 0; k < b->members.size(); ++k) {
         cah_state* s = b->members[k];
         const int32_t nbands = (int32_t)((s->n1 + ow - 1) / ow);
-        const int32_t rows = (int32_t)std::min<int64_t>(BATCH_ROWS, s->n2);
+        const int32_t rows = (int32_t)std::min<int64_t>(brows, s->n2);
         const int32_t nw = nbands * (int32_t)((s->n2 + rows - 1) / rows);
         wave_base.push_back((int32_t)wave_item.size());
         for (int32_t w = 0; w < nw; ++w) wave_item.push_back((int32_t)k);
@@ -1280,8 +1291,10 @@ extern "C" int cah_batch_destroy(cah_batch* b) {
 // the host between them and a new request waits at most about one tick.  The
 // per-tick item and wave tables are written to pinned host memory and copied in
 // one transfer on the upload stream, which also carries the admitted states'
-// initial contents; every family stream waits for that copy.  Results stay on the
-// device until fetched, through a separate copy stream.
+// initial contents; every family stream waits for that copy.  (A pipeline of
+// ticks per family, so that a family with little work per tick need not wait
+// for one with much, measured slower on ensemble traffic on MI355X.)  Results
+// stay on the device until fetched, through a separate copy stream.
 
 static double steady_now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -1299,7 +1312,9 @@ struct SrvReq {
     std::vector<int32_t> offsets, taps;
     std::vector<double> weights;
     std::vector<uint8_t> mask;
-    std::vector<char> u0;
+    std::vector<char> u0;                // initial state, when it is not in pinned memory
+    char* u0_pinned = nullptr;           // initial state in the server's pinned arena
+    size_t u0_bytes = 0;
     int64_t t0 = 0, t1 = 0, queued = 0;  // queued: the time its queued steps reach
     cah_params p{};
     cah_state* st = nullptr;
@@ -1307,6 +1322,7 @@ struct SrvReq {
     bool tail_queued = false;            // its last step is in a queued tick
     bool in_tick = false;                // some queued tick follows its uploads
     int32_t status = CAH_REQ_PENDING;
+    bool fetching = false;               // a fetch is copying its result out
     size_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int32_t steps = 0;                   // steps in the tick being built
@@ -1324,9 +1340,17 @@ struct SrvSlot {
     bool busy = false;
 };
 
+// Pinned memory that the initial states of waiting requests may hold; beyond it
+// they wait in pageable memory and are staged when admitted.
+constexpr size_t SRV_PENDING_PINNED = (size_t)4 << 30;
+
 struct cah_server {
+    // every entry point holds it, but a fetch drops it while its result is copied
+    // out, so one thread can fetch while another submits and ticks
+    std::mutex mu;
     cah_ctx* ctx = nullptr;
     size_t budget = 0, used = 0;
+    size_t pending_pinned = 0;  // pinned bytes of waiting requests' initial states
     int32_t chunk = 0;
     Arena arena;   // device memory of the resident lattices and tick tables
     Arena pinned;  // staging of initial states and results
@@ -1428,6 +1452,7 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
     const size_t want = (size_t)cells * (d->dtype == CAH_DTYPE_F32 ? 4 : 1);
     if (u0 && nbytes != want) return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", want, nbytes);
     const size_t bytes = request_bytes(*d);
+    std::unique_lock<std::mutex> lk(s->mu);
     if (bytes > s->budget)
         return fail(CAH_ERR_BUSY, "the request needs %zu bytes, more than the whole budget", bytes);
     SrvReq* r = new SrvReq();
@@ -1444,7 +1469,21 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
         r->mask.assign(d->mask, d->mask + cells);
         r->d.mask = r->mask.data();
     }
-    if (u0) r->u0.assign((const char*)u0, (const char*)u0 + nbytes);
+    if (u0 && s->pending_pinned + nbytes <= SRV_PENDING_PINNED) {
+        hipSetDevice(s->ctx->device);
+        r->u0_pinned = (char*)s->pinned.alloc(nbytes);
+        if (r->u0_pinned) {
+            r->u0_bytes = nbytes;
+            s->pending_pinned += nbytes;
+        }
+    }
+    // the copy runs without the lock; the request is not visible until it ends
+    lk.unlock();
+    if (r->u0_pinned)
+        hostcopy::copy(r->u0_pinned, u0, nbytes);
+    else if (u0)
+        r->u0.assign((const char*)u0, (const char*)u0 + nbytes);
+    lk.lock();
     r->t0 = r->queued = t0;
     r->t1 = t1;
     if (p) r->p = *p;
@@ -1474,17 +1513,28 @@ static int admit(cah_server* s, SrvReq* r) {
         return rc;
     }
     r->st = st;
-    if (!r->u0.empty()) {
+    if (r->u0_pinned || !r->u0.empty()) {
         void* h;
-        if (int rc = stage(st, r->u0.data(), r->u0.size(), &h)) return rc;
+        size_t n;
+        if (r->u0_pinned) {
+            // released with the state's other staging once the upload has completed
+            h = r->u0_pinned;
+            n = r->u0_bytes;
+            st->keep.push_back(h);
+            s->pending_pinned -= n;
+            r->u0_pinned = nullptr;
+        } else {
+            n = r->u0.size();
+            if (int rc = stage(st, r->u0.data(), n, &h)) return rc;
+        }
         const size_t row = (size_t)st->n1 * st->es, pitchb = (size_t)st->pitch * st->es;
         void* dense = row == pitchb ? st->buf[st->cur] : st->buf[1 - st->cur];
-        HIPCHK(hipMemcpyAsync(dense, h, r->u0.size(), hipMemcpyHostToDevice, s->up));
+        HIPCHK(hipMemcpyAsync(dense, h, n, hipMemcpyHostToDevice, s->up));
         if (row != pitchb)
             if (int rc = copy_rows(dense, (int64_t)row, st->buf[st->cur], (int64_t)pitchb, (int64_t)row,
                                    st->n2, s->up))
                 return rc;
-        if (st->es == 1) st->bin_clean = scan_binary((const int8_t*)r->u0.data(), r->u0.size());
+        if (st->es == 1) st->bin_clean = scan_binary((const int8_t*)h, n);
         std::vector<char>().swap(r->u0);
     }
     switch (st->path) {
@@ -1544,6 +1594,7 @@ struct SrvGroup {
     int cls = 0, key = 0;
     std::vector<SrvReq*> reqs;
     size_t off_items = 0, off_map = 0, off_base = 0;  // in the tick's table buffer
+    int64_t rows = 64;                                 // rows per wave (band walks)
     std::vector<int32_t> base;                         // first wave/block of each item, + total
 };
 
@@ -1608,6 +1659,12 @@ static int build_tick(cah_server* s) {
         std::stable_sort(g.reqs.begin(), g.reqs.end(),
                          [](const SrvReq* a, const SrvReq* b) { return a->steps > b->steps; });
         std::vector<int32_t> map;
+        int64_t work = 0;
+        for (const SrvReq* r : g.reqs) {
+            const int64_t ow = 62 * 4 * (g.cls == SC_LIFE && r->st->life_R == 5 ? 2 : 1);
+            work += (r->st->n1 + ow - 1) / ow * r->st->n2;
+        }
+        g.rows = batch_rows(work);
         for (size_t k = 0; k < g.reqs.size(); ++k) {
             const cah_state* st = g.reqs[k]->st;
             int32_t n;
@@ -1616,7 +1673,7 @@ static int build_tick(cah_server* s) {
             } else {
                 const int vb = g.cls == SC_LIFE && st->life_R == 5 ? 2 : 1;
                 const int64_t ow = 62 * 4 * vb;
-                const int64_t rows = std::min<int64_t>(BATCH_ROWS, st->n2);
+                const int64_t rows = std::min<int64_t>(g.rows, st->n2);
                 n = (int32_t)(((st->n1 + ow - 1) / ow) * ((st->n2 + rows - 1) / rows));
             }
             g.base.push_back((int32_t)map.size());
@@ -1663,7 +1720,7 @@ static int build_tick(cah_server* s) {
                 a.cv1 = (int32_t)st->d.boundary_value[0];
                 a.cv2 = (int32_t)st->d.boundary_value[1];
                 a.nbands = (int32_t)((st->n1 + 62 * 4 * vb - 1) / (62 * 4 * vb));
-                a.rows = (int32_t)std::min<int64_t>(BATCH_ROWS, st->n2);
+                a.rows = (int32_t)std::min<int64_t>(g.rows, st->n2);
                 a.nwaves = n;
                 std::memcpy(sl.host + g.off_items + k * sizeof a, &a, sizeof a);
             } else if (g.cls == SC_FOREST) {
@@ -1674,7 +1731,7 @@ static int build_tick(cah_server* s) {
                 a.cv1 = (int32_t)st->d.boundary_value[0];
                 a.cv2 = (int32_t)st->d.boundary_value[1];
                 a.nbands = (int32_t)((st->n1 + 247) / 248);
-                a.rows = (int32_t)std::min<int64_t>(BATCH_ROWS, st->n2);
+                a.rows = (int32_t)std::min<int64_t>(g.rows, st->n2);
                 a.nwaves = n;
                 a.seed = st->d.seed;
                 a.t = r->queued;
@@ -1766,6 +1823,7 @@ static int build_tick(cah_server* s) {
 
 extern "C" int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending) {
     if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
     HIPCHK(hipSetDevice(s->ctx->device));
     if (int rc = retire(s)) return rc;
     // admit in arrival order while the budget allows
@@ -1787,6 +1845,7 @@ extern "C" int cah_server_tune(cah_server* s, int32_t chunk, double tick_us) {
     if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
     if (chunk < 0) return fail(CAH_ERR_ARGUMENT, "chunk must be >= 0");
     if (!(tick_us >= 0)) return fail(CAH_ERR_ARGUMENT, "tick_us must be >= 0");
+    std::lock_guard<std::mutex> lk(s->mu);
     s->chunk = chunk;
     s->tick_ps = tick_us * 1e6;
     return CAH_OK;
@@ -1794,6 +1853,7 @@ extern "C" int cah_server_tune(cah_server* s, int32_t chunk, double tick_us) {
 
 extern "C" int cah_server_poll(cah_server* s, uint64_t* ids, int32_t cap, int32_t* n) {
     if (!s || !n || (cap > 0 && !ids)) return fail(CAH_ERR_ARGUMENT, "cah_server_poll: NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
     int32_t k = 0;
     while (k < cap && !s->finished.empty()) {
         ids[k++] = s->finished.front();
@@ -1811,6 +1871,8 @@ static int find_req(cah_server* s, uint64_t id, SrvReq** r) {
 }
 
 extern "C" int cah_server_status(cah_server* s, uint64_t id, int32_t* status) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
     SrvReq* r;
     if (int rc = find_req(s, id, &r)) return rc;
     if (!status) return fail(CAH_ERR_ARGUMENT, "status is NULL");
@@ -1819,9 +1881,12 @@ extern "C" int cah_server_status(cah_server* s, uint64_t id, int32_t* status) {
 }
 
 extern "C" int cah_server_fetch(cah_server* s, uint64_t id, void* host, size_t nbytes) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
     SrvReq* r;
     if (int rc = find_req(s, id, &r)) return rc;
-    if (r->status != CAH_REQ_DONE) return fail(CAH_ERR_BUSY, "request %llu is not finished", (unsigned long long)id);
+    if (r->status != CAH_REQ_DONE || r->fetching)
+        return fail(CAH_ERR_BUSY, "request %llu is not finished or is being fetched", (unsigned long long)id);
     if (!host) return fail(CAH_ERR_ARGUMENT, "cah_server_fetch: NULL argument");
     cah_state* st = r->st;
     if (nbytes != state_bytes(st))
@@ -1837,11 +1902,21 @@ extern "C" int cah_server_fetch(cah_server* s, uint64_t id, void* host, size_t n
     }
     char* h = (char*)s->pinned.alloc(nbytes);
     if (!h) return fail(CAH_ERR_OUT_OF_MEMORY, "cannot stage %zu bytes in pinned memory", nbytes);
-    const hipError_t e = hipMemcpyAsync(h, dense, nbytes, hipMemcpyDeviceToHost, s->copy);
-    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(s->copy) : e;
-    if (e2 == hipSuccess) std::memcpy(host, h, nbytes);
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipMemcpyAsync(h, dense, nbytes, hipMemcpyDeviceToHost, s->copy);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(ev, s->copy);
+    // wait and copy without the lock: the request stays DONE and is not freed
+    // meanwhile, and a second fetch of it is refused
+    r->fetching = true;
+    lk.unlock();
+    if (e == hipSuccess) e = hipEventSynchronize(ev);
+    if (e == hipSuccess) hostcopy::copy(host, h, nbytes);
+    if (ev) hipEventDestroy(ev);
+    lk.lock();
+    r->fetching = false;
     s->pinned.release(h);
-    HIPCHK(e2);
+    HIPCHK(e);
     // every launch that used its blocks has completed: back to the arena
     free_state(st);
     r->st = nullptr;
@@ -1852,6 +1927,8 @@ extern "C" int cah_server_fetch(cah_server* s, uint64_t id, void* host, size_t n
 
 extern "C" int cah_server_times(cah_server* s, uint64_t id, double* t_submit, double* t_admit,
                                 double* t_done) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
     SrvReq* r;
     if (int rc = find_req(s, id, &r)) return rc;
     if (t_submit) *t_submit = r->t_submit;

@@ -230,8 +230,11 @@ int cah_batch_destroy(cah_batch* b);
  * requests of one kernel family advance together, one batched launch per step
  * over all their lattices; so requests that arrive while others run start as
  * soon as memory frees up and their work overlaps on the device.  A request's
- * result is exactly what a cah_state with the same descriptor would give.  One
- * host thread drives a server (calls on one server must not overlap). */
+ * result is exactly what a cah_state with the same descriptor would give.
+ * Calls on one server may come from several host threads and are serialized,
+ * except that a fetch copies its result out without holding the server: one
+ * thread can fetch results while another submits and ticks.  No call may
+ * overlap cah_server_destroy. */
 typedef struct cah_server cah_server;
 
 #define CAH_REQ_PENDING 0   /* waiting for memory            */
@@ -261,7 +264,8 @@ int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending);
 int cah_server_poll(cah_server* s, uint64_t* ids, int32_t cap, int32_t* n);
 /* CAH_REQ_* of a request. */
 int cah_server_status(cah_server* s, uint64_t id, int32_t* status);
-/* Copy a finished request's result out and free its device memory. */
+/* Copy a finished request's result out and free its device memory
+ * (CAH_ERR_BUSY while another thread fetches it). */
 int cah_server_fetch(cah_server* s, uint64_t id, void* host, size_t nbytes);
 /* Host timestamps (seconds, steady clock) of submission, admission and the tick
  * that saw the request finish. */

@@ -63,6 +63,21 @@ def test_fft_needs_periodic_edges():
         HIPBackend(lenia_field="fft_f32").check_problem(prob, M.Synchronous())
 
 
+@pytest.mark.parametrize("edges", [(M.Reflecting(), M.Constant(0.25)), (M.Constant(0.5), M.Reflecting()),
+                                   (M.Periodic(), M.Constant(0.75))])
+def test_mfma_chunked_edges(oracle, edges):
+    # wide enough (first side >= 116, a multiple of 4) for the tile fill that maps
+    # whole float4 chunks at the edges; the state reaches every edge
+    dims = (160, 136)
+    u = np.asfortranarray(np.random.default_rng(11).random(dims).astype(np.float32))
+    prob = CAProblem(M.Lenia(R=9), u, 3, space=M.Lattice(dims, edges), seed=2)
+    sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field="mfma_f16x2"), saveat=1)
+    d, keep = desc_for(prob, M.Synchronous(), field_method=0)
+    for t in range(3):
+        ref = oracle.advance_plain(d, sol.u[t], t, 1)
+        assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() < 1e-5
+
+
 def test_mfma_with_edges_and_gates(oracle):
     # Reflecting/Constant edges and a Stochastic gate through the fused MFMA kernel
     u = patch_state((96, 80), (10, 10), (60, 50), 3)

@@ -3,6 +3,7 @@ others run give, each, exactly the result of solving that problem alone."""
 import numpy as np
 import pytest
 
+from cahip import _lib as L
 from cahip import model as M
 from cahip.backend import HIPBackend
 from cahip.problem import CAProblem, final_state, solve
@@ -112,4 +113,64 @@ def test_budget_refuses_a_request_larger_than_it():
     srv = Server(budget_bytes=10_000, chunk=8)
     with pytest.raises(M.OutOfMemoryError):
         srv.submit(CAProblem(M.Life(), np.zeros((200, 200), np.int8), 5))
+    srv.close()
+
+
+def test_fetch_from_threads_while_ticking():
+    """Client threads copy results out (lattices of 5 MB and more, which the host
+    copy splits over its helper threads) while the serving thread submits and
+    ticks; the budget holds a few lattices, so fetches free room for waiting
+    requests."""
+    import queue
+    import threading
+    rng = np.random.default_rng(21)
+    probs = []
+    for i in range(9):
+        if i % 3 == 0:
+            u = rng.integers(0, 2, (2304, 2112)).astype(np.int8)
+            probs.append(CAProblem(M.Life(), u, 12, seed=i))
+        elif i % 3 == 1:
+            u = rng.choice(np.array([0, 1, 2], np.int8), (2200, 2400))
+            probs.append(CAProblem(M.ForestFire(), u, 9, {"p": 0.05, "f": 0.001}, seed=i))
+        else:
+            u = np.zeros((1200, 1100), np.float32)
+            u[100:400, 200:500] = rng.random((300, 300)).astype(np.float32)
+            probs.append(CAProblem(M.Lenia(), u, 5, seed=i))
+    srv = Server(budget_bytes=40_000_000, chunk=4, lenia_field="mfma_f16")
+    work, results, errors = queue.Queue(), {}, []
+
+    def client():
+        while (rid := work.get()) is not None:
+            try:
+                results[rid] = srv.fetch(rid)
+            except Exception as e:  # reported below
+                errors.append(e)
+
+    threads = [threading.Thread(target=client) for _ in range(3)]
+    for t in threads:
+        t.start()
+    ids, polled = [], 0
+    try:
+        for p in probs:
+            ids.append(srv.submit(p))
+            srv.tick()
+        for _ in range(1000000):
+            if polled == len(ids):
+                break
+            srv.tick()
+            for rid in srv.poll():
+                work.put(rid)
+                polled += 1
+    finally:
+        for _ in threads:
+            work.put(None)
+        for t in threads:
+            t.join()
+    assert not errors, errors
+    assert polled == len(ids) and len(results) == len(ids)
+    be = HIPBackend(lenia_field="mfma_f16")
+    for rid, p in zip(ids, probs):
+        alone = final_state(solve(p, backend=be, save_everystep=False))
+        assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs"
+        assert srv.status(rid) == L.REQ_FETCHED
     srv.close()
