@@ -588,8 +588,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNELS["forest"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("forest", {}).get("bytes_per_launch"),
-                     "note": "one Philox4x32-10 draw per cell (20 v_mad_u64_u32): bound by "
-                             "the VALU, not by HBM"},
+                     "note": "one Philox4x32-10 draw per cell (17 v_mad_u64_u32 / v_mul_*_u32 "
+                             "and 18 v_bitop3_b32): bound by the VALU, not by HBM"},
         "roofline_ltl": {"bound": "hbm", "kernel": KERNELS["ltl"], "achieved": round(achieved_ltl, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved_ltl / HBM_PEAK_GBS, 4),
