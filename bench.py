@@ -167,6 +167,27 @@ def timed(ev, stream, fn):
     return ev.elapsed_ms(a, b)
 
 
+def copy_peak(ev, nbytes, reps=40):
+    """The device's copy rate at a sweep launch's working set: torch's copy_ (a
+    blit kernel) of nbytes to another buffer, `reps` times between HIP events on
+    torch's stream; bytes read + written per second, in GB/s."""
+    import torch
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(5):
+        dst.copy_(src)
+
+    def run():
+        for _ in range(reps):
+            dst.copy_(src)
+
+    ms = timed(ev, stream, run)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes * reps / (ms * 1e-3) / 1e9
+
+
 def bench_life512(be, ev, steps=1000):
     """Conway Life B3/S23, Torus 512^2, Bernoulli(0.5) seed 1, 1000 steps."""
     from cahip import model as M
@@ -556,6 +577,10 @@ def main():
             traffic = json.load(f)
     if args.copy_calibration:
         copy_calibration(be, s_for)
+    # measured copy rate at each kernel's working set (half its algorithmic
+    # bytes read, half written), beside the nominal 8 TB/s
+    cp_ltl = copy_peak(ev, ALGO_BYTES["ltl"] // 2)
+    cp_for = copy_peak(ev, ALGO_BYTES["forest"] // 2)
     # the same K steps with the states handed over in host memory: one
     # cah_state_set before and one cah_state_get after each lattice's run
     host_ms = 0.0
@@ -588,12 +613,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNELS["forest"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("forest", {}).get("bytes_per_launch"),
+                     "copy_gbs": round(cp_for, 1), "frac_of_copy": round(achieved / cp_for, 4),
                      "note": "one Philox4x32-10 draw per cell (17 v_mad_u64_u32 / v_mul_*_u32 "
                              "and 18 v_bitop3_b32): bound by the VALU, not by HBM"},
         "roofline_ltl": {"bound": "hbm", "kernel": KERNELS["ltl"], "achieved": round(achieved_ltl, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved_ltl / HBM_PEAK_GBS, 4),
-                         "traffic": traffic.get("ltl", {}).get("bytes_per_launch")},
+                         "traffic": traffic.get("ltl", {}).get("bytes_per_launch"),
+                         "copy_gbs": round(cp_ltl, 1), "frac_of_copy": round(achieved_ltl / cp_ltl, 4)},
     }
     be.free_state(s_ltl)
     be.free_state(s_for)

@@ -88,6 +88,7 @@ SIGNATURES = {
     "cah_state_get": (_I, [_P, _P, _SZ]),
     "cah_state_set_device": (_I, [_P, _P, _SZ]),
     "cah_state_get_device": (_I, [_P, _P, _SZ]),
+    "cah_state_counts": (_I, [_P, _P, _P, _P]),
     "cah_state_read_cell": (_I, [_P, _I64, _I64, _P]),
     "cah_state_write_cell": (_I, [_P, _I64, _I64, _P]),
     "cah_advance": (_I, [_P, _I64, _I64, _PP]),
@@ -102,6 +103,7 @@ SIGNATURES = {
     "cah_batch_get": (_I, [_P, _I32, _P, _SZ]),
     "cah_batch_fill_bernoulli": (_I, [_P, _I32, _U64, _D]),
     "cah_batch_advance": (_I, [_P, _I64, _I64, _PP, _I32]),
+    "cah_batch_counts": (_I, [_P, _P, _P, _P]),
     "cah_batch_sync": (_I, [_P]),
     "cah_batch_stream": (_I, [_P, ctypes.POINTER(_P)]),
     "cah_batch_destroy": (_I, [_P]),

@@ -10,6 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
+from . import measures as MS
 from . import model as M
 
 _ERRORS = {
@@ -97,6 +98,11 @@ def make_desc(prob, scheme, field_method=L.FIELD_DIRECT_F32):
         keep.append(m)
         d.mask = m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
     return d, keep
+
+
+def count_dict(counts):
+    """{Int8 value: count} over the values present, from counts by byte value."""
+    return {(v if v < 128 else v - 256): int(c) for v, c in enumerate(counts) if c}
 
 
 class HIPState:
@@ -197,10 +203,51 @@ class HIPBackend:
             self.lib.cah_state_destroy(st.handle)
             st.handle = None
 
+    # -------------------------------------------------------------- summaries
+
+    def _counts(self, st, other=None):
+        """cah_state_counts: counts by byte value (256) and, with `other`, the cells
+        where the two states agree."""
+        counts = np.zeros(256, np.int64)
+        eq = np.zeros(1, np.int64)
+        check(self.lib.cah_state_counts(st.handle, other.handle if other is not None else None,
+                                        counts.ctypes.data, eq.ctypes.data if other is not None else None))
+        return counts, (int(eq[0]) if other is not None else None)
+
+    # state_counts(current_state(backend, st)) (src/measures.jl:14-20), counted on
+    # the device (Int8 states)
+    def state_counts(self, st):
+        return count_dict(self._counts(st)[0])
+
+    # cohens_kappa(current_state(st), current_state(observed)) (src/evaluation.jl:30-50)
+    # from device counts; `observed` is a state of the same lattice
+    def cohens_kappa(self, st, observed):
+        cp, eq = self._counts(st, observed)
+        co, _ = self._counts(observed)
+        return MS.kappa_from_counts(count_dict(cp), count_dict(co), eq, int(np.prod(st.dims)))
+
+    def state_like(self, prob, scheme, u):
+        """A state of prob's lattice and rule holding `u` (an observed map to
+        compare trajectories with)."""
+        d, _keep = make_desc(prob, scheme, self.lenia_field)
+        h = ctypes.c_void_p()
+        check(self.lib.cah_state_create(self.ctx(), ctypes.byref(d), ctypes.byref(h)))
+        st = HIPState(h, prob.space.dims, prob.u0.dtype)
+        self.set_state(st, u)
+        return st
+
     # -------------------------------------------------------------- batches
 
-    def solve_batch(self, probs, scheme, use_graph=True):
-        """Final states of several problems of one Int8 rule, advanced together."""
+    def solve_batch(self, probs, scheme, use_graph=True, output="state", observed=None):
+        """Several problems of one Int8 rule, advanced together.  output="state":
+        their final states; "counts": state_counts of each final state; "kappa":
+        cohens_kappa of each final state against `observed` (a map of the
+        problems' common dims).  The last two are counted on the device, so only
+        counters leave it."""
+        if output not in ("state", "counts", "kappa"):
+            raise M.ArgumentError(f"unknown output {output!r}")
+        if output == "kappa" and observed is None:
+            raise M.ArgumentError("output='kappa' needs the observed map")
         descs, keeps = [], []
         t0, t1 = probs[0].tspan
         for pr in probs:
@@ -218,6 +265,8 @@ class HIPBackend:
                 check(self.lib.cah_batch_set(h, i, u.ctypes.data, u.nbytes))
             cp = make_params(probs[0], probs[0].p)
             check(self.lib.cah_batch_advance(h, t0, t1 - t0, ctypes.byref(cp), int(use_graph)))
+            if output != "state":
+                return self._batch_summaries(h, probs, scheme, output, observed)
             outs = []
             for i, pr in enumerate(probs):
                 out = np.empty(pr.space.dims, dtype=pr.u0.dtype, order="F")
@@ -226,6 +275,23 @@ class HIPBackend:
             return outs
         finally:
             self.lib.cah_batch_destroy(h)
+
+    def _batch_summaries(self, h, probs, scheme, output, observed):
+        n = len(probs)
+        counts = np.zeros((n, 256), np.int64)
+        eq = np.zeros(n, np.int64)
+        obs = self.state_like(probs[0], scheme, observed) if output == "kappa" else None
+        try:
+            check(self.lib.cah_batch_counts(h, obs.handle if obs is not None else None, counts.ctypes.data,
+                                            eq.ctypes.data if obs is not None else None))
+            if output == "counts":
+                return [count_dict(c) for c in counts]
+            co = count_dict(self._counts(obs)[0])
+            cells = int(np.prod(obs.dims))
+            return [MS.kappa_from_counts(count_dict(c), co, int(e), cells) for c, e in zip(counts, eq)]
+        finally:
+            if obs is not None:
+                self.free_state(obs)
 
     def __del__(self):
         if getattr(self, "_ctx", None) is not None:

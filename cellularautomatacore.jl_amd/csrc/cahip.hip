@@ -25,6 +25,7 @@
 #include "../../include/cahip.h"
 #include "device.hpp"
 #include "host_copy.hpp"
+#include "kernels_counts.hpp"
 #include "kernels_generic.hpp"
 #include "launch.hpp"
 #include "kernels_lenia.hpp"
@@ -205,6 +206,7 @@ struct cah_state {
     float* wtab_d = nullptr;   // dense table for k_lenia_f32col
     void* field_d = nullptr;   // field buffer for the sequential path
     void* frag_d = nullptr;    // MFMA A fragments
+    unsigned long long* counts_d = nullptr;  // CNT_SLOTS counters of cah_state_counts
     int lenia_mode = 0;        // LM_F16 / LM_BF16 / LM_F16X2
     LeniaFFT* fft = nullptr;
     int32_t halo = 1;
@@ -381,7 +383,8 @@ static void free_state(cah_state* s) {
     if (s->staging) release_staging(s);
     if (s->stream && !s->arena) hipStreamSynchronize(s->stream);
     for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
-                    (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d, s->frag_d})
+                    (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d, s->frag_d,
+                    (void*)s->counts_d})
         dev_free(s, p);
     lenia_fft_destroy(s->fft);
     if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
@@ -691,6 +694,83 @@ extern "C" int cah_state_get_device(cah_state* s, void* dptr, size_t nbytes) {
                            (int64_t)(s->n1 * s->es), s->n2, s->stream))
         return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+// ---------------------------------------------------------------- counts
+
+// States the rule produces (0 .. K-1), which k_counts counts in registers.
+static int rule_values(const cah_state* s) {
+    switch (s->d.rule) {
+    case CAH_RULE_LIFE: return 2;
+    case CAH_RULE_FOREST_FIRE: return 3;
+    case CAH_RULE_GENERATIONS: return std::max(2, s->d.gen_states);
+    default: return 16;
+    }
+}
+
+static int counts_check(const cah_state* s, const cah_state* o) {
+    if (s->d.dtype != CAH_DTYPE_I8)
+        return fail(CAH_ERR_ARGUMENT, "cell counts on the device take Int8 states");
+    if (o && (o->d.dtype != CAH_DTYPE_I8 || o->n1 != s->n1 || o->n2 != s->n2 || o->pitch != s->pitch))
+        return fail(CAH_ERR_STATE_SHAPE, "the states compared must be Int8 lattices of the same dims");
+    return CAH_OK;
+}
+
+// Queue the counts of the values 0 .. K-1 of s, and its agreement with o, into
+// out_d[0 .. CNT_SLOTS) (cleared first).
+static int queue_counts(const cah_state* s, const cah_state* o, unsigned long long* out_d, hipStream_t st) {
+    HIPCHK(hipMemsetAsync(out_d, 0, CNT_SLOTS * sizeof(unsigned long long), st));
+    const int64_t work = (s->n1 + CNT_SEG - 1) / CNT_SEG * s->n2;  // wave-iterations
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((work + 15) / 16, 2048));
+    const uint8_t* a = (const uint8_t*)s->buf[s->cur];
+    const uint8_t* b = o ? (const uint8_t*)o->buf[o->cur] : nullptr;
+    const int K = rule_values(s);
+    if (K <= 2) hipLaunchKernelGGL(k_counts<2>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
+    else if (K <= 3) hipLaunchKernelGGL(k_counts<3>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
+    else if (K <= 4) hipLaunchKernelGGL(k_counts<4>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
+    else if (K <= 8) hipLaunchKernelGGL(k_counts<8>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
+    else hipLaunchKernelGGL(k_counts<16>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
+    HIPCHK(hipGetLastError());
+    return CAH_OK;
+}
+
+// h: the CNT_SLOTS counters of s copied back.  When the counted values do not
+// cover every cell, the lattice holds values outside 0 .. K-1: count all 256
+// byte values instead (h[0 .. 256) replaced; h[256] kept).
+static int finish_counts(const cah_state* s, unsigned long long* out_d, unsigned long long* h, hipStream_t st) {
+    unsigned long long sum = 0;
+    for (int v = 0; v < 256; ++v) sum += h[v];
+    const int64_t n = s->n1 * s->n2;
+    if (sum == (unsigned long long)n) return CAH_OK;
+    HIPCHK(hipMemsetAsync(out_d, 0, 256 * sizeof(unsigned long long), st));
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_hist256, blocks, 256, 0, st, (const uint8_t*)s->buf[s->cur], s->pitch, s->n1, s->n2,
+                       out_d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h, out_d, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return CAH_OK;
+}
+
+static void store_counts(const unsigned long long* h, int64_t* counts, int64_t* equal) {
+    for (int v = 0; v < 256; ++v) counts[v] = (int64_t)h[v];
+    if (equal) *equal = (int64_t)h[256];
+}
+
+extern "C" int cah_state_counts(cah_state* s, const cah_state* other, int64_t* counts, int64_t* equal) {
+    if (!s || !counts || (other && !equal)) return fail(CAH_ERR_ARGUMENT, "cah_state_counts: NULL argument");
+    if (int rc = counts_check(s, other)) return rc;
+    HIPCHK(hipSetDevice(s->device));
+    if (other && other->stream != s->stream) HIPCHK(hipStreamSynchronize(other->stream));
+    if (!s->counts_d)
+        if (int rc = dev_alloc(s, (void**)&s->counts_d, CNT_SLOTS * sizeof(unsigned long long))) return rc;
+    if (int rc = queue_counts(s, other, s->counts_d, s->stream)) return rc;
+    unsigned long long h[CNT_SLOTS];
+    HIPCHK(hipMemcpyAsync(h, s->counts_d, sizeof h, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (int rc = finish_counts(s, s->counts_d, h, s->stream)) return rc;
+    store_counts(h, counts, equal);
     return CAH_OK;
 }
 
@@ -1035,6 +1115,7 @@ struct cah_batch {
     std::vector<ForestArgs> fitems;  // host copy of the forest items (p and f change)
     bool fp_set = false;
     cah_params fp{};
+    unsigned long long* counts_d = nullptr;  // CNT_SLOTS counters per member
 };
 
 static void free_batch(cah_batch* b) {
@@ -1042,7 +1123,7 @@ static void free_batch(cah_batch* b) {
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
-    for (void* p : {b->items_d, (void*)b->wave_item_d, (void*)b->wave_base_d})
+    for (void* p : {b->items_d, (void*)b->wave_item_d, (void*)b->wave_base_d, (void*)b->counts_d})
         if (p) hipFree(p);
     for (auto* m : b->members) free_state(m);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -1172,6 +1253,29 @@ extern "C" int cah_batch_get(cah_batch* b, int32_t i, void* host, size_t nbytes)
     if (int rc = member(b, i, &s)) return rc;
     HIPCHK(hipStreamSynchronize(b->stream));
     return cah_state_get(s, host, nbytes);
+}
+
+extern "C" int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* counts, int64_t* equal) {
+    if (!b || !counts || (other && !equal)) return fail(CAH_ERR_ARGUMENT, "cah_batch_counts: NULL argument");
+    for (cah_state* m : b->members)
+        if (int rc = counts_check(m, other)) return rc;
+    HIPCHK(hipSetDevice(b->device));
+    if (other && other->stream != b->stream) HIPCHK(hipStreamSynchronize(other->stream));
+    const size_t n = b->members.size();
+    if (!b->counts_d) HIPCHK(hipMalloc((void**)&b->counts_d, n * CNT_SLOTS * sizeof(unsigned long long)));
+    // every member's counts on the batch stream, then one copy back
+    for (size_t i = 0; i < n; ++i)
+        if (int rc = queue_counts(b->members[i], other, b->counts_d + i * CNT_SLOTS, b->stream)) return rc;
+    std::vector<unsigned long long> h(n * CNT_SLOTS);
+    HIPCHK(hipMemcpyAsync(h.data(), b->counts_d, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < n; ++i) {
+        unsigned long long* hi = h.data() + i * CNT_SLOTS;
+        if (int rc = finish_counts(b->members[i], b->counts_d + i * CNT_SLOTS, hi, b->stream)) return rc;
+        store_counts(hi, counts + i * 256, equal ? equal + i : nullptr);
+    }
+    return CAH_OK;
 }
 
 extern "C" int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, double p) {

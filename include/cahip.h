@@ -166,6 +166,15 @@ int cah_state_get_device(cah_state* st, void* dptr, size_t nbytes);
 int cah_state_read_cell(cah_state* st, int64_t i1, int64_t i2, void* value);
 int cah_state_write_cell(cah_state* st, int64_t i1, int64_t i2, const void* value);
 
+/* state_counts(current_state(backend, st)) (src/measures.jl:14-20) on the
+ * device, for Int8 states: counts[v] = cells holding the byte value v (256
+ * entries; index a negative Int8 value by v & 0xFF).  With `other` (an Int8
+ * state of the same dims, or NULL), *equal = the cells where the two states
+ * agree, the count(predicted .== observed) of cohens_kappa
+ * (src/evaluation.jl:43-47).  2 KiB leave the device instead of the state;
+ * synchronises. */
+int cah_state_counts(cah_state* st, const cah_state* other, int64_t* counts, int64_t* equal);
+
 /* advance!(backend, st, prob, scheme, p, t) (src/backends.jl:130, :494-506;
  * ext/...KernelAbstractionsExt.jl:92-110), `nsteps` times: steps t0, t0+1, ...
  * Work is queued on the state's stream; it returns before the GPU finishes
@@ -216,6 +225,11 @@ int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, double p);
  * launches are captured once into a hipGraph and replayed. */
 int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const cah_params* p,
                       int32_t use_graph);
+/* cah_state_counts of every trajectory: counts[i*256 + v], equal[i] (other may
+ * be NULL; it must match every trajectory's dims otherwise), as an ensemble's
+ * output_func that reduces each trajectory to its state counts or its agreement
+ * with an observed map (src/calibration.jl:76) would compute them. */
+int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* counts, int64_t* equal);
 int cah_batch_sync(cah_batch* b);
 int cah_batch_stream(cah_batch* b, void** stream);
 int cah_batch_destroy(cah_batch* b);
