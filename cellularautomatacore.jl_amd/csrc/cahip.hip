@@ -717,20 +717,32 @@ static int counts_check(const cah_state* s, const cah_state* o) {
     return CAH_OK;
 }
 
+// Device words of a counts buffer: the CNT_SLOTS results of `n` lattices and
+// one set of per-block partial sums that their launches share in turn.
+static size_t counts_words(size_t n) { return n * CNT_SLOTS + (size_t)CNT_BLOCKS * CNT_PART; }
+
+template <int K>
+static void launch_counts(int blocks, const uint8_t* a, const uint8_t* b, const cah_state* s,
+                          unsigned long long* part, unsigned long long* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_counts<K>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, part);
+    hipLaunchKernelGGL(k_counts_reduce<K>, 1, 256, 0, st, part, blocks, out);
+}
+
 // Queue the counts of the values 0 .. K-1 of s, and its agreement with o, into
-// out_d[0 .. CNT_SLOTS) (cleared first).
-static int queue_counts(const cah_state* s, const cah_state* o, unsigned long long* out_d, hipStream_t st) {
-    HIPCHK(hipMemsetAsync(out_d, 0, CNT_SLOTS * sizeof(unsigned long long), st));
+// out_d[0 .. CNT_SLOTS) (every slot written), through the partial sums part_d
+// (CNT_BLOCKS * CNT_PART words; launches on one stream may share them).
+static int queue_counts(const cah_state* s, const cah_state* o, unsigned long long* out_d,
+                        unsigned long long* part_d, hipStream_t st) {
     const int64_t work = (s->n1 + CNT_SEG - 1) / CNT_SEG * s->n2;  // wave-iterations
-    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((work + 15) / 16, 2048));
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((work + 15) / 16, CNT_BLOCKS));
     const uint8_t* a = (const uint8_t*)s->buf[s->cur];
     const uint8_t* b = o ? (const uint8_t*)o->buf[o->cur] : nullptr;
     const int K = rule_values(s);
-    if (K <= 2) hipLaunchKernelGGL(k_counts<2>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
-    else if (K <= 3) hipLaunchKernelGGL(k_counts<3>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
-    else if (K <= 4) hipLaunchKernelGGL(k_counts<4>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
-    else if (K <= 8) hipLaunchKernelGGL(k_counts<8>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
-    else hipLaunchKernelGGL(k_counts<16>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, out_d);
+    if (K <= 2) launch_counts<2>(blocks, a, b, s, part_d, out_d, st);
+    else if (K <= 3) launch_counts<3>(blocks, a, b, s, part_d, out_d, st);
+    else if (K <= 4) launch_counts<4>(blocks, a, b, s, part_d, out_d, st);
+    else if (K <= 8) launch_counts<8>(blocks, a, b, s, part_d, out_d, st);
+    else launch_counts<16>(blocks, a, b, s, part_d, out_d, st);
     HIPCHK(hipGetLastError());
     return CAH_OK;
 }
@@ -764,8 +776,8 @@ extern "C" int cah_state_counts(cah_state* s, const cah_state* other, int64_t* c
     HIPCHK(hipSetDevice(s->device));
     if (other && other->stream != s->stream) HIPCHK(hipStreamSynchronize(other->stream));
     if (!s->counts_d)
-        if (int rc = dev_alloc(s, (void**)&s->counts_d, CNT_SLOTS * sizeof(unsigned long long))) return rc;
-    if (int rc = queue_counts(s, other, s->counts_d, s->stream)) return rc;
+        if (int rc = dev_alloc(s, (void**)&s->counts_d, counts_words(1) * sizeof(unsigned long long))) return rc;
+    if (int rc = queue_counts(s, other, s->counts_d, s->counts_d + CNT_SLOTS, s->stream)) return rc;
     unsigned long long h[CNT_SLOTS];
     HIPCHK(hipMemcpyAsync(h, s->counts_d, sizeof h, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1262,10 +1274,12 @@ extern "C" int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* c
     HIPCHK(hipSetDevice(b->device));
     if (other && other->stream != b->stream) HIPCHK(hipStreamSynchronize(other->stream));
     const size_t n = b->members.size();
-    if (!b->counts_d) HIPCHK(hipMalloc((void**)&b->counts_d, n * CNT_SLOTS * sizeof(unsigned long long)));
+    if (!b->counts_d) HIPCHK(hipMalloc((void**)&b->counts_d, counts_words(n) * sizeof(unsigned long long)));
     // every member's counts on the batch stream, then one copy back
     for (size_t i = 0; i < n; ++i)
-        if (int rc = queue_counts(b->members[i], other, b->counts_d + i * CNT_SLOTS, b->stream)) return rc;
+        if (int rc = queue_counts(b->members[i], other, b->counts_d + i * CNT_SLOTS, b->counts_d + n * CNT_SLOTS,
+                                  b->stream))
+            return rc;
     std::vector<unsigned long long> h(n * CNT_SLOTS);
     HIPCHK(hipMemcpyAsync(h.data(), b->counts_d, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           b->stream));

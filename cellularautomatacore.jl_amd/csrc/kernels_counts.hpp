@@ -11,9 +11,12 @@
 // sets the top bit of every zero byte and of nothing else (the addition cannot
 // carry between bytes, since each byte sum is at most 0xFE).  The popcount of
 // that word, masked to the row's cells, adds to the lane's counter for v.  The
-// counters stay in registers; each wave adds its sums to out[] with one atomic
-// per value.  Values outside 0..K-1 are not counted here: the host compares the
-// total with the cell count and, when cells are left over, runs k_hist256.
+// counters stay in registers; a block's four waves add theirs in LDS and the
+// block stores its sums in a slot of its own, and k_counts_reduce adds the slots
+// (one atomic per wave on a handful of shared counters serialises in L2: 0.3 ms
+// on a 16384^2 lattice).  Values outside 0..K-1 are not counted here: the host
+// compares the total with the cell count and, when cells are left over, runs
+// k_hist256.
 //
 // k_hist256: a histogram of all 256 byte values with LDS atomics, for the rare
 // lattice holding values the rule does not produce.
@@ -24,6 +27,8 @@
 
 constexpr int CNT_SEG = 1024;      // bytes of a row per wave-iteration
 constexpr int CNT_SLOTS = 258;     // per lattice: counts[256], equal, spare
+constexpr int CNT_BLOCKS = 2048;   // most blocks of one k_counts launch
+constexpr int CNT_PART = 17;       // per block: counts of 0..15, equal
 
 __device__ inline uint32_t zero_byte_tops(uint32_t x) {
     return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
@@ -36,32 +41,32 @@ __device__ inline unsigned long long wave_sum64(unsigned long long v) {
 
 // s, o: pitched rows (pitch bytes apart, n1 cells each; o may be null); the
 // pitch is a multiple of 16, so a 16-byte load that starts inside a row stays
-// inside its pitch.  out[k] += cells equal to k (k < K); out[256] += cells where
-// s and o agree.
+// inside its pitch.  part[b * CNT_PART + k] = block b's cells equal to k (k < K),
+// part[b * CNT_PART + 16] = its cells where s and o agree (0 without o).
 template <int K>
 __global__ __launch_bounds__(256) void k_counts(const uint8_t* __restrict__ s, const uint8_t* __restrict__ o,
                                                 int64_t pitch, int64_t n1, int64_t n2,
-                                                unsigned long long* __restrict__ out) {
+                                                unsigned long long* __restrict__ part) {
+    static_assert(K <= 16, "counted in registers: values 0..15 at most");
+    __shared__ unsigned long long sums[4][K + 1];
     const int lane = threadIdx.x & 63;
     const int64_t segs = (n1 + CNT_SEG - 1) / CNT_SEG;
-    const int64_t total = segs * n2;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     uint32_t c[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) c[k] = 0;
     uint32_t eq = 0;
-    for (int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < total; w += nw) {
-        const int64_t y = w / segs;
-        const int64_t x = (w - y * segs) * CNT_SEG + lane * 16;
-        if (x >= n1) continue;
-        const uint4 v = *reinterpret_cast<const uint4*>(s + y * pitch + x);
+    // the counts of one lane's 16 bytes at column x (x < n1) of a row
+    auto count16 = [&](uint4 v, uint4 u, int64_t x) {
         const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-        uint32_t m[4];
-        const int64_t nv = n1 - x;
+        uint32_t m[4] = {0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u};
+        const int32_t nv = (int32_t)(n1 - x < 16 ? n1 - x : 16);
+        if (nv < 16) {  // the lane that reaches the row's end
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t r = nv - 4 * j;
-            m[j] = r >= 4 ? 0x80808080u : r <= 0 ? 0u : 0x80808080u & ((1u << (8 * (int)r)) - 1u);
+            for (int j = 0; j < 4; ++j) {
+                const int32_t r = nv - 4 * j;
+                m[j] = r >= 4 ? 0x80808080u : r <= 0 ? 0u : 0x80808080u & ((1u << (8 * r)) - 1u);
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -70,20 +75,68 @@ __global__ __launch_bounds__(256) void k_counts(const uint8_t* __restrict__ s, c
             for (int j = 0; j < 4; ++j) c[k] += __popc(zero_byte_tops(wd[j] ^ pat) & m[j]);
         }
         if (o) {
-            const uint4 u = *reinterpret_cast<const uint4*>(o + y * pitch + x);
             const uint32_t ud[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) eq += __popc(zero_byte_tops(wd[j] ^ ud[j]) & m[j]);
         }
+    };
+    // segment w = y * segs + g; the grid steps w by nw, so y and g advance by
+    // fixed amounts with one carry (no division in the loop)
+    // (a wave past the last segment starts at y >= n2 and skips the loop, but
+    // still takes part in the block's sums below)
+    const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t dy = nw / segs, dg = nw - dy * segs;
+    int64_t y = w0 / segs, g = w0 - y * segs;
+    const uint4 zero = {0u, 0u, 0u, 0u};
+    for (; y < n2;) {
+        const int64_t x = g * CNT_SEG + lane * 16;
+        const bool ok = x < n1;
+        uint4 v = zero, u = zero;
+        if (ok) {
+            v = *reinterpret_cast<const uint4*>(s + y * pitch + x);
+            if (o) u = *reinterpret_cast<const uint4*>(o + y * pitch + x);
+        }
+        if (ok) count16(v, u, x);
+        y += dy;
+        g += dg;
+        if (g >= segs) {
+            g -= segs;
+            ++y;
+        }
     }
+    const int wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const unsigned long long t = wave_sum64(c[k]);
-        if (lane == 0 && t) atomicAdd(out + k, t);
+        if (lane == 0) sums[wv][k] = t;
     }
-    if (o) {
-        const unsigned long long t = wave_sum64(eq);
-        if (lane == 0 && t) atomicAdd(out + 256, t);
+    const unsigned long long te = wave_sum64(eq);
+    if (lane == 0) sums[wv][K] = te;
+    __syncthreads();
+    if (threadIdx.x <= K) {
+        const int k = threadIdx.x;
+        const unsigned long long t = sums[0][k] + sums[1][k] + sums[2][k] + sums[3][k];
+        part[(int64_t)blockIdx.x * CNT_PART + (k == K ? 16 : k)] = t;
+    }
+}
+
+// out[0 .. CNT_SLOTS): the sums of k_counts' nblocks slots (counts of 0..K-1,
+// the agreement in out[256]); every other slot is set to 0.  One block.
+template <int K>
+__global__ __launch_bounds__(256) void k_counts_reduce(const unsigned long long* __restrict__ part, int nblocks,
+                                                       unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long w[4];
+    for (int i = threadIdx.x; i < CNT_SLOTS; i += blockDim.x) out[i] = 0;
+    __syncthreads();
+    for (int k = 0; k <= K; ++k) {
+        const int slot = k == K ? 16 : k;
+        unsigned long long t = 0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) t += part[(int64_t)b * CNT_PART + slot];
+        t = wave_sum64(t);
+        if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) out[k == K ? 256 : k] = w[0] + w[1] + w[2] + w[3];
+        __syncthreads();
     }
 }
 
