@@ -4,13 +4,15 @@ CellularAutomataCore.jl, driven through the C ABI of libcahip.so.
 The host mirror of the reference's interface: model types (cahip.model), the
 problem / solve callers (cahip.problem) and the backend (cahip.backend).
 """
-from .model import (ArgumentError, CellularAutomataCoreError, Checkerboard, Constant,
-                    DeviceError, ForestFire, Generations, Grid, IncompatibleSpaceError,
+from .measures import cohens_kappa, state_counts
+from .model import (AnisotropicKernel, ArgumentError, CellularAutomataCoreError, Checkerboard,
+                    Constant, DeviceError, EmpiricalKernel, ExponentialKernel, FieldEpidemic,
+                    ForestFire, GaussianKernel, Generations, Grid, IncompatibleSpaceError,
                     InvalidStateError, Lattice, Lenia, LeniaKernel, Life, Margolus, Moore,
-                    Offsets, OutOfMemoryError, Periodic, RandomSequential, Reflecting,
-                    RuleParseError, StateShapeError, Stochastic, Synchronous, Torus,
-                    UnsupportedCombinationError, VonNeumann, kernel_weights, map_axis,
-                    neighbourhood_offsets)
+                    Offsets, OutOfMemoryError, Periodic, PowerLawKernel, RandomSequential,
+                    Reflecting, RuleParseError, StateShapeError, Stochastic, Synchronous, Torus,
+                    UniformKernel, UnsupportedCombinationError, VonNeumann, kernel_weights,
+                    map_axis, neighbourhood_offsets)
 from .problem import (CAProblem, CASolution, EnsembleProblem, final_state, seed_trajectory,
                       solve, solve_ensemble)
 

@@ -23,7 +23,8 @@ CAH_ERR_BUSY = -8
 DTYPE_I8, DTYPE_F32 = 1, 2
 BOUNDARY_PERIODIC, BOUNDARY_REFLECTING, BOUNDARY_CONSTANT = 0, 1, 2
 SCHEME_SYNCHRONOUS, SCHEME_STOCHASTIC, SCHEME_CHECKERBOARD = 0, 1, 2
-RULE_LIFE, RULE_GENERATIONS, RULE_FOREST_FIRE, RULE_LENIA = 1, 2, 3, 4
+RULE_LIFE, RULE_GENERATIONS, RULE_FOREST_FIRE, RULE_LENIA, RULE_FIELD_EPIDEMIC = 1, 2, 3, 4, 5
+EPI_BERNOULLI, EPI_STATED = 0, 1
 FIELD_DIRECT_SEQ, FIELD_DIRECT_F32, FIELD_MFMA_F16, FIELD_MFMA_BF16, FIELD_FFT_F32, \
     FIELD_MFMA_F16X2 = 0, 1, 2, 3, 4, 5
 
@@ -54,6 +55,10 @@ class CahDesc(ctypes.Structure):
         ("sigma", ctypes.c_double),
         ("dt", ctypes.c_double),
         ("mask", ctypes.POINTER(ctypes.c_uint8)),
+        ("epi_form", ctypes.c_int32),
+        ("epi_recover", ctypes.c_int32),
+        ("covariate", ctypes.POINTER(ctypes.c_double)),
+        ("covariate_slices", ctypes.c_int64),
     ]
 
 

@@ -36,20 +36,32 @@ FIELD_METHODS = {"seq": L.FIELD_DIRECT_SEQ, "f32": L.FIELD_DIRECT_F32,
                  "mfma_f16x2": L.FIELD_MFMA_F16X2, "fft_f32": L.FIELD_FFT_F32}
 
 
-def _p_value(p, name):
+def _p_value(p, name, what="the forest fire", default=None):
     if p is None:
-        raise M.ArgumentError(f"the forest fire reads p.{name}, and the problem has no parameters")
-    return p[name] if isinstance(p, dict) else getattr(p, name)
+        if default is not None:
+            return default
+        raise M.ArgumentError(f"{what} reads p.{name}, and the problem has no parameters")
+    if isinstance(p, dict):
+        if name not in p and default is not None:
+            return default
+        return p[name]
+    return getattr(p, name, default) if default is not None else getattr(p, name)
 
 
 def make_params(prob, p):
-    """cah_params from the integrator's p (only the forest fire reads any)."""
+    """cah_params from the integrator's p: the forest fire's (p, f), the field
+    epidemic's (beta, gamma)."""
     cp = L.CahParams()
     if isinstance(prob.rule, M.ForestFire):
         pv, fv = _p_value(p, "p"), _p_value(p, "f")
-        cp.ff_p, cp.ff_f = float(pv), float(fv)
-        cp.ff_p_is_f32 = int(isinstance(pv, np.float32))
-        cp.ff_f_is_f32 = int(isinstance(fv, np.float32))
+    elif isinstance(prob.rule, M.FieldEpidemic):
+        pv = _p_value(p, "beta", "the field epidemic")
+        fv = _p_value(p, "gamma", "the field epidemic") if prob.rule.recover else 0.0
+    else:
+        return cp
+    cp.ff_p, cp.ff_f = float(pv), float(fv)
+    cp.ff_p_is_f32 = int(isinstance(pv, np.float32))
+    cp.ff_f_is_f32 = int(isinstance(fv, np.float32))
     return cp
 
 
@@ -93,6 +105,25 @@ def make_desc(prob, scheme, field_method=L.FIELD_DIRECT_F32):
         d.mu, d.sigma, d.dt = rule.mu, rule.sigma, rule.dt
         d.lenia_precision = rule.precision
         d.field_method = L.FIELD_FFT_F32 if rule.method == "fft" else field_method
+    if isinstance(rule, M.FieldEpidemic):
+        taps, w = prob.taps
+        to = np.array(taps, dtype=np.int32).ravel()
+        tw = np.ascontiguousarray(w, dtype=np.float64)
+        keep += [to, tw]
+        d.n_taps = len(taps)
+        d.tap_offsets = to.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        d.tap_weights = tw.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        d.lenia_precision = rule.precision
+        d.epi_form = L.EPI_STATED if rule.form == "stated" else L.EPI_BERNOULLI
+        d.epi_recover = int(rule.recover)
+        if rule.covariate is not None:
+            cv = prob.covariates[rule.covariate]
+            n = int(np.prod(sp.dims))
+            slices = 1 if cv.ndim == sp.ndims else cv.shape[-1]
+            flat = np.ascontiguousarray(cv.reshape((n, slices), order="F").T).ravel()  # slice-major
+            keep.append(flat)
+            d.covariate = flat.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            d.covariate_slices = slices
     if prob.mask is not None:
         m = np.asfortranarray(prob.mask).ravel(order="F").astype(np.uint8)
         keep.append(m)

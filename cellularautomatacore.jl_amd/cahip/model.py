@@ -369,7 +369,65 @@ class ForestFire:
         return "ForestFire()"
 
 
-# ------------------------------------------------------------------ Lenia (src/lenia.jl)
+# ------------------------------------------------------------------ dispersal kernels (src/kernels.jl)
+
+
+def _positive_radius(radius):
+    if radius < 1:
+        raise ArgumentError(f"radius must be at least 1, got {radius}")
+    return int(radius)
+
+
+class GaussianKernel:
+    """GaussianKernel(sigma; radius=ceil(3 sigma)): weight exp(-d^2 / 2 sigma^2)
+    (src/kernels.jl:15-30)."""
+
+    def __init__(self, sigma, radius=None):
+        if not sigma > 0:
+            raise ArgumentError(f"σ must be positive, got {sigma}")
+        self.sigma = float(sigma)
+        self.radius = _positive_radius(math.ceil(3 * self.sigma) if radius is None else radius)
+
+    def value(self, d, o):
+        # d^2 and σ^2 are x*x in Julia (literal powers)
+        return math.exp(-(d * d) / (2 * (self.sigma * self.sigma)))
+
+
+class ExponentialKernel:
+    """ExponentialKernel(scale; radius=ceil(5 scale)): weight exp(-d / scale)
+    (src/kernels.jl:32-47)."""
+
+    def __init__(self, scale, radius=None):
+        if not scale > 0:
+            raise ArgumentError(f"scale must be positive, got {scale}")
+        self.scale = float(scale)
+        self.radius = _positive_radius(math.ceil(5 * self.scale) if radius is None else radius)
+
+    def value(self, d, o):
+        return math.exp(-d / self.scale)
+
+
+class PowerLawKernel:
+    """PowerLawKernel(alpha; radius=10): weight (1 + d)^(-alpha) (src/kernels.jl:49-64)."""
+
+    def __init__(self, alpha, radius=10):
+        if not alpha > 0:
+            raise ArgumentError(f"α must be positive, got {alpha}")
+        self.alpha = float(alpha)
+        self.radius = _positive_radius(radius)
+
+    def value(self, d, o):
+        return (1 + d) ** (-self.alpha)
+
+
+class UniformKernel:
+    """UniformKernel(radius): equal weight within distance radius (src/kernels.jl:66-79)."""
+
+    def __init__(self, radius):
+        self.radius = _positive_radius(radius)
+
+    def value(self, d, o):
+        return 1.0
 
 
 class LeniaKernel:
@@ -383,7 +441,7 @@ class LeniaKernel:
         self.radius = int(R)
         self.peaks = tuple(float(p) for p in peaks)
 
-    def value(self, d):
+    def value(self, d, o=None):
         r = d / self.radius
         if r <= 0 or r >= 1:
             return 0.0
@@ -394,25 +452,131 @@ class LeniaKernel:
         return self.peaks[i] * math.exp(4 - 1 / (f * (1 - f))) if 0 < f < 1 else 0.0
 
 
+class AnisotropicKernel:
+    """AnisotropicKernel(base, direction, strength): the base weight times
+    exp(strength * cos(theta - direction)), theta the angle of the offset with
+    east = +column (second axis), north = -row (src/kernels.jl:106-131)."""
+
+    def __init__(self, base, direction, strength):
+        if not strength >= 0:
+            raise ArgumentError(f"strength must be nonnegative, got {strength}")
+        self.base = base
+        self.direction = float(direction)
+        self.strength = float(strength)
+        self.radius = base.radius
+
+    def value(self, d, o):
+        w = self.base.value(d, o)
+        if d == 0:
+            return w
+        theta = math.atan2(-o[0], o[1])
+        return w * math.exp(self.strength * math.cos(theta - self.direction))
+
+
+class EmpiricalKernel:
+    """EmpiricalKernel(weights): a matrix of weights with odd sides centred on the
+    cell (src/kernels.jl:133-150)."""
+
+    def __init__(self, weights):
+        w = np.array(weights, dtype=np.float64)
+        if w.ndim != 2 or not all(n % 2 == 1 for n in w.shape):
+            raise ArgumentError(f"kernel matrix sides must be odd, got {w.shape}")
+        if not (np.isfinite(w).all() and (w >= 0).all()):
+            raise ArgumentError("kernel weights must be finite and nonnegative")
+        if not w.sum() > 0:
+            raise ArgumentError("kernel weights must not all be zero")
+        self.weights = w
+        self.radius = max(n // 2 for n in w.shape)
+
+
 def kernel_weights(kernel, space, include_centre=True):
-    """kernel_weights (src/kernels.jl:174-189): offsets and normalised FP64 weights."""
-    r = kernel.radius
+    """kernel_weights (src/kernels.jl:174-212): the kernel's offsets, in
+    column-major order, and their FP64 weights normalised to sum to one."""
+    if not isinstance(space, Lattice):
+        raise IncompatibleSpaceError(f"{type(kernel).__name__}: dispersal kernels need a lattice")
     offs, ws = [], []
-    for o2 in range(-r, r + 1):
-        for o1 in range(-r, r + 1):
-            if o1 == 0 and o2 == 0 and not include_centre:
+    if isinstance(kernel, EmpiricalKernel):
+        if space.ndims != 2:
+            raise IncompatibleSpaceError("EmpiricalKernel: an empirical kernel matrix is two-dimensional")
+        c1, c2 = (n // 2 for n in kernel.weights.shape)
+        for j in range(kernel.weights.shape[1]):
+            for i in range(kernel.weights.shape[0]):
+                o = (i - c1, j - c2)
+                if o == (0, 0) and not include_centre:
+                    continue
+                w = float(kernel.weights[i, j])
+                if w > 0:
+                    offs.append(o)
+                    ws.append(w)
+        if not ws:
+            raise ArgumentError("the empirical kernel has no positive weight")
+    else:
+        r = kernel.radius
+        if isinstance(kernel, AnisotropicKernel) and space.ndims != 2:
+            raise IncompatibleSpaceError("AnisotropicKernel: directions need a two-dimensional lattice")
+        grid = ([(o1,) for o1 in range(-r, r + 1)] if space.ndims == 1 else
+                [(o1, o2) for o2 in range(-r, r + 1) for o1 in range(-r, r + 1)])
+        for o in grid:
+            if all(x == 0 for x in o) and not include_centre:
                 continue
-            d = math.sqrt(o1 * o1 + o2 * o2)
+            d = math.sqrt(sum(x * x for x in o))
             if not d <= r + 1e-9:
                 continue
-            w = float(kernel.value(d))
+            w = float(kernel.value(d, o))
             if w > 0:
-                offs.append((o1, o2))
+                offs.append(o)
                 ws.append(w)
-    if not ws:
-        raise ArgumentError(f"kernel has no positive weight on {space}")
+        if not ws:
+            raise ArgumentError(f"kernel {type(kernel).__name__} has no positive weight on {space}")
     w = np.asarray(ws, dtype=np.float64)
     return offs, w / w.sum()
+
+
+# ------------------------------------------------------------------ the field epidemic
+
+
+class FieldEpidemic:
+    """FieldEpidemic(kernel; include_centre=True, precision=64, form="bernoulli",
+    recover=False, covariate=None): an SI (or SIR) epidemic on Int8 states S=0,
+    I=1, R=2 driven by a KernelField whose source is the infected cells,
+    KernelField(kernel, ==(1); include_centre, precision) (src/environment.jl:5-36).
+
+    A susceptible cell is infected with probability q = 1 - exp(-beta * c * field),
+    with c the covariate named `covariate` (1 without one) and beta, gamma the
+    problem's parameters p.beta, p.gamma.  form="bernoulli" is the LocalRule
+    `bernoulli(ctx, q) ? 1 : 0` (test/backend_conformance.jl:63-68, :82-85);
+    form="stated" is the ProbabilisticRule with outcomes (0 => 1 - q, 1 => q)
+    (:101-104).  With recover=True an infected cell is removed with probability
+    gamma (draw k = 2, or the stated outcomes (1 => 1 - gamma, 2 => gamma)).  The
+    arithmetic is Float32 when beta (numpy.float32), the field (precision=32) are
+    and there is no covariate, else Float64."""
+    code = 5
+
+    def __init__(self, kernel, include_centre=True, precision=64, form="bernoulli",
+                 recover=False, covariate=None):
+        if precision not in (32, 64):
+            raise ArgumentError("precision must be 32 or 64")
+        if form not in ("bernoulli", "stated"):
+            raise ArgumentError(f"form must be 'bernoulli' or 'stated', got {form!r}")
+        self.kernel = kernel
+        self.include_centre = bool(include_centre)
+        self.precision = precision
+        self.form = form
+        self.recover = bool(recover)
+        self.covariate = covariate
+        self.neighbourhood = None
+
+    def validate_state(self, u0):
+        allowed = (0, 1, 2) if self.recover else (0, 1)
+        if not np.isin(u0, allowed).all():
+            raise InvalidStateError(f"allowed states are {list(allowed)}")
+
+    def __repr__(self):
+        return (f"FieldEpidemic({type(self.kernel).__name__}, form={self.form!r}, "
+                f"recover={self.recover})")
+
+
+# ------------------------------------------------------------------ Lenia (src/lenia.jl)
 
 
 class Lenia:

@@ -21,9 +21,11 @@ def _tspan(ts):
 
 
 class CAProblem:
-    """CAProblem(rule, u0, tspan, p=None; space, seed, mask) (src/problem.jl:67-92)."""
+    """CAProblem(rule, u0, tspan, p=None; space, seed, mask, covariates)
+    (src/problem.jl:67-92).  covariates: {name: array} of arrays the size of the
+    space, or with one more axis of time slices (src/environment.jl:78-113)."""
 
-    def __init__(self, rule, u0, tspan, p=None, space=None, seed=0, mask=None):
+    def __init__(self, rule, u0, tspan, p=None, space=None, seed=0, mask=None, covariates=None):
         u0 = np.asarray(u0)
         if u0.ndim == 0:
             raise M.ArgumentError("u0 must be an array")
@@ -67,11 +69,23 @@ class CAProblem:
             if rule.method == "fft" and not all(isinstance(b, M.Periodic) for b in space.boundaries):
                 raise M.ArgumentError("field :potential uses method=:fft, which needs periodic boundaries on every axis")
             self.taps = ([(-a, -b) for a, b in offs], w)
+        self.covariates = {}
+        for name, a in (covariates or {}).items():
+            a = np.asarray(a, dtype=np.float64)
+            N = space.ndims
+            if a.ndim not in (N, N + 1) or tuple(a.shape[:N]) != space.dims:
+                raise M.StateShapeError(f"covariate :{name}: expected size {space.dims}, got {a.shape}")
+            self.covariates[name] = np.asfortranarray(a)
+        if isinstance(rule, M.FieldEpidemic):
+            offs, w = M.kernel_weights(rule.kernel, space, include_centre=rule.include_centre)
+            self.taps = ([tuple(-x for x in (o + (0,))[:2]) for o in offs], w)
+            if rule.covariate is not None and rule.covariate not in self.covariates:
+                raise M.ArgumentError(f"the rule reads covariate :{rule.covariate}, which the problem lacks")
 
     def remake(self, **kw):
         """remake(prob; ...) (src/problem.jl:103-113)."""
         args = dict(rule=self.rule, u0=self.u0, tspan=self.tspan, p=self.p, space=self.space,
-                    seed=self.seed, mask=self.mask)
+                    seed=self.seed, mask=self.mask, covariates=self.covariates)
         args.update(kw)
         return CAProblem(**args)
 

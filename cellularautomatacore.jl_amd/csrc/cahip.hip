@@ -26,6 +26,7 @@
 #include "device.hpp"
 #include "host_copy.hpp"
 #include "kernels_counts.hpp"
+#include "kernels_epidemic.hpp"
 #include "kernels_generic.hpp"
 #include "launch.hpp"
 #include "kernels_lenia.hpp"
@@ -175,9 +176,34 @@ extern "C" int cah_check_problem(const cah_desc* d) {
                 return fail(CAH_ERR_UNSUPPORTED, "the FFT field runs on two-dimensional lattices");
         }
         break;
+    case CAH_RULE_FIELD_EPIDEMIC:
+        if (d->dtype != CAH_DTYPE_I8) return fail(CAH_ERR_ARGUMENT, "the field epidemic runs on Int8 states");
+        if (d->n_taps < 1 || !d->tap_offsets || !d->tap_weights)
+            return fail(CAH_ERR_ARGUMENT, "the field epidemic needs its kernel field's taps");
+        if (max_abs(d->tap_offsets, d->n_taps) > GEN_MAX_HALO || max_abs(d->offsets, d->n_offsets) > GEN_MAX_HALO)
+            return fail(CAH_ERR_ARGUMENT, "kernel fields wider than %d cells are not supported", GEN_MAX_HALO);
+        if (d->n_offsets < 0 || (d->n_offsets > 0 && !d->offsets))
+            return fail(CAH_ERR_ARGUMENT, "bad neighbourhood");
+        if (d->lenia_precision != 32 && d->lenia_precision != 64)
+            return fail(CAH_ERR_ARGUMENT, "the field's precision must be 32 or 64");
+        if (d->epi_form != CAH_EPI_BERNOULLI && d->epi_form != CAH_EPI_STATED)
+            return fail(CAH_ERR_ARGUMENT, "unknown epidemic form %d", d->epi_form);
+        if (d->epi_recover != 0 && d->epi_recover != 1)
+            return fail(CAH_ERR_ARGUMENT, "epi_recover must be 0 or 1");
+        if (d->covariate && d->covariate_slices < 1)
+            return fail(CAH_ERR_ARGUMENT, "a covariate needs at least one slice");
+        for (int a = 0; a < d->ndims; ++a)
+            if (d->boundary[a] == CAH_BOUNDARY_CONSTANT &&
+                (d->boundary_value[a] != std::floor(d->boundary_value[a]) ||
+                 d->boundary_value[a] < -128 || d->boundary_value[a] > 127))
+                return fail(CAH_ERR_INCOMPATIBLE_SPACE,
+                            "the boundary Constant(%g): a state of Int8 cannot hold it",
+                            d->boundary_value[a]);
+        break;
     default:
         return fail(CAH_ERR_ARGUMENT,
-                    "the HIP backend runs Life, Generations, the forest fire and Lenia (rule %d)",
+                    "the HIP backend runs Life, Generations, the forest fire, Lenia and the field "
+                    "epidemic (rule %d)",
                     d->rule);
     }
     return CAH_OK;
@@ -207,6 +233,9 @@ struct cah_state {
     void* field_d = nullptr;   // field buffer for the sequential path
     void* frag_d = nullptr;    // MFMA A fragments
     unsigned long long* counts_d = nullptr;  // CNT_SLOTS counters of cah_state_counts
+    double* cov_d = nullptr;   // field epidemic: covariate slices
+    std::vector<double> cov;   //   ... host copy while the state is created
+    int32_t epi_h = 0;         //   ... its field's halo
     int lenia_mode = 0;        // LM_F16 / LM_BF16 / LM_F16X2
     LeniaFFT* fft = nullptr;
     int32_t halo = 1;
@@ -384,7 +413,7 @@ static void free_state(cah_state* s) {
     if (s->stream && !s->arena) hipStreamSynchronize(s->stream);
     for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
                     (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d, s->frag_d,
-                    (void*)s->counts_d})
+                    (void*)s->counts_d, (void*)s->cov_d})
         dev_free(s, p);
     lenia_fft_destroy(s->fft);
     if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
@@ -443,14 +472,17 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
     s->es = d->dtype == CAH_DTYPE_F32 ? 4 : 1;
     s->pitch = d->dtype == CAH_DTYPE_F32 ? (s->n1 + 15) / 16 * 16 : (s->n1 + 63) / 64 * 64;
     s->offsets.assign(d->offsets, d->offsets + 2 * (size_t)d->n_offsets);
-    if (d->rule == CAH_RULE_LENIA) {
+    if (d->rule == CAH_RULE_LENIA || d->rule == CAH_RULE_FIELD_EPIDEMIC) {
         s->taps.assign(d->tap_offsets, d->tap_offsets + 2 * (size_t)d->n_taps);
         s->weights.assign(d->tap_weights, d->tap_weights + d->n_taps);
     }
+    if (d->rule == CAH_RULE_FIELD_EPIDEMIC && d->covariate)
+        s->cov.assign(d->covariate, d->covariate + (size_t)(s->n1 * s->n2) * (size_t)d->covariate_slices);
     s->d.offsets = nullptr;
     s->d.tap_offsets = nullptr;
     s->d.tap_weights = nullptr;
     s->d.mask = nullptr;
+    s->d.covariate = nullptr;
     const size_t bytes = (size_t)s->pitch * (size_t)s->n2 * s->es;
     for (int b = 0; b < 2; ++b) {
         if (int rc = dev_alloc(s, &s->buf[b], bytes)) return rc;
@@ -511,6 +543,25 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
     }
 
     s->path = P_GENERIC;
+    if (d->rule == CAH_RULE_FIELD_EPIDEMIC) {
+        // the field's taps and weights in its precision, its buffer, the covariate
+        s->epi_h = std::max(1, max_abs(s->taps.data(), d->n_taps));
+        if (int rc = upload(s, &s->taps_d, s->taps.data(), s->taps.size())) return rc;
+        if (d->lenia_precision == 32) {
+            std::vector<float> w32(s->weights.begin(), s->weights.end());
+            if (int rc = upload(s, &s->w32_d, w32.data(), w32.size())) return rc;
+        } else if (int rc = upload(s, &s->w64_d, s->weights.data(), s->weights.size())) {
+            return rc;
+        }
+        if (int rc = dev_alloc(s, &s->field_d, (size_t)(s->n1 * s->n2) * (d->lenia_precision == 32 ? 4 : 8)))
+            return rc;
+        if (!s->cov.empty()) {
+            if (int rc = upload(s, &s->cov_d, s->cov.data(), s->cov.size())) return rc;
+            std::vector<double>().swap(s->cov);  // copied (or staged in pinned memory)
+        }
+        s->bin_clean = true;
+        return CAH_OK;
+    }
     const bool plain2d = d->ndims == 2 && d->scheme == CAH_SCHEME_SYNCHRONOUS && !d->mask;
     if (plain2d && d->rule == CAH_RULE_LIFE) {
         const int R = moore_radius(s->offsets);
@@ -952,10 +1003,33 @@ static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p
         a.ffp_f32 = p->ff_p_is_f32;
         a.fff_f32 = p->ff_f_is_f32;
     }
+    if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC) {
+        a.field = s->field_d;
+        a.field32 = s->d.lenia_precision == 32;
+        a.epi_form = s->d.epi_form;
+        a.epi_recover = s->d.epi_recover;
+        a.cov = s->cov_d;
+        a.cov_slices = (int32_t)s->d.covariate_slices;
+    }
     dim3 grid((unsigned)((s->n1 + GEN_TX - 1) / GEN_TX), (unsigned)((s->n2 + GEN_TY - 1) / GEN_TY));
     const size_t lds = (size_t)(GEN_TX + 2 * a.h) * (GEN_TY + 2 * a.h);
     k_generic_i8<<<grid, 256, lds, s->stream>>>(a);
     return CAH_OK;
+}
+
+// The field epidemic's kernel field of the step-start state, read by every gate.
+static void launch_epi_field(cah_state* s) {
+    const Lat L = lat_of(s);
+    dim3 grid((unsigned)((s->n1 + EPI_TX - 1) / EPI_TX), (unsigned)((s->n2 + EPI_TY - 1) / EPI_TY));
+    const size_t lds = (size_t)(EPI_TX + 2 * s->epi_h) * (EPI_TY + 2 * s->epi_h);
+    const int8_t* cur = (const int8_t*)s->buf[s->cur];
+    const int8_t cv1 = (int8_t)s->d.boundary_value[0], cv2 = (int8_t)s->d.boundary_value[1];
+    if (s->d.lenia_precision == 32)
+        k_epi_field<float><<<grid, 256, lds, s->stream>>>(cur, L, cv1, cv2, s->taps_d, s->w32_d, s->d.n_taps,
+                                                          s->epi_h, (float*)s->field_d);
+    else
+        k_epi_field<double><<<grid, 256, lds, s->stream>>>(cur, L, cv1, cv2, s->taps_d, s->w64_d, s->d.n_taps,
+                                                           s->epi_h, (double*)s->field_d);
 }
 
 static LeniaArgs lenia_args(cah_state* s, int64_t t, int gate) {
@@ -1049,10 +1123,13 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
     if (nsteps < 0 || t0 < 0) return fail(CAH_ERR_ARGUMENT, "t0 and nsteps must be >= 0");
     if (s->d.rule == CAH_RULE_FOREST_FIRE && !p)
         return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
+    if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC && !p)
+        return fail(CAH_ERR_ARGUMENT, "the field epidemic needs its parameters (beta, gamma)");
     HIPCHK(hipSetDevice(s->device));
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t t = t0 + k;
         if (s->path == P_LENIA_SEQ) launch_lenia_field_seq(s, t);
+        if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC) launch_epi_field(s);
         if (s->path == P_LENIA_FFT && lenia_fft_field(s->fft, (const float*)s->buf[s->cur], s->stream))
             return fail(CAH_ERR_DEVICE, "hipFFT failed in the Lenia field");
         for (int g = 0; g < ngates(s->d); ++g) {
@@ -1430,6 +1507,7 @@ struct SrvReq {
     std::vector<int32_t> offsets, taps;
     std::vector<double> weights;
     std::vector<uint8_t> mask;
+    std::vector<double> cov;             // field epidemic: covariate slices
     std::vector<char> u0;                // initial state, when it is not in pinned memory
     char* u0_pinned = nullptr;           // initial state in the server's pinned arena
     size_t u0_bytes = 0;
@@ -1497,6 +1575,11 @@ static size_t request_bytes(const cah_desc& d) {
         const bool mfma = fm == CAH_FIELD_MFMA_F16 || fm == CAH_FIELD_MFMA_BF16 || fm == CAH_FIELD_MFMA_F16X2;
         b += (size_t)(n1 * n2) * (fm == CAH_FIELD_FFT_F32 ? 16 : mfma ? 0 : 8);
         b += (size_t)1 << 20;  // taps, weights, tables, fragments
+    }
+    if (d.rule == CAH_RULE_FIELD_EPIDEMIC) {
+        b += (size_t)(n1 * n2) * (d.lenia_precision == 32 ? 4 : 8);
+        if (d.covariate) b += (size_t)(n1 * n2) * (size_t)d.covariate_slices * 8;
+        b += (size_t)d.n_taps * 16;
     }
     return b;
 }
@@ -1577,11 +1660,15 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
     r->d = *d;
     r->offsets.assign(d->offsets, d->offsets + 2 * (size_t)d->n_offsets);
     r->d.offsets = r->offsets.data();
-    if (d->rule == CAH_RULE_LENIA) {
+    if (d->rule == CAH_RULE_LENIA || d->rule == CAH_RULE_FIELD_EPIDEMIC) {
         r->taps.assign(d->tap_offsets, d->tap_offsets + 2 * (size_t)d->n_taps);
         r->weights.assign(d->tap_weights, d->tap_weights + d->n_taps);
         r->d.tap_offsets = r->taps.data();
         r->d.tap_weights = r->weights.data();
+    }
+    if (d->rule == CAH_RULE_FIELD_EPIDEMIC && d->covariate) {
+        r->cov.assign(d->covariate, d->covariate + (size_t)cells * (size_t)d->covariate_slices);
+        r->d.covariate = r->cov.data();
     }
     if (d->mask) {
         r->mask.assign(d->mask, d->mask + cells);

@@ -26,7 +26,53 @@ struct GenArgs {
     double sp;
     double ffp, fff;
     int32_t ffp_f32, fff_f32;
+    // the field epidemic: its field (dense, FP64 or FP32), covariate slices
+    const void* field;
+    int32_t field32, epi_form, epi_recover, cov_slices;
+    const double* cov;
 };
+
+// Outcome inversion over (a => 1 - q, b => q) with the draw u (_pick,
+// src/likelihood.jl:74-83).
+template <typename T>
+CAH_DEV int pick2(int a, int b, T q, T u) {
+    const T pa = T(1) - q;
+    if (pa > T(0) && u < pa) return a;
+    return q > T(0) ? b : a;
+}
+
+// The field epidemic on S=0, I=1, R=2 (include/cahip.h, CAH_RULE_FIELD_EPIDEMIC).
+CAH_DEV int epidemic_rule(const GenArgs& a, int c, int64_t lin, uint64_t cellid) {
+    if (c == 1) {
+        if (!a.epi_recover) return 1;
+        if (a.epi_form == CAH_EPI_STATED) {
+            if (a.fff_f32) return pick2<float>(1, 2, (float)a.fff, uniform32(a.seed, cellid, a.t, 1));
+            return pick2<double>(1, 2, a.fff, uniform64(a.seed, cellid, a.t, 1));
+        }
+        return bernoulli(a.seed, cellid, a.t, 2, a.fff, a.fff_f32) ? 2 : 1;
+    }
+    if (c != 0) return c;
+    if (a.ffp_f32 && a.field32 && !a.cov) {
+        // beta, field and q in Float32; the 24-bit draw
+        const float lam = (float)a.ffp * static_cast<const float*>(a.field)[lin];
+        const float q = 1.0f - expf(-lam);
+        const float u = uniform32(a.seed, cellid, a.t, 1);
+        return a.epi_form == CAH_EPI_STATED ? pick2<float>(0, 1, q, u) : (u < q ? 1 : 0);
+    }
+    const double beta = a.ffp_f32 ? (double)(float)a.ffp : a.ffp;
+    const double f = a.field32 ? (double)static_cast<const float*>(a.field)[lin]
+                               : static_cast<const double*>(a.field)[lin];
+    double lam;
+    if (a.cov) {
+        const int64_t n = (int64_t)a.L.n1 * a.L.n2;
+        lam = (beta * a.cov[(a.t % a.cov_slices) * n + lin]) * f;
+    } else {
+        lam = beta * f;
+    }
+    const double q = 1.0 - exp(-lam);
+    const double u = uniform64(a.seed, cellid, a.t, 1);
+    return a.epi_form == CAH_EPI_STATED ? pick2<double>(0, 1, q, u) : (u < q ? 1 : 0);
+}
 
 constexpr int GEN_TX = 64, GEN_TY = 16;
 
@@ -68,6 +114,8 @@ __global__ __launch_bounds__(256) void k_generic_i8(GenArgs a) {
                 if (c == 0) out = n1 < 64 ? (int)((a.birth >> n1) & 1u) : 0;
                 else if (c == 1) out = (n1 < 64 && ((a.survive >> n1) & 1u)) ? 1 : (a.C > 2 ? 2 : 0);
                 else out = c + 1 < a.C ? c + 1 : 0;
+            } else if (a.rule == CAH_RULE_FIELD_EPIDEMIC) {
+                out = epidemic_rule(a, c, lin, cellid);
             } else {
                 // the forest-fire LocalRule (README.md:56-60), Int8 encoding
                 if (c == 2) out = 0;

@@ -61,6 +61,23 @@ extern "C" {
 #define CAH_RULE_FOREST_FIRE 3  /* the forest-fire LocalRule of README.md:56-60,
                                    Int8 states empty=0, tree=1, fire=2 */
 #define CAH_RULE_LENIA       4  /* Lenia(; R, mu, sigma, dt)  src/lenia.jl:17-47   */
+#define CAH_RULE_FIELD_EPIDEMIC 5 /* an SI / SIR epidemic driven by a KernelField
+                                   whose source is the infected cells (==(1)),
+                                   src/environment.jl:5-69: the kernel-field
+                                   LocalRule / ProbabilisticRule cases of
+                                   test/backend_conformance.jl:63-68, 82-104.
+                                   Int8 states S=0, I=1, R=2.  A susceptible
+                                   cell is infected with probability
+                                   q = 1 - exp(-beta * covariate * field);
+                                   an infected one is removed with probability
+                                   gamma when epi_recover is set.            */
+
+/* How the epidemic draws its outcome (src/likelihood.jl:67-98). */
+#define CAH_EPI_BERNOULLI 0  /* LocalRule: bernoulli(ctx, q, 1) infects,
+                                bernoulli(ctx, gamma, 2) removes            */
+#define CAH_EPI_STATED    1  /* ProbabilisticRule outcomes (0 => 1-q, 1 => q),
+                                (1 => 1-gamma, 2 => gamma), one inversion
+                                draw uniform(ctx, 1) per cell              */
 
 /* How the Lenia potential (its :potential KernelField) is computed. */
 #define CAH_FIELD_DIRECT_SEQ  0  /* one sequential sum per cell in tap order, in the
@@ -105,13 +122,28 @@ typedef struct cah_desc {
     double   mu, sigma, dt;      /* Lenia growth parameters                     */
     const uint8_t* mask;         /* NULL, or prod(dims) bytes; 0 = never update
                                     (src/environment.jl:126-148)                */
+    /* CAH_RULE_FIELD_EPIDEMIC: the field's taps are tap_offsets / tap_weights
+     * (read offsets, normalised weights), its precision lenia_precision (64 or
+     * 32, KernelField(...; precision)); beta and gamma are cah_params ff_p and
+     * ff_f.  The arithmetic is Float32 when beta, the field and the covariate
+     * are (no covariate counts as Float32), else Float64, as Julia promotes. */
+    int32_t  epi_form;           /* CAH_EPI_*                                   */
+    int32_t  epi_recover;        /* 1: SIR (I -> R with gamma), 0: SI           */
+    const double* covariate;     /* NULL, or prod(dims) * covariate_slices values,
+                                    column-major, slice-major: the multiplier of
+                                    beta at time t is slice t mod slices
+                                    (covariate(ctx, name), src/environment.jl:93-113) */
+    int64_t  covariate_slices;
 } cah_desc;
 
 /* Parameters that may change between steps: the integrator's `p`
- * (src/backends.jl:121-130).  Only the forest fire reads any. */
+ * (src/backends.jl:121-130).  The forest fire reads (p, f), the field epidemic
+ * (beta, gamma) from the same two slots. */
 typedef struct cah_params {
-    double  ff_p;      /* growth probability  ctx.p.p  (draw k = 1) */
-    double  ff_f;      /* lightning probability ctx.p.f (draw k = 2) */
+    double  ff_p;      /* forest fire: growth probability ctx.p.p (draw k = 1);
+                          field epidemic: ctx.p.beta                     */
+    double  ff_f;      /* forest fire: lightning probability ctx.p.f (draw k = 2);
+                          field epidemic: ctx.p.gamma                    */
     int32_t ff_p_is_f32;  /* p::Float32: bernoulli uses the 24-bit draw (src/random.jl:129-130) */
     int32_t ff_f_is_f32;
 } cah_params;

@@ -28,6 +28,16 @@
  *                       (test/backend_conformance.jl:207-231).
  *   ref_update_cell     equal to one synchronous step over all cells
  *                       (test/test_backend_interface.jl:40-68).
+ *   field epidemic      the field: the kernel-field values of
+ *                       test/test_environment.jl:46-80 (a uniform radius-1 field is
+ *                       1/5 on the cross, 2/5 in a corner beyond Constant(1) edges,
+ *                       an eastward kernel carries a source east); the rule: beta = 0
+ *                       infects nobody, a huge beta infects every susceptible cell
+ *                       with an infected cell in reach and no other, infection is
+ *                       permanent without recovery, masked cells never change, the
+ *                       stated form equals the Bernoulli form with u -> 1 - u
+ *                       swapped in (checked against the draws), infection rates
+ *                       match 1 - exp(-beta * field) over many cells.
  */
 #include "caref.h"
 
@@ -211,6 +221,9 @@ typedef struct {
     uint64_t seed;
     int64_t cellid, t;
     const cah_params* p;
+    double f64;   /* the field epidemic's field at the cell (precision 64) */
+    float f32;    /*   ... (precision 32)                                  */
+    double cov;   /*   ... and its covariate (1 without one)               */
 } ctx_t;
 
 /* local_rule(::Life) (src/rules.jl:377-389); (mask >> n) & 1 is 0 for n >= 64. */
@@ -245,12 +258,67 @@ static int forest(const ctx_t* cx, int c, const int8_t* nb, int32_t m) {
     return bern(cx->seed, cx->cellid, cx->t, 2, cx->p->ff_f, cx->p->ff_f_is_f32) ? 2 : 1;
 }
 
+/* Outcome inversion over (a => 1 - q, b => q) with the draw u (_pick,
+ * src/likelihood.jl:74-83): a when 1 - q > 0 and u < 1 - q, otherwise b when
+ * q > 0, otherwise a (the last outcome of positive probability, or the first). */
+static int pick64(int a, int b, double q, double u) {
+    double pa = 1.0 - q;
+    if (pa > 0 && u < pa) return a;
+    return q > 0 ? b : a;
+}
+static int pick32(int a, int b, float q, float u) {
+    float pa = 1.0f - q;
+    if (pa > 0 && u < pa) return a;
+    return q > 0 ? b : a;
+}
+
+/* The kernel-field epidemic on Int8 states S=0, I=1, R=2, after the kernel-field
+ * rules of test/backend_conformance.jl:63-68 (LocalRule, Float64 with a
+ * covariate), :82-85 (LocalRule, Float32) and :101-104 (ProbabilisticRule,
+ * Float32):
+ *   S: q = 1 - exp(-(beta * cov) * field), Julia's promotion deciding the type:
+ *      Float32 only when beta and the field are and there is no covariate;
+ *      LocalRule form: infected iff bernoulli(ctx, q, 1); stated form: the
+ *      inversion of (0 => 1 - q, 1 => q) with uniform(ctx, 1);
+ *   I: with recovery, removed iff bernoulli(ctx, gamma, 2) (stated form: the
+ *      inversion of (1 => 1 - gamma, 2 => gamma) with uniform(ctx, 1)); else I;
+ *   anything else keeps its state. */
+static int epidemic(const cah_desc* d, const ctx_t* cx, int c) {
+    const cah_params* p = cx->p;
+    if (c == 1) {
+        if (!d->epi_recover) return 1;
+        if (d->epi_form == CAH_EPI_STATED) {
+            if (p->ff_f_is_f32)
+                return pick32(1, 2, (float)p->ff_f, ref_uniform32(cx->seed, cx->cellid, cx->t, 1));
+            return pick64(1, 2, p->ff_f, ref_uniform(cx->seed, cx->cellid, cx->t, 1));
+        }
+        return bern(cx->seed, cx->cellid, cx->t, 2, p->ff_f, p->ff_f_is_f32) ? 2 : 1;
+    }
+    if (c != 0) return c;
+    int f32 = p->ff_p_is_f32 && d->lenia_precision == 32 && d->covariate == NULL;
+    if (f32) {
+        float lam = (float)p->ff_p * cx->f32;
+        float q = 1.0f - expf(-lam);
+        float u = ref_uniform32(cx->seed, cx->cellid, cx->t, 1);
+        if (d->epi_form == CAH_EPI_STATED) return pick32(0, 1, q, u);
+        return u < q;
+    }
+    double beta = p->ff_p_is_f32 ? (double)(float)p->ff_p : p->ff_p;
+    double f = d->lenia_precision == 32 ? (double)cx->f32 : cx->f64;
+    double lam = d->covariate ? (beta * cx->cov) * f : beta * f;
+    double q = 1.0 - exp(-lam);
+    double u = ref_uniform(cx->seed, cx->cellid, cx->t, 1);
+    if (d->epi_form == CAH_EPI_STATED) return pick64(0, 1, q, u);
+    return u < q;
+}
+
 static int8_t rule_i8(const cah_desc* d, const ctx_t* cx, int8_t c, const int8_t* nb,
                       int32_t m) {
     switch (d->rule) {
     case CAH_RULE_LIFE: return (int8_t)life(d, c, nb, m);
     case CAH_RULE_GENERATIONS: return (int8_t)generations(d, c, nb, m);
     case CAH_RULE_FOREST_FIRE: return (int8_t)forest(cx, c, nb, m);
+    case CAH_RULE_FIELD_EPIDEMIC: return (int8_t)epidemic(d, cx, c);
     default: return c;
     }
 }
@@ -326,15 +394,61 @@ int ref_lenia_field(const cah_desc* d, const float* u, double* out) {
     return 0;
 }
 
-/* _plain_sweep! (src/backends.jl:264-278) */
+/* The epidemic's KernelField at one cell (_compute_field_reference!,
+ * src/environment.jl:151-160, source ==(1)): s += w * (x == 1) in tap order, in
+ * the field's precision; w * false is +0, which leaves s as it is. */
+static double epi_field64(const lat_t* L, const int8_t* u, int64_t x, int64_t y) {
+    const cah_desc* d = L->d;
+    double s = 0.0;
+    for (int32_t k = 0; k < d->n_taps; ++k)
+        s += d->tap_weights[k] *
+             (double)(read_i8(L, u, x + d->tap_offsets[2 * k], y + d->tap_offsets[2 * k + 1]) == 1);
+    return s;
+}
+
+static float epi_field32(const lat_t* L, const int8_t* u, int64_t x, int64_t y) {
+    const cah_desc* d = L->d;
+    float s = 0.0f;
+    for (int32_t k = 0; k < d->n_taps; ++k)
+        s += (float)d->tap_weights[k] *
+             (float)(read_i8(L, u, x + d->tap_offsets[2 * k], y + d->tap_offsets[2 * k + 1]) == 1);
+    return s;
+}
+
+/* Field values of the epidemic over the lattice, widened to double. */
+int ref_epi_field(const cah_desc* d, const int8_t* u, double* out) {
+    lat_t L = lat_of(d);
+    for (int64_t y = 0; y < L.n2; ++y)
+        for (int64_t x = 0; x < L.n1; ++x)
+            out[x + y * L.n1] = d->lenia_precision == 32 ? (double)epi_field32(&L, u, x, y)
+                                                          : epi_field64(&L, u, x, y);
+    return 0;
+}
+
+/* The epidemic's per-cell inputs: field (both precisions) and covariate at time t. */
+static void epi_inputs(const lat_t* L, const int8_t* u, int64_t x, int64_t y, int64_t t, ctx_t* cx) {
+    const cah_desc* d = L->d;
+    if (d->lenia_precision == 32) cx->f32 = epi_field32(L, u, x, y);
+    else cx->f64 = epi_field64(L, u, x, y);
+    cx->cov = 1.0;
+    if (d->covariate) {
+        int64_t n = L->n1 * L->n2, sl = t % d->covariate_slices;
+        cx->cov = d->covariate[sl * n + x + y * L->n1];
+    }
+}
+
+/* _plain_sweep! (src/backends.jl:264-278).  `fs` (field epidemic) is the state
+ * the step's field is computed from: the state at the start of the step, for
+ * every gate (src/backends.jl:314-322). */
 static void sweep_plain_i8(const lat_t* L, const int8_t* cur, int8_t* next, int64_t t,
-                           int gate, const cah_params* p) {
+                           int gate, const cah_params* p, const int8_t* fs) {
     const cah_desc* d = L->d;
     int8_t nb[1024];
     for (int64_t y = 0; y < L->n2; ++y)
         for (int64_t x = 0; x < L->n1; ++x) {
             int64_t lin = x + y * L->n1;
-            ctx_t cx = {d->seed, lin + 1, t, p};
+            ctx_t cx = {d->seed, lin + 1, t, p, 0.0, 0.0f, 1.0};
+            if (d->rule == CAH_RULE_FIELD_EPIDEMIC) epi_inputs(L, fs, x, y, t, &cx);
             int8_t c = cur[lin];
             if ((d->mask && !d->mask[lin]) || !gate_open(d, gate, d->seed, lin + 1, t, x, y)) {
                 next[lin] = c;
@@ -376,17 +490,20 @@ int ref_advance_plain(const cah_desc* d, void* u, int64_t t0, int64_t nsteps,
         f64 = (double*)malloc((size_t)n * sizeof(double));
         f32 = (float*)malloc((size_t)n * sizeof(float));
     }
+    /* the epidemic's field reads the state at the start of the step in every gate */
+    int8_t* start = d->rule == CAH_RULE_FIELD_EPIDEMIC ? (int8_t*)malloc((size_t)n) : NULL;
     for (int64_t s = 0; s < nsteps; ++s) {
         int64_t t = t0 + s;
         if (d->rule == CAH_RULE_LENIA) {
             if (d->lenia_precision == 32) field_plain32(&L, (const float*)a, f32);
             else field_plain64(&L, (const float*)a, f64);
         }
+        if (start) memcpy(start, a, (size_t)n);
         for (int g = 0; g < ngates(d); ++g) {
             if (d->rule == CAH_RULE_LENIA)
                 sweep_plain_lenia(&L, (const float*)a, (float*)b, t, g, f64, f32);
             else
-                sweep_plain_i8(&L, (const int8_t*)a, (int8_t*)b, t, g, p);
+                sweep_plain_i8(&L, (const int8_t*)a, (int8_t*)b, t, g, p, start);
             void* tmp = a;
             a = b;
             b = tmp;
@@ -399,6 +516,7 @@ int ref_advance_plain(const cah_desc* d, void* u, int64_t t0, int64_t nsteps,
     free(b);
     free(f64);
     free(f32);
+    free(start);
     return 0;
 }
 
@@ -413,7 +531,8 @@ int ref_update_cell(const cah_desc* d, const void* u, int64_t i1, int64_t i2, in
     int8_t nb[1024];
     for (int32_t k = 0; k < d->n_offsets; ++k)
         nb[k] = read_i8(&L, cur, i1 + d->offsets[2 * k], i2 + d->offsets[2 * k + 1]);
-    ctx_t cx = {d->seed, lin + 1, t, p};
+    ctx_t cx = {d->seed, lin + 1, t, p, 0.0, 0.0f, 1.0};
+    if (d->rule == CAH_RULE_FIELD_EPIDEMIC) epi_inputs(&L, cur, i1, i2, t, &cx);
     return rule_i8(d, &cx, c, nb, d->n_offsets);
 }
 
@@ -482,7 +601,7 @@ static void sweep_slab_i8(const halo_t* H, const int8_t* cur, int8_t* next, int6
                 continue;
             }
             for (int32_t k = 0; k < m; ++k) nb[k] = cur[P + H->lin_off[k]];
-            ctx_t cx = {d->seed, lin + 1, t, p};
+            ctx_t cx = {d->seed, lin + 1, t, p, 0.0, 0.0f, 1.0};
             next[P] = rule_i8(d, &cx, c, nb, m);
         }
 }
@@ -530,6 +649,7 @@ int ref_advance_halo(const cah_desc* d, void* u, int64_t t0, int64_t nsteps,
     H.P1 = H.L.n1 + 2 * H.h;
     H.P2 = H.L.n2 + 2 * H.h;
     if (d->rule == CAH_RULE_LENIA && ngates(d) > 1) return -1; /* field reuse across gates */
+    if (d->rule == CAH_RULE_FIELD_EPIDEMIC) return -1;  /* restated in the plain engine only */
     size_t es = d->dtype == CAH_DTYPE_F32 ? 4 : 1;
     H.lin_off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(d->n_offsets + 1));
     H.tap_off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(d->n_taps + 1));
@@ -630,6 +750,10 @@ int ref_desc_layout(int64_t* out) {
     out[k++] = (int64_t)offsetof(cah_desc, sigma);
     out[k++] = (int64_t)offsetof(cah_desc, dt);
     out[k++] = (int64_t)offsetof(cah_desc, mask);
+    out[k++] = (int64_t)offsetof(cah_desc, epi_form);
+    out[k++] = (int64_t)offsetof(cah_desc, epi_recover);
+    out[k++] = (int64_t)offsetof(cah_desc, covariate);
+    out[k++] = (int64_t)offsetof(cah_desc, covariate_slices);
     out[k++] = (int64_t)sizeof(cah_params);
     out[k++] = (int64_t)offsetof(cah_params, ff_p);
     out[k++] = (int64_t)offsetof(cah_params, ff_f);

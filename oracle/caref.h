@@ -51,6 +51,9 @@ int ref_advance_halo(const cah_desc* d, void* u, int64_t t0, int64_t nsteps,
 /* The Lenia potential alone, FP64 sequential sum with boundary reads
  * (_compute_field_reference!, src/environment.jl:151-160), from a Float32 state. */
 int ref_lenia_field(const cah_desc* d, const float* u, double* out);
+/* The field epidemic's KernelField (source ==(1)) over an Int8 state, in the
+ * field's precision and widened to double (src/environment.jl:151-160). */
+int ref_epi_field(const cah_desc* d, const int8_t* u, double* out);
 
 /* The build's synthetic-input generator (no reference counterpart; documented in
  * include/cahip.h at cah_state_fill_bernoulli). */

@@ -43,6 +43,7 @@ def load():
             "ref_update_cell": (ctypes.c_int, [P, P, I64, I64, I64, P]),
             "ref_advance_halo": (ctypes.c_int, [P, P, I64, I64, P, I32]),
             "ref_lenia_field": (ctypes.c_int, [P, P, P]),
+            "ref_epi_field": (ctypes.c_int, [P, P, P]),
             "ref_fill_bernoulli": (None, [U64, I64, D, P]),
             "ref_fill_uniform_box": (None, [U64, I64, I64, I64, I64, I64, I64, P]),
             "ref_desc_layout": (ctypes.c_int, [P]),
@@ -125,6 +126,14 @@ def lenia_field(desc, u):
     u = np.asfortranarray(u, dtype=np.float32)
     out = np.empty(u.shape, dtype=np.float64, order="F")
     load().ref_lenia_field(ctypes.byref(desc), u.ctypes.data, out.ctypes.data)
+    return out
+
+
+def epi_field(desc, u):
+    """The field epidemic's kernel field over the Int8 state u."""
+    u = np.asfortranarray(u, dtype=np.int8)
+    out = np.empty(u.shape, dtype=np.float64, order="F")
+    load().ref_epi_field(ctypes.byref(desc), u.ctypes.data, out.ctypes.data)
     return out
 
 
