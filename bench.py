@@ -13,6 +13,7 @@ the reference's threaded ArrayBackend, timed on a few steps of each lattice.
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -282,6 +283,54 @@ def bench_batched(be, ev, steps=300, n=256):
             "identical_trajectories": bool(same)}
 
 
+def bench_field_epidemic(be, ev, steps=100):
+    """The field epidemic on a 2048^2 torus: SI, AnisotropicKernel(GaussianKernel(2),
+    pi/4, 1) (113 taps), 0.1% of cells infected from the build's generator (seed 3),
+    beta 0.05; ms/step (field + rule) for Float64 and Float32 fields, HIP events on
+    the state's stream."""
+    from cahip import model as M
+    from cahip.backend import check, make_params
+    from cahip.problem import CAProblem
+    n = 2048
+    out = {"workload": f"field epidemic SI, Torus {n}^2, AnisotropicKernel(GaussianKernel(2), pi/4, 1), "
+                       f"0.1% infected (generator seed 3), beta 0.05, {steps} steps"}
+    for precision, beta in ((64, 0.05), (32, np.float32(0.05))):
+        rule = M.FieldEpidemic(M.AnisotropicKernel(M.GaussianKernel(2.0), math.pi / 4, 1.0),
+                               precision=precision)
+        prob = CAProblem(rule, np.zeros((n, n), np.int8), steps, {"beta": beta}, seed=5)
+        cp = make_params(prob, prob.p)
+        st = be.allocate_state(prob, M.Synchronous())
+        check(be.lib.cah_state_fill_bernoulli(st.handle, 3, 0.001))
+        check(be.lib.cah_advance(st.handle, 0, 3, ctypes.byref(cp)))
+        ms = timed(ev, be.stream(st), lambda: check(
+            be.lib.cah_advance(st.handle, 3, steps, ctypes.byref(cp))))
+        infected = be.state_counts(st).get(1, 0)
+        be.free_state(st)
+        out[f"f{precision}"] = {"ms_per_step": round(ms / steps, 4), "taps": len(prob.taps[0]),
+                                "gcups": round(n * n * steps / (ms * 1e-3) / 1e9, 2),
+                                "infected_after": infected}
+    return out
+
+
+def bench_counts(be, ev, reps=20):
+    """cah_state_counts on the 16384^2 forest fire after 20 steps: ms per call
+    (kernels + 2 KiB back), and the state's bytes read per second."""
+    from cahip import model as M
+    from cahip.backend import check, make_params
+    from cahip.problem import CAProblem
+    n = 16384
+    prob = CAProblem(M.ForestFire(), np.zeros((n, n), np.int8), 20, {"p": 0.03, "f": 1e-4},
+                     space=M.Grid(n, n, boundary=M.Reflecting()), seed=7)
+    st = be.allocate_state(prob, M.Synchronous())
+    check(be.lib.cah_advance(st.handle, 0, 20, ctypes.byref(make_params(prob, prob.p))))
+    for _ in range(3):
+        counts = be.state_counts(st)
+    ms = timed(ev, be.stream(st), lambda: [be.state_counts(st) for _ in range(reps)]) / reps
+    be.free_state(st)
+    return {"workload": f"state_counts of a {n}^2 Int8 forest fire after 20 steps", "ms_per_call": round(ms, 4),
+            "gbs_state_read": round(n * n / (ms * 1e-3) / 1e9, 1), "counts": counts}
+
+
 def bench_lenia(be, ev, steps=500):
     """Lenia R=13 (mu 0.15, sigma 0.015, dt 0.1), Float32 Torus 2048^2, uniform[0,1]
     in the central 256^2 patch: ms/step per field method and the per-step max-abs
@@ -517,6 +566,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (Life 512^2, batched, Lenia, serving)")
+    ap.add_argument("--configs", default=None,
+                    help="comma-separated subset of the other configurations to run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the plain copy that scales the HBM counters")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
@@ -625,11 +676,18 @@ def main():
     be.free_state(s_ltl)
     be.free_state(s_for)
     if not args.no_configs:
-        line["configs"] = {"life_512": bench_life512(be, ev),
-                           "batched_forest": bench_batched(be, ev),
-                           "lenia_2048": bench_lenia(be, ev),
-                           "serving": bench_serving(),
-                           "serving_ensembles": bench_serving_ensembles()}
+        configs = {"life_512": lambda: bench_life512(be, ev),
+                   "batched_forest": lambda: bench_batched(be, ev),
+                   "lenia_2048": lambda: bench_lenia(be, ev),
+                   "field_epidemic_2048": lambda: bench_field_epidemic(be, ev),
+                   "counts_16384": lambda: bench_counts(be, ev),
+                   "serving": bench_serving,
+                   "serving_ensembles": bench_serving_ensembles}
+        names = list(configs) if args.configs is None else args.configs.split(",")
+        unknown = [k for k in names if k not in configs]
+        if unknown:
+            sys.exit(f"unknown configs {unknown}; known: {list(configs)}")
+        line["configs"] = {k: configs[k]() for k in names}
     if not args.no_cpu:
         n = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(n)

@@ -754,7 +754,8 @@ extern "C" int cah_state_get_device(cah_state* s, void* dptr, size_t nbytes) {
 static int rule_values(const cah_state* s) {
     switch (s->d.rule) {
     case CAH_RULE_LIFE: return 2;
-    case CAH_RULE_FOREST_FIRE: return 3;
+    case CAH_RULE_FOREST_FIRE:
+    case CAH_RULE_FIELD_EPIDEMIC: return 3;
     case CAH_RULE_GENERATIONS: return std::max(2, s->d.gen_states);
     default: return 16;
     }
@@ -776,7 +777,7 @@ template <int K>
 static void launch_counts(int blocks, const uint8_t* a, const uint8_t* b, const cah_state* s,
                           unsigned long long* part, unsigned long long* out, hipStream_t st) {
     hipLaunchKernelGGL(k_counts<K>, blocks, 256, 0, st, a, b, s->pitch, s->n1, s->n2, part);
-    hipLaunchKernelGGL(k_counts_reduce<K>, 1, 256, 0, st, part, blocks, out);
+    hipLaunchKernelGGL(k_counts_reduce<K>, 1, CNT_RED, 0, st, part, blocks, out);
 }
 
 // Queue the counts of the values 0 .. K-1 of s, and its agreement with o, into
@@ -1004,6 +1005,9 @@ static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p
         a.fff_f32 = p->ff_f_is_f32;
     }
     if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC) {
+        // the rule reads its field, not its neighbours: no count, no halo
+        a.nofs = 0;
+        a.h = 0;
         a.field = s->field_d;
         a.field32 = s->d.lenia_precision == 32;
         a.epi_form = s->d.epi_form;

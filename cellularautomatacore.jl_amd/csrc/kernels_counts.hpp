@@ -28,7 +28,7 @@
 constexpr int CNT_SEG = 1024;      // bytes of a row per wave-iteration
 constexpr int CNT_SLOTS = 258;     // per lattice: counts[256], equal, spare
 constexpr int CNT_BLOCKS = 2048;   // most blocks of one k_counts launch
-constexpr int CNT_PART = 17;       // per block: counts of 0..15, equal
+constexpr int CNT_PART = 17;       // slots of partial sums: counts of 0..15, equal
 
 __device__ inline uint32_t zero_byte_tops(uint32_t x) {
     return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
@@ -41,8 +41,9 @@ __device__ inline unsigned long long wave_sum64(unsigned long long v) {
 
 // s, o: pitched rows (pitch bytes apart, n1 cells each; o may be null); the
 // pitch is a multiple of 16, so a 16-byte load that starts inside a row stays
-// inside its pitch.  part[b * CNT_PART + k] = block b's cells equal to k (k < K),
-// part[b * CNT_PART + 16] = its cells where s and o agree (0 without o).
+// inside its pitch.  part[k * CNT_BLOCKS + b] = block b's cells equal to k
+// (k < K), part[16 * CNT_BLOCKS + b] = its cells where s and o agree (0
+// without o): slot-major, so the reduction reads each slot coalesced.
 template <int K>
 __global__ __launch_bounds__(256) void k_counts(const uint8_t* __restrict__ s, const uint8_t* __restrict__ o,
                                                 int64_t pitch, int64_t n1, int64_t n2,
@@ -116,27 +117,39 @@ __global__ __launch_bounds__(256) void k_counts(const uint8_t* __restrict__ s, c
     if (threadIdx.x <= K) {
         const int k = threadIdx.x;
         const unsigned long long t = sums[0][k] + sums[1][k] + sums[2][k] + sums[3][k];
-        part[(int64_t)blockIdx.x * CNT_PART + (k == K ? 16 : k)] = t;
+        part[(int64_t)(k == K ? 16 : k) * CNT_BLOCKS + blockIdx.x] = t;
     }
 }
 
 // out[0 .. CNT_SLOTS): the sums of k_counts' nblocks slots (counts of 0..K-1,
-// the agreement in out[256]); every other slot is set to 0.  One block.
+// the agreement in out[256]); every other slot is set to 0.  One block of
+// CNT_RED threads, two partial sums each at most.
+constexpr int CNT_RED = 1024;
 template <int K>
-__global__ __launch_bounds__(256) void k_counts_reduce(const unsigned long long* __restrict__ part, int nblocks,
-                                                       unsigned long long* __restrict__ out) {
-    __shared__ unsigned long long w[4];
+__global__ __launch_bounds__(CNT_RED) void k_counts_reduce(const unsigned long long* __restrict__ part,
+                                                           int nblocks, unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long w[CNT_RED / 64][K + 1];
+    unsigned long long t[K + 1];
+#pragma unroll
+    for (int k = 0; k <= K; ++k) t[k] = 0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) t[k] += part[(int64_t)k * CNT_BLOCKS + b];
+        t[K] += part[(int64_t)16 * CNT_BLOCKS + b];
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k <= K; ++k) {
+        const unsigned long long v = wave_sum64(t[k]);
+        if (lane == 0) w[wv][k] = v;
+    }
     for (int i = threadIdx.x; i < CNT_SLOTS; i += blockDim.x) out[i] = 0;
     __syncthreads();
-    for (int k = 0; k <= K; ++k) {
-        const int slot = k == K ? 16 : k;
-        unsigned long long t = 0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) t += part[(int64_t)b * CNT_PART + slot];
-        t = wave_sum64(t);
-        if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = t;
-        __syncthreads();
-        if (threadIdx.x == 0) out[k == K ? 256 : k] = w[0] + w[1] + w[2] + w[3];
-        __syncthreads();
+    if (threadIdx.x <= K) {
+        const int k = threadIdx.x;
+        unsigned long long v = 0;
+        for (int i = 0; i < CNT_RED / 64; ++i) v += w[i][k];
+        out[k == K ? 256 : k] = v;
     }
 }
 

@@ -25,14 +25,28 @@ __global__ __launch_bounds__(256) void k_epi_field(const int8_t* __restrict__ cu
         etile[i] = read_mapped<int8_t>(cur, L, cv1, cv2, x0 - h + lx, y0 - h + ly);
     }
     __syncthreads();
-    const int tx = threadIdx.x % EPI_TX;
-    for (int ty = threadIdx.x / EPI_TX; ty < EPI_TY; ty += blockDim.x / EPI_TX) {
-        const int x = x0 + tx, y = y0 + ty;
-        if (x >= L.n1 || y >= L.n2) continue;
-        const int8_t* cp = etile + (tx + h) + (ty + h) * W;
-        T s = T(0);
-        for (int k = 0; k < ntaps; ++k)
-            if (cp[taps[2 * k] + taps[2 * k + 1] * W] == 1) s = s + w[k];
-        field[(int64_t)y * L.n1 + x] = s;
+    // a thread's four cells (rows ty, ty + 4, ty + 8, ty + 12 of the tile) keep four
+    // independent sums, so the tap loop issues four LDS reads and four adds a tap;
+    // each cell still adds its taps in order
+    const int tx = threadIdx.x % EPI_TX, ty = threadIdx.x / EPI_TX;
+    constexpr int R = EPI_TY / 4;
+    static_assert(EPI_TX * 4 == 256, "four rows of threads a block");
+    const int8_t* cp = etile + (tx + h) + (ty + h) * W;
+    T s[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) s[r] = T(0);
+#pragma unroll 4
+    for (int k = 0; k < ntaps; ++k) {
+        const int off = taps[2 * k] + taps[2 * k + 1] * W;
+        const T wk = w[k];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (cp[off + 4 * r * W] == 1) s[r] = s[r] + wk;
+    }
+    const int x = x0 + tx;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = y0 + ty + 4 * r;
+        if (x < L.n1 && y < L.n2) field[(int64_t)y * L.n1 + x] = s[r];
     }
 }
