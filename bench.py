@@ -331,6 +331,46 @@ def bench_counts(be, ev, reps=20):
             "gbs_state_read": round(n * n / (ms * 1e-3) / 1e9, 1), "counts": counts}
 
 
+def bench_sparse(be, ev, steps=200):
+    """The sparse sweep against the dense one on invasions of a 16384^2 lattice
+    (the SparseBackend's use, vignettes/10_backends/10_backends.md:175-197): a fire
+    burning out from the centre of a full forest (forest fire, background 1, p 0.01,
+    f 0, Reflecting, seed 7) and an R-pentomino on a torus (Conway Life, background
+    0); tiles of 32 x 32.  ms/step over `steps` steps from the seed state, the tiles
+    stored at the end, and whether both sweeps end in the same state."""
+    from cahip import model as M
+    from cahip.backend import HIPSparseBackend, check, make_params
+    from cahip.problem import CAProblem
+    n = 16384
+    fire = np.ones((n, n), np.int8)
+    fire[n // 2, n // 2] = 2
+    life = np.zeros((n, n), np.int8)
+    life[n // 2:n // 2 + 3, n // 2:n // 2 + 3] = np.array([[0, 1, 1], [1, 1, 0], [0, 1, 0]], np.int8)
+    cases = {"forest_fire": (CAProblem(M.ForestFire(), fire, steps, {"p": 0.01, "f": 0.0}, seed=7,
+                                       space=M.Grid(n, n, boundary=M.Reflecting())), 1),
+             "life_r_pentomino": (CAProblem(M.Life("conway"), life, steps, space=M.Torus(n, n)), 0)}
+    out = {"workload": f"{n}^2: forest fire burning out of a full forest (p 0.01, f 0), and an R-pentomino "
+                       f"under Conway Life on a torus; {steps} steps, tiles 32 x 32"}
+    del fire, life
+    for name, (prob, bg) in cases.items():
+        cp = make_params(prob, prob.p)
+        r = {}
+        finals = {}
+        for kind, b in (("dense", be), ("sparse", HIPSparseBackend(bg, tilesize=32))):
+            st = b.allocate_state(prob, M.Synchronous())
+            ms = timed(ev, b.stream(st), lambda: check(b.lib.cah_advance(st.handle, 0, steps, ctypes.byref(cp))))
+            r[f"{kind}_ms_per_step"] = round(ms / steps, 4)
+            if kind == "sparse":
+                r["stored_tiles"] = b.stored_tiles(st)
+                r["tiles"] = (n // 32) ** 2
+            finals[kind] = b.state_counts(st)
+            b.free_state(st)
+        r["speedup"] = round(r["dense_ms_per_step"] / r["sparse_ms_per_step"], 1)
+        r["same_counts"] = finals["dense"] == finals["sparse"]
+        out[name] = r
+    return out
+
+
 def bench_lenia(be, ev, steps=500):
     """Lenia R=13 (mu 0.15, sigma 0.015, dt 0.1), Float32 Torus 2048^2, uniform[0,1]
     in the central 256^2 patch: ms/step per field method and the per-step max-abs
@@ -681,6 +721,7 @@ def main():
                    "lenia_2048": lambda: bench_lenia(be, ev),
                    "field_epidemic_2048": lambda: bench_field_epidemic(be, ev),
                    "counts_16384": lambda: bench_counts(be, ev),
+                   "sparse_16384": lambda: bench_sparse(be, ev),
                    "serving": bench_serving,
                    "serving_ensembles": bench_serving_ensembles}
         names = list(configs) if args.configs is None else args.configs.split(",")

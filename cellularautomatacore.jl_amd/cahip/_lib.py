@@ -100,6 +100,8 @@ SIGNATURES = {
     "cah_state_sync": (_I, [_P]),
     "cah_state_stream": (_I, [_P, ctypes.POINTER(_P)]),
     "cah_state_tune": (_I, [_P, _I32, _I32]),
+    "cah_state_sparse": (_I, [_P, ctypes.c_int8, _I32]),
+    "cah_state_active_tiles": (_I, [_P, ctypes.POINTER(_I64)]),
     "cah_state_destroy": (_I, [_P]),
     "cah_state_fill_bernoulli": (_I, [_P, _U64, _D]),
     "cah_state_fill_uniform_box": (_I, [_P, _U64, _I64, _I64, _I64, _I64]),

@@ -330,3 +330,42 @@ class HIPBackend:
                 self.lib.cah_close(self._ctx)
             except Exception:
                 pass
+
+
+class HIPSparseBackend(HIPBackend):
+    """HIPSparseBackend(background; tilesize=32): SparseBackend(background;
+    tilesize) (src/sparse.jl:6-40) on the device.  Each step updates only the
+    tiles within reach of a tile holding a non-background cell; the states and
+    results are those of the dense sweep.  Synchronous and Stochastic schemes, no
+    kernel fields, quiescent rules only (checked when the state is allocated, and
+    at every step for the forest fire's parameters)."""
+
+    def __init__(self, background, tilesize=32, device=0):
+        super().__init__(device)
+        if tilesize < 1:
+            raise M.ArgumentError(f"tilesize must be at least 1, got {tilesize}")
+        self.background = int(background)
+        self.tilesize = int(tilesize)
+
+    def supports(self, scheme):
+        return isinstance(scheme, (M.Synchronous, M.Stochastic))
+
+    def check_problem(self, prob, scheme):
+        if isinstance(prob.rule, (M.Lenia, M.FieldEpidemic)):
+            raise M.ArgumentError("SparseBackend does not support kernel fields; use ArrayBackend")
+        super().check_problem(prob, scheme)
+
+    def allocate_state(self, prob, scheme):
+        st = super().allocate_state(prob, scheme)
+        try:
+            check(self.lib.cah_state_sparse(st.handle, self.background, self.tilesize))
+        except Exception:
+            self.free_state(st)
+            raise
+        return st
+
+    # stored_tiles(integ) (src/sparse.jl)
+    def stored_tiles(self, st):
+        n = ctypes.c_int64()
+        check(self.lib.cah_state_active_tiles(st.handle, ctypes.byref(n)))
+        return n.value

@@ -234,6 +234,17 @@ struct cah_state {
     void* frag_d = nullptr;    // MFMA A fragments
     unsigned long long* counts_d = nullptr;  // CNT_SLOTS counters of cah_state_counts
     double* cov_d = nullptr;   // field epidemic: covariate slices
+    // the sparse sweep (cah_state_sparse): tile flags of the current, previous and
+    // next state, always-swept tiles, the candidate list and its length
+    bool sparse = false;
+    int8_t sp_bg = 0;
+    int32_t sp_ts = 0, sp_tsy = 0, sp_ntx = 0, sp_nty = 0, sp_rx = 0, sp_ry = 0;
+    uint8_t* sp_flags[3] = {nullptr, nullptr, nullptr};
+    int sp_cur = 0;            // sp_flags[sp_cur]: current; +1: previous; +2: next (mod 3)
+    uint8_t* sp_edge = nullptr;
+    int32_t* sp_list = nullptr;
+    int32_t* sp_count = nullptr;   // two counters, used in alternate steps
+    int sp_par = 0;
     std::vector<double> cov;   //   ... host copy while the state is created
     int32_t epi_h = 0;         //   ... its field's halo
     int lenia_mode = 0;        // LM_F16 / LM_BF16 / LM_F16X2
@@ -413,7 +424,8 @@ static void free_state(cah_state* s) {
     if (s->stream && !s->arena) hipStreamSynchronize(s->stream);
     for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
                     (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d, s->frag_d,
-                    (void*)s->counts_d, (void*)s->cov_d})
+                    (void*)s->counts_d, (void*)s->cov_d, (void*)s->sp_flags[0], (void*)s->sp_flags[1],
+                    (void*)s->sp_flags[2], (void*)s->sp_edge, (void*)s->sp_list, (void*)s->sp_count})
         dev_free(s, p);
     lenia_fft_destroy(s->fft);
     if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
@@ -613,6 +625,10 @@ extern "C" int cah_state_tune(cah_state* s, int32_t vb, int32_t rows) {
 
 static size_t state_bytes(const cah_state* s) { return (size_t)(s->n1 * s->n2) * s->es; }
 
+// the sparse sweep's bookkeeping (defined with it below)
+static int sparse_refresh(cah_state* s);
+static uint8_t* sp_flag(cah_state* s, int which);
+
 static bool scan_binary(const int8_t* h, size_t n) {
     for (size_t i = 0; i < n; ++i)
         if ((uint8_t)h[i] > 1) return false;
@@ -702,6 +718,15 @@ static int copy_out(cah_state* s, void* host) {
         HIPCHK(hipStreamSynchronize(s->stream));
         hostcopy::copy((char*)host + (size_t)y * row, g_stage, nb);
     }
+    // the sparse sweep leaves tiles of the other buffer unwritten and needs them
+    // to hold the background: after its use as scratch, refill it and forget the
+    // previous step's tiles (none of them is left in it)
+    if (row != pitchb && s->sparse) {
+        HIPCHK(hipMemsetAsync(s->buf[1 - s->cur], (int)(uint8_t)s->sp_bg, (size_t)s->pitch * (size_t)s->n2,
+                              s->stream));
+        HIPCHK(hipMemsetAsync(sp_flag(s, 1), 0, (size_t)s->sp_ntx * (size_t)s->sp_nty, s->stream));
+        HIPCHK(hipStreamSynchronize(s->stream));
+    }
     return CAH_OK;
 }
 
@@ -712,6 +737,7 @@ extern "C" int cah_state_set(cah_state* s, const void* host, size_t nbytes) {
     HIPCHK(hipSetDevice(s->device));
     if (int rc = copy_in(s, host)) return rc;
     if (s->es == 1) s->bin_clean = scan_binary((const int8_t*)host, nbytes);
+    if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
 
@@ -733,6 +759,7 @@ extern "C" int cah_state_set_device(cah_state* s, const void* dptr, size_t nbyte
         return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->es == 1) s->bin_clean = false;  // not scanned
+    if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
 
@@ -862,6 +889,8 @@ extern "C" int cah_state_write_cell(cah_state* s, int64_t i1, int64_t i2, const 
     if (int rc = cell_ptr(s, i1, i2, &p)) return rc;
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipMemcpyAsync(p, value, s->es, hipMemcpyHostToDevice, s->stream));
+    if (s->sparse)  // the cell's tile may now hold a non-background cell
+        HIPCHK(hipMemsetAsync(sp_flag(s, 0) + (i2 / s->sp_tsy) * s->sp_ntx + i1 / s->sp_ts, 1, 1, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->es == 1 && *(const uint8_t*)value > 1) s->bin_clean = false;
     return CAH_OK;
@@ -977,7 +1006,7 @@ static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
     return CAH_OK;
 }
 
-static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p) {
+static GenArgs generic_args(cah_state* s, int64_t t, int gate, const cah_params* p) {
     GenArgs a{};
     a.cur = (const int8_t*)s->buf[s->cur];
     a.next = (int8_t*)s->buf[1 - s->cur];
@@ -1015,9 +1044,167 @@ static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p
         a.cov = s->cov_d;
         a.cov_slices = (int32_t)s->d.covariate_slices;
     }
+    return a;
+}
+
+static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p) {
+    const GenArgs a = generic_args(s, t, gate, p);
     dim3 grid((unsigned)((s->n1 + GEN_TX - 1) / GEN_TX), (unsigned)((s->n2 + GEN_TY - 1) / GEN_TY));
     const size_t lds = (size_t)(GEN_TX + 2 * a.h) * (GEN_TY + 2 * a.h);
     k_generic_i8<<<grid, 256, lds, s->stream>>>(a);
+    return CAH_OK;
+}
+
+// ---------------------------------------------------------------- sparse sweep
+
+// Flags of the current (0), previous (1) and next (2) state.
+static uint8_t* sp_flag(cah_state* s, int which) { return s->sp_flags[(s->sp_cur + which) % 3]; }
+
+// After the state is set or the sweep is switched on: the other buffer all
+// background, the current flags from the current buffer, no previous flags.
+static int sparse_refresh(cah_state* s) {
+    const size_t nt = (size_t)s->sp_ntx * (size_t)s->sp_nty;
+    HIPCHK(hipMemsetAsync(s->buf[1 - s->cur], (int)(uint8_t)s->sp_bg, (size_t)s->pitch * (size_t)s->n2, s->stream));
+    for (uint8_t* f : s->sp_flags) HIPCHK(hipMemsetAsync(f, 0, nt, s->stream));
+    HIPCHK(hipMemsetAsync(s->sp_count, 0, 2 * sizeof(int32_t), s->stream));
+    s->sp_par = 0;
+    dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
+    k_sparse_flags<<<grid, 256, 0, s->stream>>>((const int8_t*)s->buf[s->cur], lat_of(s), s->sp_ts, s->sp_tsy,
+                                                s->sp_ntx, s->sp_bg, sp_flag(s, 0));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return CAH_OK;
+}
+
+// Whether a background cell whose neighbours are all background keeps the
+// background (the SparseBackend's requirement, src/sparse.jl:49-69), decided
+// exactly for the rule families the sparse sweep runs; the forest fire depends on
+// its parameters.
+static bool quiescent(const cah_state* s, int8_t bg, const cah_params* p) {
+    const cah_desc& d = s->d;
+    const int n1 = bg == 1 ? d.n_offsets : 0;  // neighbours equal to 1
+    switch (d.rule) {
+    case CAH_RULE_LIFE: {
+        const uint64_t m = bg == 1 ? d.survive : d.birth;
+        return (n1 < 64 ? (int)((m >> n1) & 1u) : 0) == bg;
+    }
+    case CAH_RULE_GENERATIONS: {
+        int out;
+        if (bg == 0) out = n1 < 64 ? (int)((d.birth >> n1) & 1u) : 0;
+        else if (bg == 1) out = (n1 < 64 && ((d.survive >> n1) & 1u)) ? 1 : (d.gen_states > 2 ? 2 : 0);
+        else out = bg + 1 < d.gen_states ? bg + 1 : 0;
+        return out == bg;
+    }
+    case CAH_RULE_FOREST_FIRE:
+        if (!p) return false;
+        if (bg == 0) return !(p->ff_p > 0);  // no growth
+        if (bg == 1) return !(p->ff_f > 0);  // no lightning
+        return false;
+    default: return false;
+    }
+}
+
+extern "C" int cah_state_sparse(cah_state* s, int8_t background, int32_t tilesize) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "state is NULL");
+    const cah_desc& d = s->d;
+    if (d.dtype != CAH_DTYPE_I8) return fail(CAH_ERR_ARGUMENT, "the sparse sweep runs Int8 states");
+    if (d.rule == CAH_RULE_FIELD_EPIDEMIC)
+        return fail(CAH_ERR_ARGUMENT, "SparseBackend does not support kernel fields; use ArrayBackend");
+    if (d.rule != CAH_RULE_LIFE && d.rule != CAH_RULE_GENERATIONS && d.rule != CAH_RULE_FOREST_FIRE)
+        return fail(CAH_ERR_ARGUMENT, "the sparse sweep runs Life, Generations and the forest fire");
+    if (d.scheme != CAH_SCHEME_SYNCHRONOUS && d.scheme != CAH_SCHEME_STOCHASTIC)
+        return fail(CAH_ERR_UNSUPPORTED, "SparseBackend runs Synchronous and Stochastic");
+    if (tilesize < 1 || tilesize > 128) return fail(CAH_ERR_ARGUMENT, "tilesize must be in 1..128, got %d", tilesize);
+    if (s->arena) return fail(CAH_ERR_ARGUMENT, "a server's states run dense");
+    if (d.rule != CAH_RULE_FOREST_FIRE && !quiescent(s, background, nullptr))
+        return fail(CAH_ERR_ARGUMENT,
+                    "SparseBackend needs a quiescent rule: a background cell (%d) with background neighbours "
+                    "changes", (int)background);
+    HIPCHK(hipSetDevice(s->device));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (void* q : {(void*)s->sp_flags[0], (void*)s->sp_flags[1], (void*)s->sp_flags[2], (void*)s->sp_edge,
+                    (void*)s->sp_list, (void*)s->sp_count})
+        dev_free(s, q);
+    const bool two = d.ndims == 2;
+    s->sp_ts = tilesize;
+    s->sp_tsy = two ? tilesize : 1;
+    s->sp_ntx = (int32_t)((s->n1 + s->sp_ts - 1) / s->sp_ts);
+    s->sp_nty = (int32_t)((s->n2 + s->sp_tsy - 1) / s->sp_tsy);
+    const int h = s->halo;
+    // tiles within reach of a cell's neighbourhood; one more on a periodic axis
+    // whose last tile is partial (a wrapped halo can reach past it)
+    s->sp_rx = (h + s->sp_ts - 1) / s->sp_ts +
+               (d.boundary[0] == CAH_BOUNDARY_PERIODIC && s->n1 % s->sp_ts ? 1 : 0);
+    s->sp_ry = two ? (h + s->sp_tsy - 1) / s->sp_tsy +
+                         (d.boundary[1] == CAH_BOUNDARY_PERIODIC && s->n2 % s->sp_tsy ? 1 : 0)
+                   : 0;
+    const size_t nt = (size_t)s->sp_ntx * (size_t)s->sp_nty;
+    // every tile holding a cell within h of a Constant edge that is not the background
+    std::vector<uint8_t> edge(nt, 0);
+    for (int ax = 0; ax < (two ? 2 : 1); ++ax) {
+        if (d.boundary[ax] != CAH_BOUNDARY_CONSTANT || (int8_t)d.boundary_value[ax] == background) continue;
+        const int64_t n = ax == 0 ? s->n1 : s->n2, ts = ax == 0 ? s->sp_ts : s->sp_tsy;
+        for (int64_t c = 0; c < n; ++c) {
+            if (c >= h && c < n - h) continue;
+            const int64_t tc = c / ts;
+            if (ax == 0)
+                for (int64_t ty = 0; ty < s->sp_nty; ++ty) edge[(size_t)(ty * s->sp_ntx + tc)] = 1;
+            else
+                for (int64_t tx = 0; tx < s->sp_ntx; ++tx) edge[(size_t)(tc * s->sp_ntx + tx)] = 1;
+        }
+    }
+    for (uint8_t*& f : s->sp_flags)
+        if (int rc = dev_alloc(s, (void**)&f, nt)) return rc;
+    if (int rc = upload(s, &s->sp_edge, edge.data(), nt)) return rc;
+    if (int rc = dev_alloc(s, (void**)&s->sp_list, nt * sizeof(int32_t))) return rc;
+    if (int rc = dev_alloc(s, (void**)&s->sp_count, 2 * sizeof(int32_t))) return rc;
+    s->sparse = true;
+    s->sp_bg = background;
+    s->sp_cur = 0;
+    return sparse_refresh(s);
+}
+
+extern "C" int cah_state_active_tiles(cah_state* s, int64_t* n) {
+    if (!s || !n) return fail(CAH_ERR_ARGUMENT, "NULL argument");
+    if (!s->sparse) return fail(CAH_ERR_ARGUMENT, "the state does not run the sparse sweep");
+    HIPCHK(hipSetDevice(s->device));
+    std::vector<uint8_t> f((size_t)s->sp_ntx * (size_t)s->sp_nty);
+    HIPCHK(hipMemcpyAsync(f.data(), sp_flag(s, 0), f.size(), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int64_t c = 0;
+    for (uint8_t v : f) c += v != 0;
+    *n = c;
+    return CAH_OK;
+}
+
+// One sparse step: list the candidate tiles, sweep them, rotate the flags.
+static int sparse_step(cah_state* s, int64_t t, const cah_params* p) {
+    SparseArgs sa{};
+    sa.act_cur = sp_flag(s, 0);
+    sa.act_prev = sp_flag(s, 1);
+    sa.act_next = sp_flag(s, 2);
+    sa.edge = s->sp_edge;
+    sa.list = s->sp_list;
+    sa.count = s->sp_count + s->sp_par;
+    sa.count_next = s->sp_count + (1 - s->sp_par);
+    sa.ntx = s->sp_ntx;
+    sa.nty = s->sp_nty;
+    sa.reach1 = s->sp_rx;
+    sa.reach2 = s->sp_ry;
+    sa.per1 = s->d.boundary[0] == CAH_BOUNDARY_PERIODIC;
+    sa.per2 = s->d.ndims == 2 && s->d.boundary[1] == CAH_BOUNDARY_PERIODIC;
+    const int64_t nt = (int64_t)s->sp_ntx * s->sp_nty;
+    k_sparse_mark<<<(unsigned)((nt + 255) / 256), 256, 0, s->stream>>>(sa);
+    const GenArgs a = generic_args(s, t, 0, p);
+    const size_t lds = (size_t)(s->sp_ts + 2 * a.h) * (size_t)(s->sp_tsy + 2 * a.h);
+    // persistent blocks walk the list (4 a CU: enough for a dense front, few idle
+    // ones for a small one)
+    const int blocks = (int)std::min<int64_t>(nt, 4 * device_cus(s->device));
+    k_sparse_sweep<<<blocks, 256, lds, s->stream>>>(a, sa, s->sp_ts, s->sp_tsy, s->sp_bg);
+    HIPCHK(hipGetLastError());
+    s->sp_par = 1 - s->sp_par;
+    s->sp_cur = (s->sp_cur + 2) % 3;  // next -> current, current -> previous
+    s->cur = 1 - s->cur;
     return CAH_OK;
 }
 
@@ -1130,6 +1317,16 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
     if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC && !p)
         return fail(CAH_ERR_ARGUMENT, "the field epidemic needs its parameters (beta, gamma)");
     HIPCHK(hipSetDevice(s->device));
+    if (s->sparse) {
+        if (!quiescent(s, s->sp_bg, p))
+            return fail(CAH_ERR_ARGUMENT,
+                        "SparseBackend needs a quiescent rule: with these parameters a background cell (%d) "
+                        "with background neighbours changes", (int)s->sp_bg);
+        for (int64_t k = 0; k < nsteps; ++k)
+            if (int rc = sparse_step(s, t0 + k, p)) return rc;
+        if (s->d.rule != CAH_RULE_LIFE && nsteps > 0) s->bin_clean = false;
+        return CAH_OK;
+    }
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t t = t0 + k;
         if (s->path == P_LENIA_SEQ) launch_lenia_field_seq(s, t);
@@ -1166,6 +1363,7 @@ extern "C" int cah_state_fill_bernoulli(cah_state* s, uint64_t seed, double p) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     s->bin_clean = true;
+    if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
 

@@ -76,6 +76,40 @@ CAH_DEV int epidemic_rule(const GenArgs& a, int c, int64_t lin, uint64_t cellid)
 
 constexpr int GEN_TX = 64, GEN_TY = 16;
 
+// One cell of the general sweep: cp points at the cell in an LDS tile of width W
+// that holds its neighbourhood; (x, y) are its 0-based coordinates.
+CAH_DEV int generic_cell(const GenArgs& a, const int8_t* cp, int W, int x, int y) {
+    const int64_t lin = (int64_t)y * a.L.n1 + x;  // unpadded, 0-based
+    const uint64_t cellid = (uint64_t)lin + 1;
+    const int c = *cp;
+    const bool open = (a.mask == nullptr || a.mask[lin] != 0) &&
+                      gate_open(a.scheme, a.gate, a.sp, a.sp_f32, a.seed, cellid, a.t, x, y, a.L.two_d);
+    if (!open) return c;
+    int n1 = 0, n2 = 0;  // neighbours equal to 1, and to 2
+    for (int k = 0; k < a.nofs; ++k) {
+        const int v = cp[a.ofs[2 * k] + a.ofs[2 * k + 1] * W];
+        n1 += (v == 1);
+        n2 += (v == 2);
+    }
+    if (a.rule == CAH_RULE_LIFE) {
+        // local_rule(::Life) (src/rules.jl:385-389)
+        const uint64_t m = c == 1 ? a.survive : a.birth;
+        return n1 < 64 ? (int)((m >> n1) & 1u) : 0;
+    }
+    if (a.rule == CAH_RULE_GENERATIONS) {
+        // local_rule(::Generations) (src/rules.jl:446-456)
+        if (c == 0) return n1 < 64 ? (int)((a.birth >> n1) & 1u) : 0;
+        if (c == 1) return (n1 < 64 && ((a.survive >> n1) & 1u)) ? 1 : (a.C > 2 ? 2 : 0);
+        return c + 1 < a.C ? c + 1 : 0;
+    }
+    if (a.rule == CAH_RULE_FIELD_EPIDEMIC) return epidemic_rule(a, c, lin, cellid);
+    // the forest-fire LocalRule (README.md:56-60), Int8 encoding
+    if (c == 2) return 0;
+    if (c == 0) return bernoulli(a.seed, cellid, a.t, 1, a.ffp, a.ffp_f32) ? 1 : 0;
+    if (n2 > 0) return 2;
+    return bernoulli(a.seed, cellid, a.t, 2, a.fff, a.fff_f32) ? 2 : 1;
+}
+
 __global__ __launch_bounds__(256) void k_generic_i8(GenArgs a) {
     extern __shared__ int8_t tile[];
     const int W = GEN_TX + 2 * a.h, H = GEN_TY + 2 * a.h;
@@ -90,42 +124,99 @@ __global__ __launch_bounds__(256) void k_generic_i8(GenArgs a) {
     for (int ty = threadIdx.x / GEN_TX; ty < GEN_TY; ty += blockDim.x / GEN_TX) {
         const int x = x0 + tx, y = y0 + ty;
         if (x >= a.L.n1 || y >= a.L.n2) continue;
-        const int64_t lin = (int64_t)y * a.L.n1 + x;  // unpadded, 0-based
-        const uint64_t cellid = (uint64_t)lin + 1;
-        const int8_t* cp = tile + (tx + a.h) + (ty + a.h) * W;
-        const int c = *cp;
-        int out = c;
-        const bool open = (a.mask == nullptr || a.mask[lin] != 0) &&
-                          gate_open(a.scheme, a.gate, a.sp, a.sp_f32, a.seed, cellid, a.t, x, y,
-                                    a.L.two_d);
-        if (open) {
-            int n1 = 0, n2 = 0;  // neighbours equal to 1, and to 2
-            for (int k = 0; k < a.nofs; ++k) {
-                const int v = cp[a.ofs[2 * k] + a.ofs[2 * k + 1] * W];
-                n1 += (v == 1);
-                n2 += (v == 2);
-            }
-            if (a.rule == CAH_RULE_LIFE) {
-                // local_rule(::Life) (src/rules.jl:385-389)
-                const uint64_t m = c == 1 ? a.survive : a.birth;
-                out = n1 < 64 ? (int)((m >> n1) & 1u) : 0;
-            } else if (a.rule == CAH_RULE_GENERATIONS) {
-                // local_rule(::Generations) (src/rules.jl:446-456)
-                if (c == 0) out = n1 < 64 ? (int)((a.birth >> n1) & 1u) : 0;
-                else if (c == 1) out = (n1 < 64 && ((a.survive >> n1) & 1u)) ? 1 : (a.C > 2 ? 2 : 0);
-                else out = c + 1 < a.C ? c + 1 : 0;
-            } else if (a.rule == CAH_RULE_FIELD_EPIDEMIC) {
-                out = epidemic_rule(a, c, lin, cellid);
-            } else {
-                // the forest-fire LocalRule (README.md:56-60), Int8 encoding
-                if (c == 2) out = 0;
-                else if (c == 0) out = bernoulli(a.seed, cellid, a.t, 1, a.ffp, a.ffp_f32) ? 1 : 0;
-                else if (n2 > 0) out = 2;
-                else out = bernoulli(a.seed, cellid, a.t, 2, a.fff, a.fff_f32) ? 2 : 1;
-            }
-        }
+        const int out = generic_cell(a, tile + (tx + a.h) + (ty + a.h) * W, W, x, y);
         a.next[(int64_t)y * a.L.pitch + x] = (int8_t)out;
     }
+}
+
+// ---------------------------------------------------------------- the sparse sweep
+//
+// The SparseBackend's active-tile step (src/sparse.jl:126-211) on a dense lattice:
+// tiles of ts x tsy cells (tsy = ts in 2-D, 1 in 1-D) carry a flag "may hold a
+// cell that is not the background".  k_sparse_mark lists the tiles within
+// `reach` tiles of a flagged one, the tiles flagged one step earlier (their cells
+// in the buffer being written still hold that step's values) and the tiles on a
+// Constant edge whose value is not the background; k_sparse_sweep updates exactly
+// those tiles and flags the ones that keep a non-background cell.  Every other
+// tile is background in both buffers, so the result is the dense sweep's.
+
+struct SparseArgs {
+    const uint8_t* act_cur;   // flags of the state being read
+    const uint8_t* act_prev;  // flags of the state one step earlier
+    uint8_t* act_next;        // flags of the state being written
+    const uint8_t* edge;      // always-swept tiles
+    int32_t* list;            // candidate tiles
+    int32_t* count;           // their number (this step's counter)
+    int32_t* count_next;      // the next step's counter, cleared by this step's sweep
+    int32_t ntx, nty, reach1, reach2;  // tiles a neighbourhood reaches, per axis
+    int32_t per1, per2;                // periodic tile axes
+};
+
+__global__ __launch_bounds__(256) void k_sparse_mark(SparseArgs s) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= s.ntx * s.nty) return;
+    s.act_next[k] = 0;
+    const int32_t tx = k % s.ntx, ty = k / s.ntx;
+    bool cand = s.act_prev[k] || s.edge[k];
+    for (int32_t dy = -s.reach2; dy <= s.reach2 && !cand; ++dy) {
+        int32_t y = ty + dy;
+        if (s.per2) y = (y % s.nty + s.nty) % s.nty;
+        if (y < 0 || y >= s.nty) continue;
+        for (int32_t dx = -s.reach1; dx <= s.reach1; ++dx) {
+            int32_t x = tx + dx;
+            if (s.per1) x = (x % s.ntx + s.ntx) % s.ntx;
+            if (x < 0 || x >= s.ntx) continue;
+            if (s.act_cur[(int64_t)y * s.ntx + x]) {
+                cand = true;
+                break;
+            }
+        }
+    }
+    if (cand) s.list[atomicAdd(s.count, 1)] = k;
+}
+
+// Blocks of 256 threads walk the candidate list; a tile's cells and their halo
+// go through an LDS tile of (ts + 2h) x (tsy + 2h) bytes.
+__global__ __launch_bounds__(256) void k_sparse_sweep(GenArgs a, SparseArgs s, int32_t ts, int32_t tsy,
+                                                      int8_t bg) {
+    extern __shared__ int8_t stile[];
+    __shared__ int any;
+    const int W = ts + 2 * a.h, H = tsy + 2 * a.h;
+    const int32_t n = *s.count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *s.count_next = 0;  // nobody reads it until the next mark
+    for (int32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const int32_t k = s.list[i];
+        const int x0 = (k % s.ntx) * ts, y0 = (k / s.ntx) * tsy;
+        if (threadIdx.x == 0) any = 0;
+        for (int j = threadIdx.x; j < W * H; j += blockDim.x) {
+            const int lx = j % W, ly = j / W;
+            stile[j] = read_mapped<int8_t>(a.cur, a.L, (int8_t)a.cv1, (int8_t)a.cv2, x0 - a.h + lx, y0 - a.h + ly);
+        }
+        __syncthreads();
+        int mine = 0;
+        for (int j = threadIdx.x; j < ts * tsy; j += blockDim.x) {
+            const int lx = j % ts, ly = j / ts;
+            const int x = x0 + lx, y = y0 + ly;
+            if (x >= a.L.n1 || y >= a.L.n2) continue;
+            const int out = generic_cell(a, stile + (lx + a.h) + (ly + a.h) * W, W, x, y);
+            a.next[(int64_t)y * a.L.pitch + x] = (int8_t)out;
+            mine |= out != bg;
+        }
+        if (mine) any = 1;
+        __syncthreads();
+        if (threadIdx.x == 0 && any) s.act_next[k] = 1;
+        __syncthreads();  // the tile and `any` are reused by the next candidate
+    }
+}
+
+// flags[k] = 1 when tile k holds a cell that is not the background (after a
+// state is set).
+__global__ __launch_bounds__(256) void k_sparse_flags(const int8_t* u, Lat L, int32_t ts, int32_t tsy,
+                                                      int32_t ntx, int8_t bg, uint8_t* flags) {
+    const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= L.n1) return;
+    for (int32_t y = blockIdx.y; y < L.n2; y += gridDim.y)
+        if (u[(int64_t)y * L.pitch + x] != bg) flags[(int64_t)(y / tsy) * ntx + x / ts] = 1;
 }
 
 // ---------------------------------------------------------------- synthetic inputs

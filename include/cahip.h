@@ -225,6 +225,21 @@ int cah_state_stream(cah_state* st, void** stream);
  * the default.  Results do not depend on it. */
 int cah_state_tune(cah_state* st, int32_t vb, int32_t rows);
 
+/* SparseBackend(background; tilesize) (src/sparse.jl:6-40, :126-211) on the
+ * device: from now on each step updates only the tiles (tilesize^2 cells, a
+ * tilesize-long segment in 1-D) within reach of a tile holding a cell that is not
+ * `background`, plus the tiles one step earlier held and the tiles beside a
+ * Constant edge whose value is not the background; every other cell stays the
+ * background.  The result equals the dense sweep's.  The rule must be quiescent
+ * (a background cell with background neighbours stays background): Life and
+ * Generations are checked here, the forest fire at every cah_advance (background
+ * 0 needs p = 0, background 1 needs f = 0); Synchronous and Stochastic only; no
+ * kernel fields (the field epidemic) and no server states.  tilesize in 1..128. */
+int cah_state_sparse(cah_state* st, int8_t background, int32_t tilesize);
+/* stored_tiles (src/sparse.jl): the tiles that may hold a non-background cell
+ * (exact after a step; a write_cell marks its tile). */
+int cah_state_active_tiles(cah_state* st, int64_t* n);
+
 /* Release a state; safe from any host thread (Julia finalizers). */
 int cah_state_destroy(cah_state* st);
 
