@@ -234,17 +234,21 @@ struct cah_state {
     void* frag_d = nullptr;    // MFMA A fragments
     unsigned long long* counts_d = nullptr;  // CNT_SLOTS counters of cah_state_counts
     double* cov_d = nullptr;   // field epidemic: covariate slices
-    // the sparse sweep (cah_state_sparse): tile flags of the current, previous and
-    // next state, always-swept tiles, the candidate list and its length
-    bool sparse = false;
+    // the sparse sweep (cah_state_sparse): two candidate lists used in alternate
+    // steps, three counters for them in rotation, per-tile stamps and edge flags,
+    // the edge tiles, two counters of non-background tiles, and while the state
+    // is rescanned, per-tile flags and the list of non-background tiles
+    bool sparse = false, sp_dirty = false;  // dirty: a cell was written since the scan
     int8_t sp_bg = 0;
-    int32_t sp_ts = 0, sp_tsy = 0, sp_ntx = 0, sp_nty = 0, sp_rx = 0, sp_ry = 0;
-    uint8_t* sp_flags[3] = {nullptr, nullptr, nullptr};
-    int sp_cur = 0;            // sp_flags[sp_cur]: current; +1: previous; +2: next (mod 3)
-    uint8_t* sp_edge = nullptr;
-    int32_t* sp_list = nullptr;
-    int32_t* sp_count = nullptr;   // two counters, used in alternate steps
-    int sp_par = 0;
+    int32_t sp_ts = 0, sp_tsy = 0, sp_ntx = 0, sp_nty = 0, sp_rx = 0, sp_ry = 0, sp_nedge = 0;
+    int32_t* sp_cand[2] = {nullptr, nullptr};
+    int32_t* sp_cnt = nullptr;     // 3 candidate counters, then 2 active counters, then 1 scan counter
+    int32_t* sp_stamp = nullptr;
+    uint8_t* sp_edgef = nullptr;
+    int32_t* sp_edges = nullptr;
+    uint8_t* sp_tmp = nullptr;
+    int32_t* sp_act = nullptr;
+    int32_t sp_step = 0;           // steps since the scan
     std::vector<double> cov;   //   ... host copy while the state is created
     int32_t epi_h = 0;         //   ... its field's halo
     int lenia_mode = 0;        // LM_F16 / LM_BF16 / LM_F16X2
@@ -424,8 +428,9 @@ static void free_state(cah_state* s) {
     if (s->stream && !s->arena) hipStreamSynchronize(s->stream);
     for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
                     (void*)s->w64_d, (void*)s->w32_d, (void*)s->wtab_d, s->field_d, s->frag_d,
-                    (void*)s->counts_d, (void*)s->cov_d, (void*)s->sp_flags[0], (void*)s->sp_flags[1],
-                    (void*)s->sp_flags[2], (void*)s->sp_edge, (void*)s->sp_list, (void*)s->sp_count})
+                    (void*)s->counts_d, (void*)s->cov_d, (void*)s->sp_cand[0], (void*)s->sp_cand[1],
+                    (void*)s->sp_cnt, (void*)s->sp_stamp, (void*)s->sp_edgef, (void*)s->sp_edges,
+                    (void*)s->sp_tmp, (void*)s->sp_act})
         dev_free(s, p);
     lenia_fft_destroy(s->fft);
     if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
@@ -627,7 +632,6 @@ static size_t state_bytes(const cah_state* s) { return (size_t)(s->n1 * s->n2) *
 
 // the sparse sweep's bookkeeping (defined with it below)
 static int sparse_refresh(cah_state* s);
-static uint8_t* sp_flag(cah_state* s, int which);
 
 static bool scan_binary(const int8_t* h, size_t n) {
     for (size_t i = 0; i < n; ++i)
@@ -724,7 +728,6 @@ static int copy_out(cah_state* s, void* host) {
     if (row != pitchb && s->sparse) {
         HIPCHK(hipMemsetAsync(s->buf[1 - s->cur], (int)(uint8_t)s->sp_bg, (size_t)s->pitch * (size_t)s->n2,
                               s->stream));
-        HIPCHK(hipMemsetAsync(sp_flag(s, 1), 0, (size_t)s->sp_ntx * (size_t)s->sp_nty, s->stream));
         HIPCHK(hipStreamSynchronize(s->stream));
     }
     return CAH_OK;
@@ -889,8 +892,7 @@ extern "C" int cah_state_write_cell(cah_state* s, int64_t i1, int64_t i2, const 
     if (int rc = cell_ptr(s, i1, i2, &p)) return rc;
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipMemcpyAsync(p, value, s->es, hipMemcpyHostToDevice, s->stream));
-    if (s->sparse)  // the cell's tile may now hold a non-background cell
-        HIPCHK(hipMemsetAsync(sp_flag(s, 0) + (i2 / s->sp_tsy) * s->sp_ntx + i1 / s->sp_ts, 1, 1, s->stream));
+    if (s->sparse) s->sp_dirty = true;  // the lists are rebuilt before the next step
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->es == 1 && *(const uint8_t*)value > 1) s->bin_clean = false;
     return CAH_OK;
@@ -1057,22 +1059,53 @@ static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p
 
 // ---------------------------------------------------------------- sparse sweep
 
-// Flags of the current (0), previous (1) and next (2) state.
-static uint8_t* sp_flag(cah_state* s, int which) { return s->sp_flags[(s->sp_cur + which) % 3]; }
+static SparseArgs sparse_args(cah_state* s) {
+    SparseArgs sa{};
+    const int32_t t = s->sp_step;
+    sa.stamp = s->sp_stamp;
+    sa.edge = s->sp_edgef;
+    sa.list = s->sp_cand[t % 2];
+    sa.count = s->sp_cnt + t % 3;
+    sa.list_next = s->sp_cand[(t + 1) % 2];
+    sa.count_next = s->sp_cnt + (t + 1) % 3;
+    sa.count_clear = s->sp_cnt + (t + 2) % 3;
+    sa.nactive = s->sp_cnt + 3 + (t + 1) % 2;
+    sa.nactive_clear = s->sp_cnt + 3 + t % 2;
+    sa.step = t;
+    sa.ntx = s->sp_ntx;
+    sa.nty = s->sp_nty;
+    sa.reach1 = s->sp_rx;
+    sa.reach2 = s->sp_ry;
+    sa.per1 = s->d.boundary[0] == CAH_BOUNDARY_PERIODIC;
+    sa.per2 = s->d.ndims == 2 && s->d.boundary[1] == CAH_BOUNDARY_PERIODIC;
+    return sa;
+}
 
-// After the state is set or the sweep is switched on: the other buffer all
-// background, the current flags from the current buffer, no previous flags.
+// After the state is set, a cell written or the sweep switched on: the other
+// buffer all background, the non-background tiles found by a scan, fresh
+// stamps, and the first candidate list from them and the edge tiles.
 static int sparse_refresh(cah_state* s) {
-    const size_t nt = (size_t)s->sp_ntx * (size_t)s->sp_nty;
+    const int32_t nt = s->sp_ntx * s->sp_nty;
     HIPCHK(hipMemsetAsync(s->buf[1 - s->cur], (int)(uint8_t)s->sp_bg, (size_t)s->pitch * (size_t)s->n2, s->stream));
-    for (uint8_t* f : s->sp_flags) HIPCHK(hipMemsetAsync(f, 0, nt, s->stream));
-    HIPCHK(hipMemsetAsync(s->sp_count, 0, 2 * sizeof(int32_t), s->stream));
-    s->sp_par = 0;
+    HIPCHK(hipMemsetAsync(s->sp_tmp, 0, (size_t)nt, s->stream));
+    HIPCHK(hipMemsetAsync(s->sp_cnt, 0, 6 * sizeof(int32_t), s->stream));
+    HIPCHK(hipMemsetAsync(s->sp_stamp, 0xFF, (size_t)nt * sizeof(int32_t), s->stream));  // -1
     dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
     k_sparse_flags<<<grid, 256, 0, s->stream>>>((const int8_t*)s->buf[s->cur], lat_of(s), s->sp_ts, s->sp_tsy,
-                                                s->sp_ntx, s->sp_bg, sp_flag(s, 0));
+                                                s->sp_ntx, s->sp_bg, s->sp_tmp);
+    k_sparse_compact<<<(unsigned)((nt + 255) / 256), 256, 0, s->stream>>>(s->sp_tmp, nt, s->sp_act, s->sp_cnt + 5);
+    s->sp_step = 0;
+    // step 0 reads candidate list 0 with counter 0; the state's non-background
+    // tiles are counted in active counter 0
+    SparseArgs sa = sparse_args(s);
+    sa.list_next = s->sp_cand[0];
+    sa.count_next = s->sp_cnt + 0;
+    k_sparse_mark<<<(unsigned)(2 * device_cus(s->device)), 256, 0, s->stream>>>(sa, s->sp_act, s->sp_cnt + 5,
+                                                                               s->sp_edges, s->sp_nedge);
+    HIPCHK(hipMemcpyAsync(s->sp_cnt + 3, s->sp_cnt + 5, sizeof(int32_t), hipMemcpyDeviceToDevice, s->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
+    s->sp_dirty = false;
     return CAH_OK;
 }
 
@@ -1120,16 +1153,18 @@ extern "C" int cah_state_sparse(cah_state* s, int8_t background, int32_t tilesiz
         return fail(CAH_ERR_ARGUMENT,
                     "SparseBackend needs a quiescent rule: a background cell (%d) with background neighbours "
                     "changes", (int)background);
+    const bool two = d.ndims == 2;
+    const int64_t ntx = (s->n1 + tilesize - 1) / tilesize, nty = two ? (s->n2 + tilesize - 1) / tilesize : s->n2;
+    if (ntx * nty >= (int64_t)1 << 31) return fail(CAH_ERR_ARGUMENT, "too many tiles; use a larger tilesize");
     HIPCHK(hipSetDevice(s->device));
     HIPCHK(hipStreamSynchronize(s->stream));
-    for (void* q : {(void*)s->sp_flags[0], (void*)s->sp_flags[1], (void*)s->sp_flags[2], (void*)s->sp_edge,
-                    (void*)s->sp_list, (void*)s->sp_count})
+    for (void* q : {(void*)s->sp_cand[0], (void*)s->sp_cand[1], (void*)s->sp_cnt, (void*)s->sp_stamp,
+                    (void*)s->sp_edgef, (void*)s->sp_edges, (void*)s->sp_tmp, (void*)s->sp_act})
         dev_free(s, q);
-    const bool two = d.ndims == 2;
     s->sp_ts = tilesize;
     s->sp_tsy = two ? tilesize : 1;
-    s->sp_ntx = (int32_t)((s->n1 + s->sp_ts - 1) / s->sp_ts);
-    s->sp_nty = (int32_t)((s->n2 + s->sp_tsy - 1) / s->sp_tsy);
+    s->sp_ntx = (int32_t)ntx;
+    s->sp_nty = (int32_t)nty;
     const int h = s->halo;
     // tiles within reach of a cell's neighbourhood; one more on a periodic axis
     // whose last tile is partial (a wrapped halo can reach past it)
@@ -1153,14 +1188,20 @@ extern "C" int cah_state_sparse(cah_state* s, int8_t background, int32_t tilesiz
                 for (int64_t tx = 0; tx < s->sp_ntx; ++tx) edge[(size_t)(tc * s->sp_ntx + tx)] = 1;
         }
     }
-    for (uint8_t*& f : s->sp_flags)
-        if (int rc = dev_alloc(s, (void**)&f, nt)) return rc;
-    if (int rc = upload(s, &s->sp_edge, edge.data(), nt)) return rc;
-    if (int rc = dev_alloc(s, (void**)&s->sp_list, nt * sizeof(int32_t))) return rc;
-    if (int rc = dev_alloc(s, (void**)&s->sp_count, 2 * sizeof(int32_t))) return rc;
+    std::vector<int32_t> edges;
+    for (size_t k = 0; k < nt; ++k)
+        if (edge[k]) edges.push_back((int32_t)k);
+    s->sp_nedge = (int32_t)edges.size();
+    for (int32_t*& l : s->sp_cand)
+        if (int rc = dev_alloc(s, (void**)&l, nt * sizeof(int32_t))) return rc;
+    if (int rc = dev_alloc(s, (void**)&s->sp_cnt, 6 * sizeof(int32_t))) return rc;
+    if (int rc = dev_alloc(s, (void**)&s->sp_stamp, nt * sizeof(int32_t))) return rc;
+    if (int rc = upload(s, &s->sp_edgef, edge.data(), nt)) return rc;
+    if (int rc = upload(s, &s->sp_edges, edges.data(), edges.size())) return rc;
+    if (int rc = dev_alloc(s, (void**)&s->sp_tmp, nt)) return rc;
+    if (int rc = dev_alloc(s, (void**)&s->sp_act, nt * sizeof(int32_t))) return rc;
     s->sparse = true;
     s->sp_bg = background;
-    s->sp_cur = 0;
     return sparse_refresh(s);
 }
 
@@ -1168,42 +1209,27 @@ extern "C" int cah_state_active_tiles(cah_state* s, int64_t* n) {
     if (!s || !n) return fail(CAH_ERR_ARGUMENT, "NULL argument");
     if (!s->sparse) return fail(CAH_ERR_ARGUMENT, "the state does not run the sparse sweep");
     HIPCHK(hipSetDevice(s->device));
-    std::vector<uint8_t> f((size_t)s->sp_ntx * (size_t)s->sp_nty);
-    HIPCHK(hipMemcpyAsync(f.data(), sp_flag(s, 0), f.size(), hipMemcpyDeviceToHost, s->stream));
+    if (s->sp_dirty)
+        if (int rc = sparse_refresh(s)) return rc;
+    int32_t c = 0;
+    HIPCHK(hipMemcpyAsync(&c, s->sp_cnt + 3 + s->sp_step % 2, sizeof c, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    int64_t c = 0;
-    for (uint8_t v : f) c += v != 0;
     *n = c;
     return CAH_OK;
 }
 
-// One sparse step: list the candidate tiles, sweep them, rotate the flags.
+// One sparse step: sweep the listed tiles, which lists the next step's.
 static int sparse_step(cah_state* s, int64_t t, const cah_params* p) {
-    SparseArgs sa{};
-    sa.act_cur = sp_flag(s, 0);
-    sa.act_prev = sp_flag(s, 1);
-    sa.act_next = sp_flag(s, 2);
-    sa.edge = s->sp_edge;
-    sa.list = s->sp_list;
-    sa.count = s->sp_count + s->sp_par;
-    sa.count_next = s->sp_count + (1 - s->sp_par);
-    sa.ntx = s->sp_ntx;
-    sa.nty = s->sp_nty;
-    sa.reach1 = s->sp_rx;
-    sa.reach2 = s->sp_ry;
-    sa.per1 = s->d.boundary[0] == CAH_BOUNDARY_PERIODIC;
-    sa.per2 = s->d.ndims == 2 && s->d.boundary[1] == CAH_BOUNDARY_PERIODIC;
-    const int64_t nt = (int64_t)s->sp_ntx * s->sp_nty;
-    k_sparse_mark<<<(unsigned)((nt + 255) / 256), 256, 0, s->stream>>>(sa);
+    if (s->sp_dirty || s->sp_step >= INT32_MAX - 1)
+        if (int rc = sparse_refresh(s)) return rc;
+    const SparseArgs sa = sparse_args(s);
     const GenArgs a = generic_args(s, t, 0, p);
     const size_t lds = (size_t)(s->sp_ts + 2 * a.h) * (size_t)(s->sp_tsy + 2 * a.h);
-    // persistent blocks walk the list (4 a CU: enough for a dense front, few idle
-    // ones for a small one)
-    const int blocks = (int)std::min<int64_t>(nt, 4 * device_cus(s->device));
+    // persistent blocks: the number of listed tiles is known only on the device
+    const int blocks = (int)std::min<int64_t>((int64_t)s->sp_ntx * s->sp_nty, 4 * device_cus(s->device));
     k_sparse_sweep<<<blocks, 256, lds, s->stream>>>(a, sa, s->sp_ts, s->sp_tsy, s->sp_bg);
     HIPCHK(hipGetLastError());
-    s->sp_par = 1 - s->sp_par;
-    s->sp_cur = (s->sp_cur + 2) % 3;  // next -> current, current -> previous
+    ++s->sp_step;
     s->cur = 1 - s->cur;
     return CAH_OK;
 }

@@ -131,48 +131,60 @@ __global__ __launch_bounds__(256) void k_generic_i8(GenArgs a) {
 
 // ---------------------------------------------------------------- the sparse sweep
 //
-// The SparseBackend's active-tile step (src/sparse.jl:126-211) on a dense lattice:
-// tiles of ts x tsy cells (tsy = ts in 2-D, 1 in 1-D) carry a flag "may hold a
-// cell that is not the background".  k_sparse_mark lists the tiles within
-// `reach` tiles of a flagged one, the tiles flagged one step earlier (their cells
-// in the buffer being written still hold that step's values) and the tiles on a
-// Constant edge whose value is not the background; k_sparse_sweep updates exactly
-// those tiles and flags the ones that keep a non-background cell.  Every other
-// tile is background in both buffers, so the result is the dense sweep's.
+// The SparseBackend's active-tile step (src/sparse.jl:126-211) on a dense lattice,
+// with work proportional to the active area and one launch per step.  Tiles are
+// ts x tsy cells (tsy = ts in 2-D, 1 in 1-D).  A step sweeps the candidate tiles of
+// a list: the tiles within reach of a tile holding a non-background cell, the
+// tiles that held one at the previous step (their cells in the buffer being
+// written still hold that step's values) and the tiles within reach of a
+// Constant edge that is not the background.  While it sweeps, it builds the next
+// step's list: a tile that ends with a non-background cell claims its reach
+// window, and a tile that began with one, or is an edge tile, claims itself.  A
+// claim is an atomicMax of the step number on the tile's stamp, so each tile is
+// listed once per step with nothing to clear.  Every other tile is background in
+// both buffers, so the result is the dense sweep's.  k_sparse_mark makes the
+// first list after the state is set (from the list of non-background tiles).
 
 struct SparseArgs {
-    const uint8_t* act_cur;   // flags of the state being read
-    const uint8_t* act_prev;  // flags of the state one step earlier
-    uint8_t* act_next;        // flags of the state being written
-    const uint8_t* edge;      // always-swept tiles
-    int32_t* list;            // candidate tiles
-    int32_t* count;           // their number (this step's counter)
-    int32_t* count_next;      // the next step's counter, cleared by this step's sweep
+    int32_t* stamp;            // per tile: the last step that listed it
+    const uint8_t* edge;       // per tile: always swept
+    const int32_t* list;       // this step's candidates
+    const int32_t* count;      //   ... their number
+    int32_t* list_next;        // the next step's candidates
+    int32_t* count_next;       //   ... their number (cleared before this step)
+    int32_t* count_clear;      // the counter the step after next claims into: cleared here
+    int32_t* nactive;          // tiles of the written state holding a non-background cell
+    int32_t* nactive_clear;    // the other such counter: cleared here
+    int32_t step;              // this step's number (the next list is stamped step + 1)
     int32_t ntx, nty, reach1, reach2;  // tiles a neighbourhood reaches, per axis
     int32_t per1, per2;                // periodic tile axes
 };
 
-__global__ __launch_bounds__(256) void k_sparse_mark(SparseArgs s) {
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= s.ntx * s.nty) return;
-    s.act_next[k] = 0;
-    const int32_t tx = k % s.ntx, ty = k / s.ntx;
-    bool cand = s.act_prev[k] || s.edge[k];
-    for (int32_t dy = -s.reach2; dy <= s.reach2 && !cand; ++dy) {
-        int32_t y = ty + dy;
-        if (s.per2) y = (y % s.nty + s.nty) % s.nty;
-        if (y < 0 || y >= s.nty) continue;
-        for (int32_t dx = -s.reach1; dx <= s.reach1; ++dx) {
-            int32_t x = tx + dx;
-            if (s.per1) x = (x % s.ntx + s.ntx) % s.ntx;
-            if (x < 0 || x >= s.ntx) continue;
-            if (s.act_cur[(int64_t)y * s.ntx + x]) {
-                cand = true;
-                break;
-            }
-        }
+CAH_DEV void sparse_claim(int32_t* stamp, int32_t step, int32_t* list, int32_t* count, int32_t k) {
+    if (atomicMax(stamp + k, step) < step) list[atomicAdd(count, 1)] = k;
+}
+
+// Tile d (0 <= d < window) of the reach window around tile k, or -1 off the lattice.
+CAH_DEV int32_t sparse_window(const SparseArgs& s, int32_t k, int32_t d) {
+    const int32_t wx = 2 * s.reach1 + 1;
+    int32_t x = k % s.ntx + d % wx - s.reach1, y = k / s.ntx + d / wx - s.reach2;
+    if (s.per1) x = (x % s.ntx + s.ntx) % s.ntx;
+    if (s.per2) y = (y % s.nty + s.nty) % s.nty;
+    if (x < 0 || x >= s.ntx || y < 0 || y >= s.nty) return -1;
+    return y * s.ntx + x;
+}
+
+// The first list (stamp `step`) from the non-background tiles `act` and the edge
+// tiles `edges`.
+__global__ __launch_bounds__(256) void k_sparse_mark(SparseArgs s, const int32_t* act, const int32_t* nact,
+                                                     const int32_t* edges, int32_t nedge) {
+    const int32_t win = (2 * s.reach1 + 1) * (2 * s.reach2 + 1);
+    const int64_t a = (int64_t)*nact * win, total = a + nedge;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t k = i < a ? sparse_window(s, act[i / win], (int32_t)(i % win)) : edges[i - a];
+        if (k >= 0) sparse_claim(s.stamp, s.step, s.list_next, s.count_next, k);
     }
-    if (cand) s.list[atomicAdd(s.count, 1)] = k;
 }
 
 // Blocks of 256 threads walk the candidate list; a tile's cells and their halo
@@ -180,43 +192,64 @@ __global__ __launch_bounds__(256) void k_sparse_mark(SparseArgs s) {
 __global__ __launch_bounds__(256) void k_sparse_sweep(GenArgs a, SparseArgs s, int32_t ts, int32_t tsy,
                                                       int8_t bg) {
     extern __shared__ int8_t stile[];
-    __shared__ int any;
+    __shared__ int in_any, out_any;
     const int W = ts + 2 * a.h, H = tsy + 2 * a.h;
     const int32_t n = *s.count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *s.count_next = 0;  // nobody reads it until the next mark
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // nobody uses these two during this step
+        *s.count_clear = 0;
+        *s.nactive_clear = 0;
+    }
+    const int32_t win = (2 * s.reach1 + 1) * (2 * s.reach2 + 1);
     for (int32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const int32_t k = s.list[i];
         const int x0 = (k % s.ntx) * ts, y0 = (k / s.ntx) * tsy;
-        if (threadIdx.x == 0) any = 0;
+        if (threadIdx.x == 0) in_any = out_any = 0;
         for (int j = threadIdx.x; j < W * H; j += blockDim.x) {
             const int lx = j % W, ly = j / W;
             stile[j] = read_mapped<int8_t>(a.cur, a.L, (int8_t)a.cv1, (int8_t)a.cv2, x0 - a.h + lx, y0 - a.h + ly);
         }
         __syncthreads();
-        int mine = 0;
+        int inm = 0, outm = 0;
         for (int j = threadIdx.x; j < ts * tsy; j += blockDim.x) {
             const int lx = j % ts, ly = j / ts;
             const int x = x0 + lx, y = y0 + ly;
             if (x >= a.L.n1 || y >= a.L.n2) continue;
-            const int out = generic_cell(a, stile + (lx + a.h) + (ly + a.h) * W, W, x, y);
+            const int8_t* cp = stile + (lx + a.h) + (ly + a.h) * W;
+            const int out = generic_cell(a, cp, W, x, y);
             a.next[(int64_t)y * a.L.pitch + x] = (int8_t)out;
-            mine |= out != bg;
+            inm |= *cp != bg;
+            outm |= out != bg;
         }
-        if (mine) any = 1;
+        if (inm) in_any = 1;
+        if (outm) out_any = 1;
         __syncthreads();
-        if (threadIdx.x == 0 && any) s.act_next[k] = 1;
-        __syncthreads();  // the tile and `any` are reused by the next candidate
+        if (out_any) {  // the written tile holds a non-background cell: list its reach window
+            if (threadIdx.x == 0) atomicAdd(s.nactive, 1);
+            for (int d = threadIdx.x; d < win; d += blockDim.x) {
+                const int32_t kk = sparse_window(s, k, d);
+                if (kk >= 0) sparse_claim(s.stamp, s.step + 1, s.list_next, s.count_next, kk);
+            }
+        } else if (threadIdx.x == 0 && (in_any || s.edge[k])) {
+            sparse_claim(s.stamp, s.step + 1, s.list_next, s.count_next, k);
+        }
+        __syncthreads();  // the tile and the flags are reused by the next candidate
     }
 }
 
-// flags[k] = 1 when tile k holds a cell that is not the background (after a
-// state is set).
+// flags[k] = 1 when tile k holds a cell that is not the background.
 __global__ __launch_bounds__(256) void k_sparse_flags(const int8_t* u, Lat L, int32_t ts, int32_t tsy,
                                                       int32_t ntx, int8_t bg, uint8_t* flags) {
     const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L.n1) return;
     for (int32_t y = blockIdx.y; y < L.n2; y += gridDim.y)
         if (u[(int64_t)y * L.pitch + x] != bg) flags[(int64_t)(y / tsy) * ntx + x / ts] = 1;
+}
+
+// The flagged tiles, appended to a list (after a state is set).
+__global__ __launch_bounds__(256) void k_sparse_compact(const uint8_t* flags, int32_t nt, int32_t* list,
+                                                        int32_t* n) {
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nt && flags[k]) list[atomicAdd(n, 1)] = k;
 }
 
 // ---------------------------------------------------------------- synthetic inputs
