@@ -99,12 +99,11 @@ LIFE_SWAR = [
 ]
 
 
-@pytest.mark.parametrize("i", range(len(LIFE_SWAR)))
-@pytest.mark.parametrize("vb", [0, 1, 2, 4])
+# every launch shape the band walk has for each case (radius 5 needs two dwords a lane)
+@pytest.mark.parametrize("i,vb", [(i, vb) for i in range(len(LIFE_SWAR)) for vb in (0, 1, 2, 4)
+                                  if not (LIFE_SWAR[i][0] == 5 and vb == 1)])
 def test_life_band_walk(oracle, i, vb):
     r, text, dims, bounds = LIFE_SWAR[i]
-    if r == 5 and vb == 1:
-        pytest.skip("radius 5 needs two dwords per lane")
     rule = ltl_rule() if text is None else M.Life(text, neighbourhood=M.Moore(r))
     space = M.Lattice(dims, bounds)
     u = rng_state(i, dims, [0, 1])
