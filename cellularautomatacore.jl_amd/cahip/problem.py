@@ -102,8 +102,15 @@ def final_state(sol):
 
 
 def _save_times(t0, t1, saveat, save_everystep):
+    """The saved times (src/problem.jl:335-363): t0, then every `saveat` steps or
+    the times of a collection within (t0, t1], and t1."""
     if saveat is not None:
-        ts = list(range(t0, t1 + 1, int(saveat)))
+        if np.ndim(saveat) == 0:
+            if int(saveat) <= 0:
+                raise M.ArgumentError(f"saveat must be positive, got {saveat}")
+            ts = list(range(t0, t1 + 1, int(saveat)))
+        else:
+            ts = [t0] + sorted({int(x) for x in saveat if t0 < x <= t1})
         if ts[-1] != t1:
             ts.append(t1)
         return ts

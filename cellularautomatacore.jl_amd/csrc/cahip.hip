@@ -27,6 +27,7 @@
 #include "host_copy.hpp"
 #include "kernels_counts.hpp"
 #include "kernels_epidemic.hpp"
+#include "kernels_life_tb.hpp"
 #include "kernels_generic.hpp"
 #include "launch.hpp"
 #include "kernels_lenia.hpp"
@@ -983,6 +984,31 @@ static int step_life(cah_state* s) {
     return CAH_OK;
 }
 
+// `steps` (1..TB_K) steps of a Moore(1) Life rule in one launch (k_life_tb): tiles
+// sized so that small lattices still spread over the device.
+static int step_life_tb(cah_state* s, int steps) {
+    LifeTbArgs a{};
+    a.cur = (const int8_t*)s->buf[s->cur];
+    a.next = (int8_t*)s->buf[1 - s->cur];
+    a.L = lat_of(s);
+    a.cv1 = (int8_t)s->d.boundary_value[0];
+    a.cv2 = (int8_t)s->d.boundary_value[1];
+    a.birth = (uint32_t)(s->d.birth & 0x1FFu);
+    a.survive = (uint32_t)(s->d.survive & 0x1FFu);
+    a.steps = steps;
+    const int64_t cells = s->n1 * s->n2;
+    auto launch = [&](auto kernel, int txw, int ty) {
+        dim3 grid((unsigned)((s->n1 + 32 * txw - 1) / (32 * txw)), (unsigned)((s->n2 + ty - 1) / ty));
+        kernel<<<grid, 256, 0, s->stream>>>(a);
+    };
+    if (cells <= ((int64_t)1 << 20)) launch(k_life_tb<2, 16, TB_K>, 2, 16);
+    else if (cells <= ((int64_t)1 << 24)) launch(k_life_tb<4, 32, TB_K>, 4, 32);
+    else launch(k_life_tb<16, 128, TB_K>, 16, 128);
+    HIPCHK(hipGetLastError());
+    s->cur = 1 - s->cur;
+    return CAH_OK;
+}
+
 static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
     ForestArgs a{};
     a.cur = (const int8_t*)s->buf[s->cur];
@@ -1352,6 +1378,18 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
             if (int rc = sparse_step(s, t0 + k, p)) return rc;
         if (s->d.rule != CAH_RULE_LIFE && nsteps > 0) s->bin_clean = false;
         return CAH_OK;
+    }
+    // Moore(1) Life: up to TB_K steps a launch in LDS, one bit a cell (the band walk
+    // takes a last single step; a launch shape set with cah_state_tune keeps it)
+    if (s->path == P_LIFE_SWAR && s->life_R == 1 && !s->tune_vb && !s->tune_rows) {
+        int64_t left = nsteps;
+        while (left >= 2) {
+            const int k = (int)std::min<int64_t>(left, TB_K);
+            if (int rc = step_life_tb(s, k)) return rc;
+            left -= k;
+        }
+        nsteps = left;
+        s->bin_clean = true;  // Life writes 0 and 1 only
     }
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t t = t0 + k;

@@ -98,10 +98,12 @@ CAH_DEV int32_t map_axis0(int32_t kind, int32_t g, int32_t n, bool& out) {
     out = false;
     if (g >= 0 && g < n) return g;
     if (kind == CAH_BOUNDARY_PERIODIC) {
+        if (g >= -n && g < 2 * n) return g < 0 ? g + n : g - n;  // within one period: no division
         int32_t m = g % n;
         return m < 0 ? m + n : m;
     }
     if (kind == CAH_BOUNDARY_REFLECTING) {
+        if (g >= -n && g < 2 * n) return g < 0 ? -1 - g : 2 * n - 1 - g;
         const int32_t n2 = 2 * n;
         int32_t m = g % n2;
         if (m < 0) m += n2;
