@@ -207,7 +207,8 @@ def bench_life512(be, ev, steps=1000):
     return {"workload": "Conway Life Moore(1) Torus 512^2 Bernoulli(0.5) seed 1, 1000 steps",
             "us_per_step": round(ms * 1e3 / steps, 3),
             "gcups": round(cells * steps / (ms * 1e-3) / 1e9, 2),
-            "bound": "launch latency (0.5 MiB working set stays in L2)"}
+            "kernel": "k_life_tb: 16 steps a launch in LDS, one bit a cell",
+            "bound": "launch latency and per-step barriers (0.5 MiB working set stays in L2)"}
 
 
 def batch_sides(n=256, seed=2026):
