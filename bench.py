@@ -25,6 +25,10 @@ sys.path.insert(0, os.path.join(ROOT, "cellularautomatacore.jl_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, in T lane-instructions/s
+VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
+# k_forest_vn1<2, true, 0> (gfx950 ISA): 342 VALU instructions per row of 8 draws in the main loop
+FOREST_VALU_PER_DRAW = 342 / 8
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_ltl8192_forest16384.json")
 COPY_N = 16384 * 16384  # bytes of the calibration copy (one 16384^2 Int8 lattice)
 KERNELS = {"ltl": "k_life_swar<5, 2, true, 1>", "forest": "k_forest_vn1<2, true, 0>",
@@ -662,6 +666,7 @@ def main():
     # dominant kernel: the forest fire sweep (one launch per step); algorithmic
     # bytes = one Int8 read and one Int8 write per cell
     achieved = ALGO_BYTES["forest"] / (ms_for * 1e-3) / 1e9
+    valu_t = c_for * FOREST_VALU_PER_DRAW / (ms_for * 1e-3) / 1e12
     achieved_ltl = ALGO_BYTES["ltl"] / (ms_ltl * 1e-3) / 1e9
     traffic = {}
     if os.path.exists(TRAFFIC_JSON):
@@ -706,8 +711,11 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("forest", {}).get("bytes_per_launch"),
                      "copy_gbs": round(cp_for, 1), "frac_of_copy": round(achieved / cp_for, 4),
+                     "valu": {"instructions_per_draw": FOREST_VALU_PER_DRAW,
+                              "achieved_t_lane_instr_s": round(valu_t, 2), "peak": round(VALU_PEAK_T, 2),
+                              "frac": round(valu_t / VALU_PEAK_T, 4)},
                      "note": "one Philox4x32-10 draw per cell (17 v_mad_u64_u32 / v_mul_*_u32 "
-                             "and 18 v_bitop3_b32): bound by the VALU, not by HBM"},
+                             "and 17-18 v_bitop3_b32): bound by the VALU, not by HBM"},
         "roofline_ltl": {"bound": "hbm", "kernel": KERNELS["ltl"], "achieved": round(achieved_ltl, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved_ltl / HBM_PEAK_GBS, 4),

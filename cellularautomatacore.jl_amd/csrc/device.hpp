@@ -76,6 +76,13 @@ CAH_DEV uint64_t thr53(double p) {
     return (uint64_t)ceil(P);
 }
 
+// x < T for the 53-bit draw x = (r0 << 32 | r1) >> 11 is r0:r1 < T * 2^11, so the
+// high word alone decides it unless r0 equals hi53(T) = floor(T * 2^11 / 2^32):
+// r0 < hi53(T) is a hit, r0 > hi53(T) a miss, and a tie (probability 2^-32) needs
+// r1.  T = 2^53 (always) maps to 0xFFFFFFFF, so that every r0 but 0xFFFFFFFF is
+// a hit and that one is a tie.
+CAH_DEV uint32_t hi53(uint64_t T) { return T >= (1ull << 53) ? 0xFFFFFFFFu : (uint32_t)(T >> 21); }
+
 // The same for the Float32 draw u = x24 * 2^-24 (_uniform32) against a Float32 p.
 CAH_DEV uint32_t thr24(float p) {
     const float P = p * 16777216.0f;

@@ -24,10 +24,11 @@ struct ForestArgs {
 // The cell is rowid + J, and k (1 or 2) comes in as c3.  With fewer than 2^32
 // cells (SMALL) the cell id is a 32-bit word without carries, so the first
 // round's product M0 * (rowid + J) is one multiply-add, M0 * rowid + M0 * J.
-template <bool SMALL, int PREC, int J>
-CAH_DEV bool forest_draw(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, uint64_t tp53,
-                         uint64_t tf53, uint32_t tp24, uint32_t tf24) {
-    const bool empty = c3 == 1u;
+// With both p and f in FP64 the walk tests the draw's high word against hi53
+// (device.hpp) and repeats a lane's draws through forest_draw only on a tie, so
+// the last round forms one word and the test is one 32-bit compare.
+template <bool SMALL, int J>
+CAH_DEV void forest_philox(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, uint32_t& r0, uint32_t& r1) {
     uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     uint32_t c0, c1, c2 = (uint32_t)(uint64_t)t;
     if constexpr (SMALL) {
@@ -42,6 +43,16 @@ CAH_DEV bool forest_draw(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, 
         c1 = (uint32_t)(cellid >> 32);
         philox10(c0, c1, c2, c3, k0, k1);
     }
+    r0 = c0;
+    r1 = c1;
+}
+
+template <bool SMALL, int PREC, int J>
+CAH_DEV bool forest_draw(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, uint64_t tp53,
+                         uint64_t tf53, uint32_t tp24, uint32_t tf24) {
+    const bool empty = c3 == 1u;
+    uint32_t c0, c1;
+    forest_philox<SMALL, J>(c3, rowid, t, seed, c0, c1);
     if constexpr (PREC == 0) {
         return draw53(c0, c1) < (empty ? tp53 : tf53);
     } else if constexpr (PREC == 3) {
@@ -80,6 +91,7 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
     };
     const uint64_t tp53 = thr53(a.p), tf53 = thr53(a.f);
     const uint32_t tp24 = thr24((float)a.p), tf24 = thr24((float)a.f);
+    const uint32_t hp53 = hi53(tp53), hf53 = hi53(tf53);
 
     uint32_t raw[S][VB], fire[S][VB];
     load(ys - 1, raw[0]);
@@ -105,23 +117,51 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
         E[VB + 1] = from_next_lane(fire[s_c][0]);
 #pragma unroll
         for (int i = 0; i < VB; ++i) E[1 + i] = fire[s_c][i];
-        uint32_t out[VB];
+        uint32_t out[VB], e4[VB], hit4[VB];
         const uint64_t rowid = (uint64_t)y * (uint64_t)a.L.n1 + (uint64_t)x + 1;
+        bool tie = false;
+        Unroll<0, VB>::run([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            e4[i] = bytes_eq(raw[s_c][i], 0u);
+            const uint32_t k4 = 0x02020202u - e4[i];  // the draw's k: 1 empty, 2 otherwise
+            hit4[i] = 0;
+            Unroll<0, 4>::run([&](auto B) {
+                constexpr int b = decltype(B)::value;
+                const uint32_t k = (k4 >> (8 * b)) & 0xFFu;
+                bool hit;
+                if constexpr (PREC == 0) {  // the high word decides all but a tie
+                    uint32_t r0, r1;
+                    forest_philox<SMALL, 4 * i + b>(k, rowid, a.t, a.seed, r0, r1);
+                    const uint32_t lim = k == 1u ? hp53 : hf53;
+                    hit = r0 < lim;
+                    tie |= r0 == lim;
+                } else {
+                    hit = forest_draw<SMALL, PREC, 4 * i + b>(k, rowid, a.t, a.seed, tp53, tf53, tp24, tf24);
+                }
+                hit4[i] |= (hit ? 1u : 0u) << (8 * b);
+            });
+        });
+        if constexpr (PREC == 0) {
+            if (tie) {  // about one lane in 2^29 rows: every draw of the lane again, in full
+                Unroll<0, VB>::run([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    const uint32_t k4 = 0x02020202u - e4[i];
+                    hit4[i] = 0;
+                    Unroll<0, 4>::run([&](auto B) {
+                        constexpr int b = decltype(B)::value;
+                        const bool hit = forest_draw<SMALL, 0, 4 * i + b>((k4 >> (8 * b)) & 0xFFu, rowid, a.t,
+                                                                          a.seed, tp53, tf53, tp24, tf24);
+                        hit4[i] |= (hit ? 1u : 0u) << (8 * b);
+                    });
+                });
+            }
+        }
         Unroll<0, VB>::run([&](auto I) {
             constexpr int i = decltype(I)::value;
             // VonNeumann(1): left, right, up, down
             const uint32_t nf = funnel(E[1 + i], E[i], 24) | funnel(E[2 + i], E[1 + i], 8) |
                                 fire[s_up][i] | fire[s_dn][i];
-            const uint32_t e4 = bytes_eq(raw[s_c][i], 0u);
-            const uint32_t k4 = 0x02020202u - e4;  // the draw's k: 1 empty, 2 otherwise
-            uint32_t hit4 = 0;
-            Unroll<0, 4>::run([&](auto B) {
-                constexpr int b = decltype(B)::value;
-                const bool hit = forest_draw<SMALL, PREC, 4 * i + b>(
-                    (k4 >> (8 * b)) & 0xFFu, rowid, a.t, a.seed, tp53, tf53, tp24, tf24);
-                hit4 |= (hit ? 1u : 0u) << (8 * b);
-            });
-            out[i] = forest_bytes(e4, fire[s_c][i], nf, hit4);
+            out[i] = forest_bytes(e4[i], fire[s_c][i], nf, hit4[i]);
         });
         if (writer) {
             if constexpr (FAST) store_mapped<VB>(rnext, a.L, x, y, out);

@@ -171,6 +171,36 @@ def test_forest_2048_from_empty(oracle, backend):
     assert (sol.u[-1] == 2).any()  # fires have started
 
 
+@pytest.mark.parametrize("which", ["growth", "lightning"])
+@pytest.mark.parametrize("hit", [True, False])
+def test_forest_draw_high_word_ties(oracle, which, hit):
+    """The band walk decides u < p from the draw's high word r0 and falls back to
+    the full 53-bit test when r0 equals the high word of p * 2^53 * 2^11.  Here p
+    (or f) is set so that one chosen cell's draw ties: p = (r0 * 2^21 + d) / 2^53
+    is exact in FP64, and the cell's draw x53 = r0 * 2^21 + (r1 >> 11) is below
+    it exactly when (r1 >> 11) < d."""
+    dims, seed, t = (300, 200), 7, 5
+    u = rng_state(3, dims, [0, 1, 1, 1, 2])
+    i1, i2 = 101, 57
+    want = 0 if which == "growth" else 1
+    u[i1, i2] = want
+    u[i1 - 1:i1 + 2, i2 - 1:i2 + 2][u[i1 - 1:i1 + 2, i2 - 1:i2 + 2] == 2] = 1  # no fire beside it
+    cellid = i1 + i2 * dims[0] + 1
+    k = 1 if which == "growth" else 2
+    r0, r1 = oracle.philox4x32((cellid & 0xFFFFFFFF, cellid >> 32, t, k), (seed, 0))[:2]
+    low = r1 >> 11
+    assert 0 < low < (1 << 21) - 1
+    d = low + 1 if hit else low
+    q = float(r0 * (1 << 21) + d) / 2.0 ** 53
+    assert int(q * 2.0 ** 53) == r0 * (1 << 21) + d  # exact
+    p, f = (q, 1e-4) if which == "growth" else (0.03, q)
+    prob = forest_problem(dims, p=p, f=f, seed=seed, u0=u, tspan=(t, t + 1))
+    sol = solve(prob, M.Synchronous(), backend=HIPBackend(), saveat=1)
+    ref = oracle_saves(oracle, prob, M.Synchronous(), sol.t)
+    assert ref[-1][i1, i2] == ((1 if which == "growth" else 2) if hit else want)
+    assert_same(sol, ref)
+
+
 # ------------------------------------------------------------------ Lenia
 
 def _lenia_ref_steps(oracle, prob, u, n):
