@@ -984,18 +984,58 @@ static int step_life(cah_state* s) {
     return CAH_OK;
 }
 
-// `steps` (1..TB_K) steps of a Moore(1) Life rule in one launch (k_life_tb): tiles
-// sized so that small lattices still spread over the device.
-static int step_life_tb(cah_state* s, int steps) {
+// k_life_tb over several lattices (batches, the server's Moore(1) Life family).
+// The tile shapes, TXW words x TY rows; a launch uses one for all its lattices.
+struct TbShape {
+    int txw, ty;
+};
+constexpr TbShape TB_SHAPES[3] = {{2, 16}, {4, 32}, {16, 128}};
+
+static int32_t tb_tiles(const cah_state* s, int shape) {
+    const TbShape& t = TB_SHAPES[shape];
+    return (int32_t)(((s->n1 + 32 * t.txw - 1) / (32 * t.txw)) * ((s->n2 + t.ty - 1) / t.ty));
+}
+
+// The shape with the least LDS work ((TXW + 2) x (TY + 2K) words a tile) among
+// those that give at least 1024 tiles (4 a CU), else the one with the most tiles.
+// A single lattice gets the shape step_life_tb picks for it.
+static int tb_pick_shape(const std::vector<const cah_state*>& m) {
+    int best = -1, most = 0;
+    double best_work = 0;
+    int64_t most_tiles = -1;
+    for (int sh = 0; sh < 3; ++sh) {
+        int64_t tiles = 0;
+        for (const cah_state* s : m) tiles += tb_tiles(s, sh);
+        const double work = (double)tiles * (TB_SHAPES[sh].txw + 2) * (TB_SHAPES[sh].ty + 2 * TB_K);
+        if (tiles > most_tiles) {
+            most_tiles = tiles;
+            most = sh;
+        }
+        if (tiles >= 1024 && (best < 0 || work < best_work)) {
+            best = sh;
+            best_work = work;
+        }
+    }
+    return best >= 0 ? best : most;
+}
+
+static LifeTbArgs tb_item(const cah_state* s, int cur, int32_t steps) {
     LifeTbArgs a{};
-    a.cur = (const int8_t*)s->buf[s->cur];
-    a.next = (int8_t*)s->buf[1 - s->cur];
+    a.cur = (const int8_t*)s->buf[cur];
+    a.next = (int8_t*)s->buf[1 - cur];
     a.L = lat_of(s);
     a.cv1 = (int8_t)s->d.boundary_value[0];
     a.cv2 = (int8_t)s->d.boundary_value[1];
     a.birth = (uint32_t)(s->d.birth & 0x1FFu);
     a.survive = (uint32_t)(s->d.survive & 0x1FFu);
     a.steps = steps;
+    return a;
+}
+
+// `steps` (1..TB_K) steps of a Moore(1) Life rule in one launch (k_life_tb): tiles
+// sized so that small lattices still spread over the device.
+static int step_life_tb(cah_state* s, int steps) {
+    const LifeTbArgs a = tb_item(s, s->cur, steps);
     const int64_t cells = s->n1 * s->n2;
     auto launch = [&](auto kernel, int txw, int ty) {
         dim3 grid((unsigned)((s->n1 + 32 * txw - 1) / (32 * txw)), (unsigned)((s->n2 + ty - 1) / ty));
@@ -1007,6 +1047,16 @@ static int step_life_tb(cah_state* s, int steps) {
     HIPCHK(hipGetLastError());
     s->cur = 1 - s->cur;
     return CAH_OK;
+}
+
+static void launch_life_tb_batch(int shape, int32_t nblocks, const LifeTbArgs* items, const int32_t* block_item,
+                                 const int32_t* block_base, int32_t parity, int32_t step0, hipStream_t st) {
+    switch (shape) {
+    case 0: k_life_tb_batch<2, 16, TB_K><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, step0); break;
+    case 1: k_life_tb_batch<4, 32, TB_K><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, step0); break;
+    default:
+        k_life_tb_batch<16, 128, TB_K><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, step0);
+    }
 }
 
 static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
@@ -1471,6 +1521,13 @@ struct cah_batch {
     bool fp_set = false;
     cah_params fp{};
     unsigned long long* counts_d = nullptr;  // CNT_SLOTS counters per member
+    // Moore(1) Life: k_life_tb tables (items with steps = TB_K, block -> item map)
+    bool tb = false;
+    int tb_shape = 0;
+    int32_t tb_blocks = 0;
+    LifeTbArgs* tb_items_d = nullptr;
+    int32_t* tb_item_d = nullptr;
+    int32_t* tb_base_d = nullptr;
 };
 
 static void free_batch(cah_batch* b) {
@@ -1478,7 +1535,8 @@ static void free_batch(cah_batch* b) {
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
-    for (void* p : {b->items_d, (void*)b->wave_item_d, (void*)b->wave_base_d, (void*)b->counts_d})
+    for (void* p : {b->items_d, (void*)b->wave_item_d, (void*)b->wave_base_d, (void*)b->counts_d,
+                    (void*)b->tb_items_d, (void*)b->tb_item_d, (void*)b->tb_base_d})
         if (p) hipFree(p);
     for (auto* m : b->members) free_state(m);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -1542,6 +1600,22 @@ static int batch_layout(cah_batch* b) {
     }
     if (int rc = upload(&b->wave_item_d, wave_item.data(), wave_item.size())) return rc;
     if (int rc = upload(&b->wave_base_d, wave_base.data(), wave_base.size())) return rc;
+    if (b->kind == B_LIFE && b->life_R == 1) {  // several steps a launch (k_life_tb)
+        std::vector<const cah_state*> m(b->members.begin(), b->members.end());
+        b->tb = true;
+        b->tb_shape = tb_pick_shape(m);
+        std::vector<LifeTbArgs> items;
+        std::vector<int32_t> block_item, block_base;
+        for (size_t k = 0; k < m.size(); ++k) {
+            items.push_back(tb_item(m[k], 0, TB_K));
+            block_base.push_back((int32_t)block_item.size());
+            block_item.insert(block_item.end(), (size_t)tb_tiles(m[k], b->tb_shape), (int32_t)k);
+        }
+        b->tb_blocks = (int32_t)block_item.size();
+        if (int rc = upload(&b->tb_items_d, items.data(), items.size())) return rc;
+        if (int rc = upload(&b->tb_item_d, block_item.data(), block_item.size())) return rc;
+        if (int rc = upload(&b->tb_base_d, block_base.data(), block_base.size())) return rc;
+    }
     return CAH_OK;
 }
 
@@ -1694,6 +1768,19 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
                         s->d.boundary_value[1] == 0 || s->d.boundary_value[1] == 1;
         bin &= s->bin_clean && e1 && e2;
     }
+    // Moore(1) Life: up to TB_K steps a launch (the items run TB_K - step0 steps),
+    // a last single step through the band walk, as a single lattice advances
+    auto queue = [&]() {
+        int64_t k = 0;
+        if (b->tb)
+            for (; nsteps - k >= 2; b->parity ^= 1) {
+                const int32_t n = (int32_t)std::min<int64_t>(nsteps - k, TB_K);
+                launch_life_tb_batch(b->tb_shape, b->tb_blocks, b->tb_items_d, b->tb_item_d, b->tb_base_d,
+                                     b->parity, TB_K - n, b->stream);
+                k += n;
+            }
+        for (; k < nsteps; ++k) launch_batch_step(b, t0 + k, bin);
+    };
     if (use_graph && nsteps > 1) {
         if (b->exec) {
             HIPCHK(hipStreamSynchronize(b->stream));
@@ -1702,18 +1789,19 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
         }
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, bin);
+        queue();
         HIPCHK(hipStreamEndCapture(b->stream, &graph));
         HIPCHK(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(graph));
         HIPCHK(hipGraphLaunch(b->exec, b->stream));
     } else {
-        for (int64_t k = 0; k < nsteps; ++k) launch_batch_step(b, t0 + k, bin);
+        queue();
     }
     HIPCHK(hipGetLastError());
     for (auto* s : b->members) {
         s->cur = b->parity;
         if (b->kind == B_FOREST) s->bin_clean = false;
+        if (b->kind == B_LIFE && nsteps > 0) s->bin_clean = true;  // Life writes 0 and 1 only
     }
     return CAH_OK;
 }
@@ -1788,6 +1876,7 @@ struct SrvReq {
     size_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int32_t steps = 0;                   // steps in the tick being built
+    bool tb = false;                     // its tick runs k_life_tb launches (TB_K steps each)
 };
 
 struct SrvSlot {
@@ -2011,7 +2100,9 @@ static int admit(cah_server* s, SrvReq* r) {
     switch (st->path) {
     case P_LIFE_SWAR:
         r->cls = SC_LIFE;
-        r->key = st->life_R * 4 + (life_niv(st->life) - 1) * 2 + (st->bin_clean && edges01(st) ? 1 : 0);
+        // Moore(1) runs k_life_tb, which takes any B/S masks and contents: one group
+        r->key = st->life_R == 1 ? 4
+                                 : st->life_R * 4 + (life_niv(st->life) - 1) * 2 + (st->bin_clean && edges01(st) ? 1 : 0);
         break;
     case P_FOREST:
         r->cls = SC_FOREST;
@@ -2067,6 +2158,8 @@ struct SrvGroup {
     size_t off_items = 0, off_map = 0, off_base = 0;  // in the tick's table buffer
     int64_t rows = 64;                                 // rows per wave (band walks)
     std::vector<int32_t> base;                         // first wave/block of each item, + total
+    bool tb = false;                                   // Moore(1) Life through k_life_tb
+    int shape = 0;                                     // its tile shape
 };
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -2116,10 +2209,12 @@ static int build_tick(cah_server* s) {
     // group by family and variant
     std::map<std::pair<int, int>, SrvGroup> groups;
     for (SrvReq* r : run) {
+        r->tb = false;
         if (r->cls == SC_LOOP || r->steps == 0) continue;
         SrvGroup& g = groups[{r->cls, r->key}];
         g.cls = r->cls;
         g.key = r->key;
+        g.tb = r->tb = r->cls == SC_LIFE && r->key == 4;
         g.reqs.push_back(r);
     }
     // table layout: per group items, wave (block) -> item map, item bases
@@ -2136,10 +2231,17 @@ static int build_tick(cah_server* s) {
             work += (r->st->n1 + ow - 1) / ow * r->st->n2;
         }
         g.rows = batch_rows(work);
+        if (g.tb) {
+            std::vector<const cah_state*> m;
+            for (const SrvReq* r : g.reqs) m.push_back(r->st);
+            g.shape = tb_pick_shape(m);
+        }
         for (size_t k = 0; k < g.reqs.size(); ++k) {
             const cah_state* st = g.reqs[k]->st;
             int32_t n;
-            if (g.cls == SC_LENIA) {
+            if (g.tb) {
+                n = tb_tiles(st, g.shape);
+            } else if (g.cls == SC_LENIA) {
                 n = (int32_t)(((st->n1 + LM_BX - 1) / LM_BX) * ((st->n2 + LM_BY - 1) / LM_BY));
             } else {
                 const int vb = g.cls == SC_LIFE && st->life_R == 5 ? 2 : 1;
@@ -2151,8 +2253,10 @@ static int build_tick(cah_server* s) {
             map.insert(map.end(), (size_t)n, (int32_t)k);
         }
         g.base.push_back((int32_t)map.size());
-        const size_t isz = g.cls == SC_LIFE ? sizeof(LifeArgs) : g.cls == SC_FOREST ? sizeof(ForestArgs)
-                                                                                    : sizeof(LeniaMfmaArgs);
+        const size_t isz = g.tb ? sizeof(LifeTbArgs)
+                           : g.cls == SC_LIFE ? sizeof(LifeArgs)
+                           : g.cls == SC_FOREST ? sizeof(ForestArgs)
+                                                : sizeof(LeniaMfmaArgs);
         g.off_items = off;
         off = align256(off + isz * g.reqs.size());
         g.off_map = off;
@@ -2182,7 +2286,10 @@ static int build_tick(cah_server* s) {
             const SrvReq* r = g.reqs[k];
             const cah_state* st = r->st;
             const int32_t n = g.base[k + 1] - g.base[k];
-            if (g.cls == SC_LIFE) {
+            if (g.tb) {
+                const LifeTbArgs a = tb_item(st, st->cur, r->steps);
+                std::memcpy(sl.host + g.off_items + k * sizeof a, &a, sizeof a);
+            } else if (g.cls == SC_LIFE) {
                 LifeArgs a = st->life;
                 const int vb = st->life_R == 5 ? 2 : 1;
                 a.cur = (const int8_t*)st->buf[st->cur];
@@ -2243,11 +2350,13 @@ static int build_tick(cah_server* s) {
     for (hipStream_t q : s->st) HIPCHK(hipStreamWaitEvent(q, sl.up, 0));
 
     // the batched families: step k advances the items with more than k steps
+    // (k_life_tb: the launch at step k = j * TB_K advances them by up to TB_K)
     int32_t kmax = 0;
     for (auto& kv : groups) kmax = std::max(kmax, kv.second.reqs.front()->steps);
     for (int32_t k = 0; k < kmax; ++k) {
         for (auto& kv : groups) {
             SrvGroup& g = kv.second;
+            if (g.tb && k % TB_K) continue;
             size_t na = 0;
             while (na < g.reqs.size() && g.reqs[na]->steps > k) ++na;
             const int32_t nw = g.base[na];
@@ -2255,7 +2364,10 @@ static int build_tick(cah_server* s) {
             const int32_t* map = (const int32_t*)(sl.dev + g.off_map);
             const int32_t* base = (const int32_t*)(sl.dev + g.off_base);
             const void* items = sl.dev + g.off_items;
-            if (g.cls == SC_LIFE) {
+            if (g.tb) {
+                launch_life_tb_batch(g.shape, nw, (const LifeTbArgs*)items, map, base, (k / TB_K) & 1, k,
+                                     s->st[SC_LIFE]);
+            } else if (g.cls == SC_LIFE) {
                 const int R = g.key / 4, niv = (g.key / 2) % 2 + 1;
                 launch_life_batch(R, R == 5 ? 2 : 1, g.key % 2, niv, waves_to_blocks(nw),
                                   (const LifeArgs*)items, map, base, nw, k & 1, s->st[SC_LIFE]);
@@ -2276,7 +2388,10 @@ static int build_tick(cah_server* s) {
     for (SrvReq* r : run) {
         if (!r->in_tick) sl.staged.push_back(r);
         r->in_tick = true;
-        if (r->cls != SC_LOOP) {
+        if (r->tb) {
+            r->st->cur ^= ((r->steps + TB_K - 1) / TB_K) & 1;  // one buffer swap a launch
+            r->st->bin_clean = true;                             // Life writes 0 and 1 only
+        } else if (r->cls != SC_LOOP) {
             r->st->cur ^= r->steps & 1;
             if (r->cls == SC_FOREST && r->steps > 0) r->st->bin_clean = false;
         }

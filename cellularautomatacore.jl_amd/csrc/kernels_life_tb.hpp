@@ -75,16 +75,17 @@ CAH_DEV uint32_t tb_rule(uint32_t x, uint32_t a, uint32_t b, uint32_t c, uint32_
     return out;
 }
 
-// LDS rows hold a zero word before and after the GW words of the grid, so every
-// word has a left and a right neighbour to read.
+// Tile (bx, by) of a lattice, by one block of 256 threads.  LDS rows hold a zero
+// word before and after the GW words of the grid, so every word has a left and a
+// right neighbour to read.
 template <int TXW, int TY, int K>
-__global__ __launch_bounds__(256) void k_life_tb(LifeTbArgs a) {
+CAH_DEV void life_tb_tile(const LifeTbArgs& a, int bx, int by) {
     static_assert(K >= 1 && K <= 32, "the 32-column halo covers at most 32 steps");
     constexpr int GW = TXW + 2, GH = TY + 2 * K, NW = GW * GH, RS = GW + 2;
     __shared__ uint32_t grid[2][GH * RS];
     __shared__ uint32_t outside[GW];  // per grid column: the bits beyond a Constant first-axis edge
-    const int X0 = (int)blockIdx.x * TXW * 32 - 32;  // column of bit 0 of grid word 0
-    const int Y0 = (int)blockIdx.y * TY - K;          // row of grid row 0
+    const int X0 = bx * TXW * 32 - 32;  // column of bit 0 of grid word 0
+    const int Y0 = by * TY - K;          // row of grid row 0
     const Lat& L = a.L;
     const bool const1 = L.b1 == CAH_BOUNDARY_CONSTANT, const2 = L.b2 == CAH_BOUNDARY_CONSTANT;
     if (threadIdx.x < GW) {
@@ -188,4 +189,35 @@ __global__ __launch_bounds__(256) void k_life_tb(LifeTbArgs a) {
             for (int i = 0; x + i < L.n1; ++i) q[i] = (int8_t)((bits >> i) & 1u);
         }
     }
+}
+
+template <int TXW, int TY, int K>
+__global__ __launch_bounds__(256) void k_life_tb(LifeTbArgs a) {
+    life_tb_tile<TXW, TY, K>(a, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// ---------------------------------------------------------------- batched lattices
+//
+// One launch advances several lattices: block b belongs to item block_item[b] and
+// is its tile (b - block_base[item]), tiles numbered along the first axis first.
+// Each item holds its buffers for parity 0 and its own step count; the launch
+// swaps the buffers when parity is set and runs min(K, steps - step0) steps of
+// each item (the caller launches only over items with steps > step0, listed
+// first).
+template <int TXW, int TY, int K>
+__global__ __launch_bounds__(256) void k_life_tb_batch(const LifeTbArgs* __restrict__ items,
+                                                       const int32_t* __restrict__ block_item,
+                                                       const int32_t* __restrict__ block_base, int32_t parity,
+                                                       int32_t step0) {
+    const int k = __builtin_amdgcn_readfirstlane(block_item[blockIdx.x]);
+    LifeTbArgs a = items[k];
+    if (parity) {
+        const int8_t* c = a.cur;
+        a.cur = a.next;
+        a.next = const_cast<int8_t*>(c);
+    }
+    a.steps = min(K, a.steps - step0);
+    const int t = (int)blockIdx.x - block_base[k];
+    const int ntx = (a.L.n1 + 32 * TXW - 1) / (32 * TXW);
+    life_tb_tile<TXW, TY, K>(a, t % ntx, t / ntx);
 }
