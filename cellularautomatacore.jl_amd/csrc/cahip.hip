@@ -1072,8 +1072,9 @@ static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
     a.f = p->ff_f;
     a.p_f32 = p->ff_p_is_f32;
     a.f_f32 = p->ff_f_is_f32;
-    // 4 bytes a lane below 2048 columns: twice the bands, for lattices with few waves
-    const int vb = s->tune_vb ? std::min(s->tune_vb, 2) : (s->n1 >= 2048 ? 2 : 1);
+    // 4 bytes a lane: 78 VGPRs, six waves a SIMD to hide the Philox chains' latency
+    // (8 bytes a lane takes 125 VGPRs, four waves: 2-3% slower at 16384^2 on MI355X)
+    const int vb = s->tune_vb ? std::min(s->tune_vb, 2) : 1;
     const int64_t ow = 62 * 4 * vb;
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
     a.rows = band_rows(s, a.nbands, 1);
