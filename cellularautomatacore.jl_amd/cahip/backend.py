@@ -269,12 +269,17 @@ class HIPBackend:
 
     # -------------------------------------------------------------- batches
 
-    def solve_batch(self, probs, scheme, use_graph=True, output="state", observed=None):
+    def solve_batch(self, probs, scheme, use_graph=True, output="state", observed=None, saveat=None):
         """Several problems of one Int8 rule, advanced together.  output="state":
         their final states; "counts": state_counts of each final state; "kappa":
         cohens_kappa of each final state against `observed` (a map of the
         problems' common dims).  The last two are counted on the device, so only
-        counters leave it."""
+        counters leave it.  With `saveat` (a step or a collection of times, as in
+        solve) each problem's result is the list of that output at every saved
+        time instead, t0 first: the batch advances between the saved times and
+        the summaries are counted there on the device, without copying a lattice
+        back."""
+        from .problem import _save_times
         if output not in ("state", "counts", "kappa"):
             raise M.ArgumentError(f"unknown output {output!r}")
         if output == "kappa" and observed is None:
@@ -287,42 +292,54 @@ class HIPBackend:
             d, k = make_desc(pr, scheme, self.lenia_field)
             descs.append(d)
             keeps.append(k)
+        times = [t0, t1] if saveat is None else _save_times(t0, t1, saveat, False)
         arr = (L.CahDesc * len(descs))(*descs)
         h = ctypes.c_void_p()
         check(self.lib.cah_batch_create(self.ctx(), len(descs), arr, ctypes.byref(h)))
+        obs = None
         try:
             for i, pr in enumerate(probs):
                 u = np.asfortranarray(pr.u0)
                 check(self.lib.cah_batch_set(h, i, u.ctypes.data, u.nbytes))
+            if output == "kappa":
+                obs = self.state_like(probs[0], scheme, observed)
             cp = make_params(probs[0], probs[0].p)
-            check(self.lib.cah_batch_advance(h, t0, t1 - t0, ctypes.byref(cp), int(use_graph)))
-            if output != "state":
-                return self._batch_summaries(h, probs, scheme, output, observed)
-            outs = []
-            for i, pr in enumerate(probs):
-                out = np.empty(pr.space.dims, dtype=pr.u0.dtype, order="F")
-                check(self.lib.cah_batch_get(h, i, out.ctypes.data, out.nbytes))
-                outs.append(out)
-            return outs
-        finally:
-            self.lib.cah_batch_destroy(h)
-
-    def _batch_summaries(self, h, probs, scheme, output, observed):
-        n = len(probs)
-        counts = np.zeros((n, 256), np.int64)
-        eq = np.zeros(n, np.int64)
-        obs = self.state_like(probs[0], scheme, observed) if output == "kappa" else None
-        try:
-            check(self.lib.cah_batch_counts(h, obs.handle if obs is not None else None, counts.ctypes.data,
-                                            eq.ctypes.data if obs is not None else None))
-            if output == "counts":
-                return [count_dict(c) for c in counts]
-            co = count_dict(self._counts(obs)[0])
-            cells = int(np.prod(obs.dims))
-            return [MS.kappa_from_counts(count_dict(c), co, int(e), cells) for c, e in zip(counts, eq)]
+            series, t = [], times[0]
+            for k, ts in enumerate(times):
+                if ts > t:
+                    check(self.lib.cah_batch_advance(h, t, ts - t, ctypes.byref(cp), int(use_graph)))
+                    t = ts
+                if saveat is None and k == 0:
+                    continue  # only the final result
+                if output == "state":
+                    outs = []
+                    for i, pr in enumerate(probs):
+                        out = np.empty(pr.space.dims, dtype=pr.u0.dtype, order="F")
+                        check(self.lib.cah_batch_get(h, i, out.ctypes.data, out.nbytes))
+                        outs.append(out)
+                    series.append(outs)
+                else:
+                    series.append(self._batch_summaries(h, probs, output, obs))
+            if saveat is None:
+                return series[-1]
+            return [[s[i] for s in series] for i in range(len(probs))]
         finally:
             if obs is not None:
                 self.free_state(obs)
+            self.lib.cah_batch_destroy(h)
+
+    def _batch_summaries(self, h, probs, output, obs):
+        """counts or kappa of every member of batch h; obs: the observed map's state."""
+        n = len(probs)
+        counts = np.zeros((n, 256), np.int64)
+        eq = np.zeros(n, np.int64)
+        check(self.lib.cah_batch_counts(h, obs.handle if obs is not None else None, counts.ctypes.data,
+                                        eq.ctypes.data if obs is not None else None))
+        if output == "counts":
+            return [count_dict(c) for c in counts]
+        co = count_dict(self._counts(obs)[0])
+        cells = int(np.prod(obs.dims))
+        return [MS.kappa_from_counts(count_dict(c), co, int(e), cells) for c, e in zip(counts, eq)]
 
     def __del__(self):
         if getattr(self, "_ctx", None) is not None:

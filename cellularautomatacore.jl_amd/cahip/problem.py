@@ -154,18 +154,21 @@ class EnsembleProblem:
         self.prob, self.prob_func = prob, prob_func
 
 
-def solve_ensemble(eprob, alg=None, backend=None, trajectories=1, batched=True):
+def solve_ensemble(eprob, alg=None, backend=None, trajectories=1, batched=True, saveat=None):
     """solve(eprob, alg, EnsembleThreads(); trajectories) for the final states.
 
     With `batched` the trajectories advance together through the backend's
     batched form (one launch per step for all of them); otherwise each runs its
     own plain sweep, as independent integrators would.  Returns the list of
-    final states, trajectory 1 first.
+    final states, trajectory 1 first; with `saveat`, the list of each
+    trajectory's states at the saved times (as solve's `u`).
     """
     from .backend import HIPBackend
     alg = M.Synchronous() if alg is None else alg
     backend = HIPBackend() if backend is None else backend
     probs = [eprob.prob_func(eprob.prob, i) for i in range(1, trajectories + 1)]
     if batched:
-        return backend.solve_batch(probs, alg)
+        return backend.solve_batch(probs, alg, saveat=saveat)
+    if saveat is not None:
+        return [solve(p, alg, backend=backend, saveat=saveat).u for p in probs]
     return [final_state(solve(p, alg, backend=backend, save_everystep=False)) for p in probs]

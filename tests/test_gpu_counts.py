@@ -116,6 +116,40 @@ def test_batch_counts_and_kappa():
         be.solve_batch(rag, M.Synchronous(), output="kappa", observed=fin[0])
 
 
+def test_batch_saveat_series():
+    """solve_batch with saveat: per trajectory, the states / counts / kappa at
+    every saved time equal those of the trajectory's own solve at those times
+    (counts and kappa counted on the device between the batch's advances)."""
+    be = HIPBackend()
+    dims = (300, 211)
+    probs = [forest_problem(dims, p=0.05, f=0.003, seed=s, tspan=(2, 40)) for s in range(1, 7)]
+    saveat = [5, 17, 18, 33]
+    times = [2, 5, 17, 18, 33, 40]
+    alone = [solve(p, M.Synchronous(), backend=be, saveat=saveat) for p in probs]
+    assert all(sol.t == times for sol in alone)
+    states = be.solve_batch(probs, M.Synchronous(), saveat=saveat)
+    counts = be.solve_batch(probs, M.Synchronous(), output="counts", saveat=saveat)
+    observed = alone[0].u[-1]
+    kappas = be.solve_batch(probs, M.Synchronous(), output="kappa", observed=observed, saveat=saveat)
+    for i, sol in enumerate(alone):
+        assert len(states[i]) == len(counts[i]) == len(kappas[i]) == len(times)
+        for k in range(len(times)):
+            assert np.array_equal(states[i][k], sol.u[k]), (i, times[k])
+            assert counts[i][k] == host_counts(sol.u[k]), (i, times[k])
+            assert kappas[i][k] == MS.cohens_kappa(sol.u[k], observed), (i, times[k])
+    # a step size: every 10 steps from t0, and t1
+    every = be.solve_batch(probs[:2], M.Synchronous(), output="counts", saveat=10)
+    sol = solve(probs[0], M.Synchronous(), backend=be, saveat=10)
+    assert sol.t == [2, 12, 22, 32, 40]
+    assert every[0] == [host_counts(u) for u in sol.u]
+    # the ensemble form, batched and one integrator per trajectory
+    from cahip.problem import EnsembleProblem, solve_ensemble
+    ep = EnsembleProblem(probs[0])
+    b = solve_ensemble(ep, backend=be, trajectories=4, saveat=saveat)
+    u = solve_ensemble(ep, backend=be, trajectories=4, batched=False, saveat=saveat)
+    assert all(len(x) == len(times) and all(np.array_equal(p, q) for p, q in zip(x, y)) for x, y in zip(b, u))
+
+
 def test_full_size_forest_counts(backend):
     """16384^2 forest fire after 30 steps: the counts add up to every cell and
     equal numpy's on the downloaded state."""
