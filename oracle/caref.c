@@ -10,7 +10,9 @@
  *   ref_philox4x32      Random123 known answers held by test/test_random.jl:4-14.
  *   ref_uniform(32)     closed form of src/random.jl:102-120 checked bit by bit
  *                       against an independent big-integer Python evaluation of the
- *                       same formula, plus the invariants of test/test_random.jl:18-56.
+ *                       same formula, plus the invariants of test/test_random.jl:18-56;
+ *                       the counter / key word order by the two draws the reference
+ *                       prints (vignettes/05_stochastic_rules/05_stochastic_rules.md:46-52).
  *   ref_map_axis        the vectors of test/test_spaces.jl:88-107.
  *   ref_offsets         counts and orders of test/test_spaces.jl:164-197 and the
  *                       doctest of src/neighbourhoods.jl:143-146.

@@ -171,6 +171,24 @@ def test_forest_2048_from_empty(oracle, backend):
     assert (sol.u[-1] == 2).any()  # fires have started
 
 
+@pytest.mark.parametrize("k", [1, 2])
+def test_forest_draw_matches_reference_printed_value(k):
+    """Cell 42 at t = 10 under seed 2026 draws u = 0.038246647972453474 for k = 1
+    and 0.403975113636703 for k = 2 (the values the reference prints,
+    vignettes/05_stochastic_rules/05_stochastic_rules.md:46-52).  On a 50 x 20
+    lattice cell 42 is (41, 0); p (or f) just above u must make its event
+    happen, p = u must not."""
+    u_ref = {1: 0.038246647972453474, 2: 0.403975113636703}[k]
+    dims = (50, 20)
+    for q, hit in [(float(np.nextafter(u_ref, 1.0)), True), (u_ref, False)]:
+        u0 = np.full(dims, 0 if k == 1 else 1, np.int8)  # empty (growth) or trees without fire (lightning)
+        p, f = (q, 0.0) if k == 1 else (0.0, q)
+        prob = forest_problem(dims, p=p, f=f, seed=2026, u0=u0, tspan=(10, 11))
+        out = solve(prob, M.Synchronous(), backend=HIPBackend(), save_everystep=False).u[-1]
+        want = (1 if hit else 0) if k == 1 else (2 if hit else 1)
+        assert out[41, 0] == want, (k, q, out[41, 0])
+
+
 @pytest.mark.parametrize("which", ["growth", "lightning"])
 @pytest.mark.parametrize("hit", [True, False])
 def test_forest_draw_high_word_ties(oracle, which, hit):

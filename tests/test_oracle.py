@@ -50,6 +50,22 @@ def test_uniform_closed_form(oracle):
         assert oracle.uniform32(seed, cell, t, k) == np.float32((r[0] >> 8) * 2.0**-24)
 
 
+def test_uniform_reference_printed_values(oracle):
+    # the draws the reference prints for CellContext(nothing, CartesianIndex(3, 4), 10, 42, 2026)
+    # (vignettes/05_stochastic_rules/05_stochastic_rules.md:46-52): they fix the order of the
+    # counter words (cellid lo, cellid hi, t, k) and key words (seed lo, seed hi)
+    for c in G["uniform_draws"]["cases"]:
+        seed, cell, t, k, u = c["seed"], c["cellid"], c["t"], c["k"], c["u"]
+        assert oracle.uniform(seed, cell, t, k) == u
+        r = _py_philox((cell & 0xFFFFFFFF, cell >> 32, t & 0xFFFFFFFF, k), (seed & 0xFFFFFFFF, seed >> 32))
+        assert (((r[0] << 32) | r[1]) >> 11) * 2.0**-53 == u
+        # the plausible mistakes give other numbers: cellid and t swapped, the key's
+        # halves swapped, the output words swapped
+        assert oracle.uniform(seed, t, cell, k) != u
+        assert oracle.uniform(seed << 32, cell, t, k) != u
+        assert (((r[1] << 32) | r[0]) >> 11) * 2.0**-53 != u
+
+
 def test_uniform_invariants(oracle):
     # test/test_random.jl:18-56
     u = oracle.uniform
