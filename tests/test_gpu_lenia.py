@@ -13,6 +13,11 @@ from cahip.problem import CAProblem, solve
 from helpers import desc_for, golden
 
 pytestmark = pytest.mark.gpu
+GPU_FAULT_UNRESOLVED = (
+    "the one GPU run of this test (with the Orbium-mass and reference-draw tests together) ended "
+    "with the GPU box lost to a reported GPU fault before any output came back; no cause found by "
+    "reading the kernels it reaches, so it stays off the GPU until one is shown (its CPU "
+    "counterpart in test_oracle.py passes)")
 
 TOL = {"fft_f32": 1e-5, "mfma_f16x2": 1e-5, "mfma_f16": 1e-3, "mfma_bf16": 2e-2, "f32": 1e-4}
 
@@ -52,6 +57,30 @@ def test_orbium_glides_on_gpu(field):
         s = s.astype(np.float64)
         return (np.arange(1, 65)[:, None] * s).sum() / s.sum()
     assert abs(centre(sol.u[1]) - centre(sol.u[0])) > 3
+
+
+@pytest.mark.skip(reason=GPU_FAULT_UNRESOLVED)
+@pytest.mark.parametrize("field", ["seq", "f32", "fft_f32", "mfma_f16x2", "mfma_f16", "mfma_bf16"])
+def test_orbium_reference_masses(field):
+    """The masses the reference prints for Orbium on a 96 x 96 torus, 600 steps
+    (vignettes/09_continuous_and_lenia/09_continuous_and_lenia.md:141-166),
+    rounded to one decimal: equal for the FP64, FP32, FFT-FP32 and split-FP16
+    fields; within 0.5% for the FP16 and BF16 MFMA fields, whose per-step error
+    (2e-4, 2e-3) is larger than the smallest margin to a rounding boundary
+    (70.94 at t = 450)."""
+    g = golden("reference_vectors.json")["orbium_masses"]
+    rows = np.array(golden("reference_vectors.json")["orbium"]["rows"], dtype=np.float32)
+    u = np.zeros(tuple(g["dims"]), np.float32)
+    c1, c2 = (c - 1 for c in g["corner"])
+    u[c1:c1 + 20, c2:c2 + 20] = rows
+    prob = CAProblem(M.Lenia(), u, g["times"][-1])
+    sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field=field), saveat=150)
+    assert sol.t == g["times"]
+    mass = [float(s.astype(np.float64).sum()) for s in sol.u]
+    if field in ("mfma_f16", "mfma_bf16"):
+        assert all(abs(m - r) <= 0.005 * r for m, r in zip(mass, g["mass_1dp"])), mass
+    else:
+        assert [round(m, 1) for m in mass] == g["mass_1dp"], mass
 
 
 def test_fft_needs_periodic_edges():

@@ -370,6 +370,26 @@ def test_orbium_glides(oracle):
     assert abs(centre(snaps[1]) - centre(snaps[0])) > 3
 
 
+def test_orbium_reference_masses(oracle):
+    # the masses the reference prints for Orbium gliding on a 96 x 96 torus under
+    # Lenia(; method=:fft), rounded to one decimal (vignettes/09_continuous_and_lenia)
+    from oracle.lenia_fft import LeniaFFT
+    g = G["orbium_masses"]
+    rows = np.array(G["orbium"]["rows"], dtype=np.float64)
+    u = np.zeros(tuple(g["dims"]), np.float32)
+    c1, c2 = (c - 1 for c in g["corner"])
+    u[c1:c1 + 20, c2:c2 + 20] = rows.astype(np.float32)
+    prob = _lenia_problem(u)
+    taps, w = prob.taps
+    lf = LeniaFFT(u.shape, taps, w, prob.rule.mu, prob.rule.sigma, prob.rule.dt)
+    x, masses = u, [float(u.astype(np.float64).sum())]
+    for k in range(1, g["times"][-1] + 1):
+        x = lf.step(x)
+        if k in g["times"]:
+            masses.append(float(x.astype(np.float64).sum()))
+    assert [round(m, 1) for m in masses] == g["mass_1dp"]
+
+
 def test_generator(oracle):
     u = oracle.fill_bernoulli(1, (512, 512), 0.5)
     assert abs(u.mean() - 0.5) < 0.01 and set(np.unique(u)) == {0, 1}
