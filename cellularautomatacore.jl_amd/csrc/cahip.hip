@@ -1513,6 +1513,7 @@ struct cah_batch {
     int vb = 1, life_R = 0, niv = 1;
     bool small = true;
     int parity = 0;
+    int32_t rows = 0;  // rows per wave of the band walk; 0: batch_rows' choice
     int32_t nwaves = 0;
     void* items_d = nullptr;
     int32_t* wave_item_d = nullptr;
@@ -1559,7 +1560,7 @@ static int batch_layout(cah_batch* b) {
     const int64_t ow = 62 * 4 * b->vb;
     int64_t work = 0;
     for (cah_state* s : b->members) work += (s->n1 + ow - 1) / ow * s->n2;
-    const int64_t brows = batch_rows(work);
+    const int64_t brows = b->rows > 0 ? b->rows : batch_rows(work);
     for (size_t k = 0; k < b->members.size(); ++k) {
         cah_state* s = b->members[k];
         const int32_t nbands = (int32_t)((s->n1 + ow - 1) / ow);
@@ -1805,6 +1806,102 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
         if (b->kind == B_LIFE && nsteps > 0) s->bin_clean = true;  // Life writes 0 and 1 only
     }
     return CAH_OK;
+}
+
+// The band walk's tables, freed before batch_layout builds them again.
+static int free_layout(cah_batch* b) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->exec) HIPCHK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+    for (void** q : {&b->items_d, (void**)&b->wave_item_d, (void**)&b->wave_base_d}) {
+        if (*q) HIPCHK(hipFree(*q));
+        *q = nullptr;
+    }
+    b->fp_set = false;  // the new forest items carry no p and f yet
+    return CAH_OK;
+}
+
+extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const cah_params* p, int32_t* rows) {
+    if (!b || !rows || steps < 1 || t0 < 0) return fail(CAH_ERR_ARGUMENT, "cah_batch_autotune: bad arguments");
+    *rows = 0;
+    if (b->kind == B_LOOP || b->tb) return CAH_OK;  // no band walk to tune
+    if (b->kind == B_FOREST && !p) return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
+    HIPCHK(hipSetDevice(b->device));
+    for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    // every trajectory's lattice, to undo the trial steps
+    const int par0 = b->parity;
+    std::vector<void*> saved(b->members.size(), nullptr);
+    std::vector<bool> bin0;
+    auto release = [&]() {
+        for (void* q : saved)
+            if (q) hipFree(q);
+    };
+    size_t total = 0;
+    for (auto* s : b->members) total += (size_t)(s->pitch * s->n2) * s->es;
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        cah_state* s = b->members[i];
+        const size_t bytes = (size_t)(s->pitch * s->n2) * s->es;
+        if (hipMalloc(&saved[i], bytes) != hipSuccess) {
+            release();
+            return fail(CAH_ERR_OUT_OF_MEMORY, "cah_batch_autotune: no device memory for a copy of the batch "
+                                               "(%zu bytes)", total);
+        }
+        HIPCHK(hipMemcpyAsync(saved[i], s->buf[s->cur], bytes, hipMemcpyDeviceToDevice, b->stream));
+        bin0.push_back(s->bin_clean);
+    }
+    auto restore = [&]() -> int {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->parity = par0;
+        for (size_t i = 0; i < b->members.size(); ++i) {
+            cah_state* s = b->members[i];
+            s->cur = par0;
+            s->bin_clean = bin0[i];
+            HIPCHK(hipMemcpyAsync(s->buf[par0], saved[i], (size_t)(s->pitch * s->n2) * s->es,
+                                  hipMemcpyDeviceToDevice, b->stream));
+        }
+        HIPCHK(hipStreamSynchronize(b->stream));
+        return CAH_OK;
+    };
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int32_t keep = b->rows;
+    int32_t best = 0;
+    float best_ms = 0;
+    int rc = CAH_OK;
+    for (int32_t cand : {4, 8, 16, 32, 64}) {
+        b->rows = cand;
+        if ((rc = free_layout(b)) || (rc = batch_layout(b))) break;
+        if ((rc = cah_batch_advance(b, t0, 1, p, 0))) break;  // warm-up: code and caches
+        if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(CAH_ERR_DEVICE, "hipEventRecord failed"); break; }
+        if ((rc = cah_batch_advance(b, t0 + 1, steps, p, 0))) break;
+        float ms = 0;
+        if (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(CAH_ERR_DEVICE, "timing a trial layout failed");
+            break;
+        }
+        if (!best || ms < best_ms) {
+            best = cand;
+            best_ms = ms;
+        }
+        if ((rc = restore())) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!rc) {
+        b->rows = best;
+        if (!(rc = free_layout(b))) rc = batch_layout(b);
+    } else {
+        b->rows = keep;
+        restore();
+        if (!free_layout(b)) batch_layout(b);
+    }
+    if (!rc) rc = restore();
+    release();
+    if (!rc) *rows = best;
+    return rc;
 }
 
 extern "C" int cah_batch_sync(cah_batch* b) {
