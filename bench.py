@@ -226,8 +226,9 @@ def batch_sides(n=256, seed=2026):
 def bench_batched(be, ev, steps=300, n=256):
     """256 forest-fire trajectories (seeds 1..256, Reflecting, all empty,
     p=0.03, f=1e-4), sides log-uniform in [64, 2048], 300 steps: one launch per
-    step for all of them under a captured hipGraph, rows per wave chosen by the
-    batch autotuner (cah_batch_autotune), against the plain per-trajectory loop."""
+    step for all of them under a captured hipGraph, the tile (cells a lane, rows a
+    wave) chosen by the batch autotuner (cah_batch_autotune), against the plain
+    per-trajectory loop."""
     from cahip import _lib as L
     from cahip import model as M
     from cahip.backend import check, make_desc, make_params
@@ -246,9 +247,10 @@ def bench_batched(be, ev, steps=300, n=256):
     check(be.lib.cah_batch_stream(h, ctypes.byref(s)))
     for i, p in enumerate(probs):
         check(be.lib.cah_batch_set(h, i, p.u0.ctypes.data, p.u0.nbytes))
-    # rows per wave chosen by timing 20 steps of each candidate (the trial steps are undone)
-    tuned = ctypes.c_int32()
-    check(be.lib.cah_batch_autotune(h, 0, 20, ctypes.byref(cp), ctypes.byref(tuned)))
+    # the tile (cells a lane, rows a wave) chosen by timing 20 steps of each candidate
+    # (the trial steps are undone)
+    tuned_vb, tuned = ctypes.c_int32(), ctypes.c_int32()
+    check(be.lib.cah_batch_autotune(h, 0, 20, ctypes.byref(cp), ctypes.byref(tuned_vb), ctypes.byref(tuned)))
     check(be.lib.cah_batch_advance(h, 0, 2, ctypes.byref(cp), 1))  # warm-up (graph path)
     check(be.lib.cah_batch_sync(h))
     for i, p in enumerate(probs):  # restart every trajectory from its empty lattice
@@ -291,7 +293,7 @@ def bench_batched(be, ev, steps=300, n=256):
             "plain_queued_sims_per_s": round(n / t_queued, 1),
             "speedup_vs_plain_step_sync": round(t_step_sync / (ms_batch * 1e-3), 2),
             "speedup_vs_plain_queued": round(t_queued / (ms_batch * 1e-3), 2),
-            "autotuned_rows_per_wave": tuned.value,
+            "autotuned_tile": {"cells_a_lane": 4 * tuned_vb.value, "rows_a_wave": tuned.value},
             "identical_trajectories": bool(same)}
 
 

@@ -111,7 +111,8 @@ SIGNATURES = {
     "cah_batch_fill_bernoulli": (_I, [_P, _I32, _U64, _D]),
     "cah_batch_advance": (_I, [_P, _I64, _I64, _PP, _I32]),
     "cah_batch_counts": (_I, [_P, _P, _P, _P]),
-    "cah_batch_autotune": (_I, [_P, _I64, _I32, _PP, ctypes.POINTER(_I32)]),
+    "cah_batch_autotune": (_I, [_P, _I64, _I32, _PP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
+    "cah_batch_tune": (_I, [_P, _I32, _I32]),
     "cah_batch_sync": (_I, [_P]),
     "cah_batch_stream": (_I, [_P, ctypes.POINTER(_P)]),
     "cah_batch_destroy": (_I, [_P]),

@@ -270,7 +270,7 @@ class HIPBackend:
     # -------------------------------------------------------------- batches
 
     def solve_batch(self, probs, scheme, use_graph=True, output="state", observed=None, saveat=None,
-                    autotune=False):
+                    autotune=False, tile=None):
         """Several problems of one Int8 rule, advanced together.  output="state":
         their final states; "counts": state_counts of each final state; "kappa":
         cohens_kappa of each final state against `observed` (a map of the
@@ -281,7 +281,9 @@ class HIPBackend:
         the summaries are counted there on the device, without copying a lattice
         back.  With `autotune` the batch first times its band walk's candidate
         rows per wave over a few steps (cah_batch_autotune, which undoes them) and
-        keeps the fastest; `self.batch_rows` is the choice (0: nothing to tune)."""
+        keeps the fastest; `self.batch_tile` is the choice, (dwords a lane, rows a
+        wave), or (0, 0) when there is nothing to tune.  `tile` = (dwords a lane, rows
+        a wave) sets the tile instead (cah_batch_tune)."""
         from .problem import _save_times
         if output not in ("state", "counts", "kappa"):
             raise M.ArgumentError(f"unknown output {output!r}")
@@ -307,10 +309,13 @@ class HIPBackend:
             if output == "kappa":
                 obs = self.state_like(probs[0], scheme, observed)
             cp = make_params(probs[0], probs[0].p)
+            if tile is not None:
+                check(self.lib.cah_batch_tune(h, int(tile[0]), int(tile[1])))
             if autotune and t1 > t0:
-                rows = ctypes.c_int32()
-                check(self.lib.cah_batch_autotune(h, t0, min(20, t1 - t0), ctypes.byref(cp), ctypes.byref(rows)))
-                self.batch_rows = rows.value
+                vb, rows = ctypes.c_int32(), ctypes.c_int32()
+                check(self.lib.cah_batch_autotune(h, t0, min(20, t1 - t0), ctypes.byref(cp), ctypes.byref(vb),
+                                                  ctypes.byref(rows)))
+                self.batch_tile = (vb.value, rows.value)
             series, t = [], times[0]
             for k, ts in enumerate(times):
                 if ts > t:

@@ -1723,7 +1723,7 @@ static int forest_prec(const cah_params& p) { return (p.ff_p_is_f32 ? 1 : 0) | (
 static void launch_batch_step(cah_batch* b, int64_t t, bool bin) {
     const int blocks = waves_to_blocks(b->nwaves);
     if (b->kind == B_FOREST)
-        launch_forest_batch(b->small, forest_prec(b->fp), blocks, (const ForestArgs*)b->items_d,
+        launch_forest_batch(b->vb, b->small, forest_prec(b->fp), blocks, (const ForestArgs*)b->items_d,
                             b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, b->stream);
     else
         launch_life_batch(b->life_R, b->vb, bin, b->niv, blocks, (const LifeArgs*)b->items_d,
@@ -1821,9 +1821,22 @@ static int free_layout(cah_batch* b) {
     return CAH_OK;
 }
 
-extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const cah_params* p, int32_t* rows) {
-    if (!b || !rows || steps < 1 || t0 < 0) return fail(CAH_ERR_ARGUMENT, "cah_batch_autotune: bad arguments");
-    *rows = 0;
+extern "C" int cah_batch_tune(cah_batch* b, int32_t vb, int32_t rows) {
+    if (!b || vb < 0 || vb > 2 || rows < 0) return fail(CAH_ERR_ARGUMENT, "cah_batch_tune: bad arguments");
+    if (b->kind == B_LOOP || b->tb) return CAH_OK;  // no band walk
+    if (b->kind == B_LIFE && b->life_R == 5 && vb == 1)
+        return fail(CAH_ERR_ARGUMENT, "radius-5 Life needs 8 cells a lane (vb = 2)");
+    HIPCHK(hipSetDevice(b->device));
+    b->vb = vb ? vb : (b->kind == B_LIFE && b->life_R == 5 ? 2 : 1);
+    b->rows = rows;
+    if (int rc = free_layout(b)) return rc;
+    return batch_layout(b);
+}
+
+extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const cah_params* p, int32_t* vb,
+                                  int32_t* rows) {
+    if (!b || !vb || !rows || steps < 1 || t0 < 0) return fail(CAH_ERR_ARGUMENT, "cah_batch_autotune: bad arguments");
+    *vb = *rows = 0;
     if (b->kind == B_LOOP || b->tb) return CAH_OK;  // no band walk to tune
     if (b->kind == B_FOREST && !p) return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
     HIPCHK(hipSetDevice(b->device));
@@ -1866,11 +1879,17 @@ extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    const int32_t keep = b->rows;
-    int32_t best = 0;
+    const int32_t keep = b->rows, keep_vb = b->vb;
+    int32_t best = 0, best_vb = 0;
     float best_ms = 0;
     int rc = CAH_OK;
-    for (int32_t cand : {4, 8, 16, 32, 64}) {
+    // cells a lane: 4 or 8 (radius 5 needs 8); rows a wave: 4 .. 64
+    std::vector<std::pair<int, int32_t>> cands;
+    for (int v : {1, 2})
+        if (!(b->kind == B_LIFE && b->life_R == 5 && v == 1))
+            for (int32_t r : {4, 8, 16, 32, 64}) cands.push_back({v, r});
+    for (auto [cand_vb, cand] : cands) {
+        b->vb = cand_vb;
         b->rows = cand;
         if ((rc = free_layout(b)) || (rc = batch_layout(b))) break;
         if ((rc = cah_batch_advance(b, t0, 1, p, 0))) break;  // warm-up: code and caches
@@ -1884,6 +1903,7 @@ extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const
         }
         if (!best || ms < best_ms) {
             best = cand;
+            best_vb = cand_vb;
             best_ms = ms;
         }
         if ((rc = restore())) break;
@@ -1891,16 +1911,21 @@ extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (!rc) {
+        b->vb = best_vb;
         b->rows = best;
         if (!(rc = free_layout(b))) rc = batch_layout(b);
     } else {
+        b->vb = keep_vb;
         b->rows = keep;
         restore();
         if (!free_layout(b)) batch_layout(b);
     }
     if (!rc) rc = restore();
     release();
-    if (!rc) *rows = best;
+    if (!rc) {
+        *vb = best_vb;
+        *rows = best;
+    }
     return rc;
 }
 
@@ -2470,7 +2495,7 @@ static int build_tick(cah_server* s) {
                 launch_life_batch(R, R == 5 ? 2 : 1, g.key % 2, niv, waves_to_blocks(nw),
                                   (const LifeArgs*)items, map, base, nw, k & 1, s->st[SC_LIFE]);
             } else if (g.cls == SC_FOREST) {
-                launch_forest_batch(g.key % 2, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
+                launch_forest_batch(1, g.key % 2, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
                                     map, base, nw, k & 1, k, s->st[SC_FOREST]);
             } else {
                 launch_lenia_mfma_batch(g.key, nw, (const LeniaMfmaArgs*)items, map, base, k & 1, k,

@@ -23,20 +23,21 @@ void launch_forest(int vb, bool small, int blocks, const ForestArgs& a, hipStrea
 }
 
 template <bool SMALL, int PREC>
-static void forest_batch_go(int blocks, const ForestArgs* it, const int32_t* wi, const int32_t* wb,
+static void forest_batch_go(int vb, int blocks, const ForestArgs* it, const int32_t* wi, const int32_t* wb,
                             int32_t nw, int32_t parity, int64_t dt, hipStream_t st) {
-    k_forest_batch<1, SMALL, PREC><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, dt);
+    if (vb == 2) k_forest_batch<2, SMALL, PREC><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, dt);
+    else k_forest_batch<1, SMALL, PREC><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, dt);
 }
 
-void launch_forest_batch(bool small, int prec, int blocks, const ForestArgs* it, const int32_t* wi,
+void launch_forest_batch(int vb, bool small, int prec, int blocks, const ForestArgs* it, const int32_t* wi,
                          const int32_t* wb, int32_t nw, int32_t parity, int64_t dt, hipStream_t st) {
     auto go = [&](auto S) {
         constexpr bool SM = decltype(S)::value;
         switch (prec) {
-        case 0: forest_batch_go<SM, 0>(blocks, it, wi, wb, nw, parity, dt, st); break;
-        case 1: forest_batch_go<SM, 1>(blocks, it, wi, wb, nw, parity, dt, st); break;
-        case 2: forest_batch_go<SM, 2>(blocks, it, wi, wb, nw, parity, dt, st); break;
-        default: forest_batch_go<SM, 3>(blocks, it, wi, wb, nw, parity, dt, st); break;
+        case 0: forest_batch_go<SM, 0>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
+        case 1: forest_batch_go<SM, 1>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
+        case 2: forest_batch_go<SM, 2>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
+        default: forest_batch_go<SM, 3>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
         }
     };
     if (small) go(std::true_type{});

@@ -278,14 +278,20 @@ int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const cah_params
  * with an observed map (src/calibration.jl:76) would compute them. */
 int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* counts, int64_t* equal);
 /* No reference counterpart: the autotuner of the batched band walk.  Times
- * `steps` steps from t0 for each candidate number of rows per wave (4, 8, 16, 32,
- * 64; the walk's tile height, one value for every trajectory) with HIP events,
- * keeps the fastest, and leaves every trajectory's state and time as it was (the
- * trial steps run on the lattices and are undone from a device copy).  *rows is
- * the choice, or 0 when the batch has no band walk to tune (trajectories that
- * advance one by one, Moore(1) Life, which runs k_life_tb).  The layout holds
- * until the next cah_batch_autotune. */
-int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const cah_params* p, int32_t* rows);
+ * `steps` steps from t0 for each candidate tile (4 or 8 cells a lane, *vb = 1 or
+ * 2, radius-5 Life 8 only; 4, 8, 16, 32 or 64 rows a wave; one tile for every
+ * trajectory) with HIP events, keeps the fastest, and leaves every trajectory's
+ * state and time as it was (the trial steps run on the lattices and are undone
+ * from a device copy).  *vb and *rows are the choice, or 0 when the batch has no
+ * band walk to tune (trajectories that advance one by one, Moore(1) Life, which
+ * runs k_life_tb).  The layout holds until the next cah_batch_autotune. */
+int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const cah_params* p, int32_t* vb,
+                       int32_t* rows);
+/* No reference counterpart: sets the band walk's tile (vb = 1 or 2 dwords a lane,
+ * rows a wave; 0 for the default of either), as a tile cah_batch_autotune chose
+ * for an earlier batch of the same shapes.  No effect on batches without a band
+ * walk. */
+int cah_batch_tune(cah_batch* b, int32_t vb, int32_t rows);
 int cah_batch_sync(cah_batch* b);
 int cah_batch_stream(cah_batch* b, void** stream);
 int cah_batch_destroy(cah_batch* b);

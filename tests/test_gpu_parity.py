@@ -327,9 +327,28 @@ def test_resume_is_exact(oracle, backend):
     assert np.array_equal(whole, rest)
 
 
+@pytest.mark.parametrize("vb,rows", [(1, 4), (1, 64), (2, 4), (2, 64)])
+def test_batch_tiles(vb, rows):
+    """Every tile the autotuner may pick (4 or 8 cells a lane, 4..64 rows a wave)
+    gives each trajectory its single run: forest fires of ragged sizes and
+    Larger-than-Life of radius 2."""
+    from cahip.problem import final_state, seed_trajectory
+    be = HIPBackend()
+    probs = []
+    for i, dims in enumerate([(150, 97), (300, 211), (517, 60), (33, 1030)]):
+        base = forest_problem(dims, p=0.1, f=0.01, seed=0, u0=rng_state(i, dims, [0, 1, 1, 2]), tspan=(3, 21))
+        probs.append(seed_trajectory(base, i + 1))
+    ltl2 = [CAProblem(M.Life("B2/S13", neighbourhood=M.Moore(2)), rng_state(9 + i, (700, 90), [0, 1]), (0, 15),
+                      space=M.Lattice((700, 90), (M.Periodic(), M.Constant(1))), seed=i) for i in range(3)]
+    for group in (probs, ltl2):
+        outs = be.solve_batch(group, M.Synchronous(), tile=(vb, rows))
+        for p, out in zip(group, outs):
+            assert np.array_equal(out, final_state(solve(p, backend=be, save_everystep=False)))
+
+
 def test_batch_autotune_keeps_results():
-    """cah_batch_autotune times each candidate rows-per-wave on the batch's own
-    lattices and undoes the trial steps: the state at t0 (saved after tuning) is
+    """cah_batch_autotune times each candidate tile (cells a lane, rows a wave)
+    on the batch's own lattices and undoes the trial steps: the state at t0 (saved after tuning) is
     still u0, and every trajectory equals its single run (forest fire with
     ragged sizes, Larger-than-Life of radius 2).  Moore(1) Life runs k_life_tb:
     nothing to tune."""
@@ -343,13 +362,13 @@ def test_batch_autotune_keeps_results():
                       space=M.Lattice((200, 120), (M.Periodic(), M.Reflecting())), seed=i) for i in range(3)]
     for group in (probs, ltl2):
         series = be.solve_batch(group, M.Synchronous(), autotune=True, saveat=[group[0].tspan[1]])
-        assert be.batch_rows in (4, 8, 16, 32, 64)
+        assert be.batch_tile[0] in (1, 2) and be.batch_tile[1] in (4, 8, 16, 32, 64)
         for p, ser in zip(group, series):
             assert np.array_equal(ser[0], p.u0)
             assert np.array_equal(ser[-1], final_state(solve(p, backend=be, save_everystep=False)))
     conway = [CAProblem(M.Life("conway"), rng_state(20 + i, (90, 70), [0, 1]), 9) for i in range(2)]
     be.solve_batch(conway, M.Synchronous(), autotune=True)
-    assert be.batch_rows == 0
+    assert be.batch_tile == (0, 0)
 
 
 def test_batch_equals_single_runs(oracle, backend):
