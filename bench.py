@@ -387,8 +387,12 @@ def bench_sparse(be, ev, steps=200):
 
 def bench_lenia(be, ev, steps=500):
     """Lenia R=13 (mu 0.15, sigma 0.015, dt 0.1), Float32 Torus 2048^2, uniform[0,1]
-    in the central 256^2 patch: ms/step per field method and the per-step max-abs
-    difference from the FP64 sequential field (the path pinned to the oracle)."""
+    in the central 256^2 patch: ms/step per field method; (i) the per-step max-abs
+    difference from the FP64 sequential field (the path pinned to the oracle): one
+    step of the method against one FP64 step from the same state, at every one of
+    the 500 steps of the method's own trajectory (compared on the device); (ii) the
+    trajectory divergence at t = 500 (max-abs and mass against the FP64 run)."""
+    import torch
     from cahip import model as M
     from cahip.backend import HIPBackend, check, make_params
     from cahip.problem import CAProblem
@@ -397,40 +401,41 @@ def bench_lenia(be, ev, steps=500):
     cp = make_params(prob, None)
     ref_be = HIPBackend(0, lenia_field="seq")
     ref = ref_be.allocate_state(prob, M.Synchronous())
-    out = {}
-    samples = (0, 100, 250, steps - 1)
-    for field in ("mfma_f16", "mfma_f16x2", "mfma_bf16", "fft_f32", "f32", "seq"):
+    nbytes = n * n * 4
+    cur, mine, theirs = (torch.empty(n * n, dtype=torch.float32, device="cuda") for _ in range(3))
+    out, seq_final = {}, None
+    for field in ("seq", "mfma_f16", "mfma_f16x2", "mfma_bf16", "fft_f32", "f32"):
         fb = HIPBackend(0, lenia_field=field)
         st = fb.allocate_state(prob, M.Synchronous())
         check(fb.lib.cah_state_fill_uniform_box(st.handle, 1, 896, 256, 896, 256))
         init = fb.current_state(st)
-        saved = {}
-        t = 0
-        for smp in samples:  # the trajectory, with the states at the sample times
-            check(fb.lib.cah_advance(st.handle, t, smp - t, ctypes.byref(cp)))
-            t = smp
-            saved[smp] = fb.current_state(st)
-        check(fb.lib.cah_advance(st.handle, t, steps - t, ctypes.byref(cp)))
-        final = fb.current_state(st)
+        err = torch.zeros((), dtype=torch.float64, device="cuda")
+        for t in range(steps):  # the trajectory, one step at a time
+            if field != "seq":
+                torch.cuda.synchronize()  # the last comparison has read the buffers
+                check(fb.lib.cah_state_get_device(st.handle, ctypes.c_void_p(cur.data_ptr()), nbytes))
+                check(ref_be.lib.cah_state_set_device(ref.handle, ctypes.c_void_p(cur.data_ptr()), nbytes))
+                check(ref_be.lib.cah_advance(ref.handle, t, 1, ctypes.byref(cp)))
+            check(fb.lib.cah_advance(st.handle, t, 1, ctypes.byref(cp)))
+            if field != "seq":
+                check(fb.lib.cah_state_get_device(st.handle, ctypes.c_void_p(mine.data_ptr()), nbytes))
+                check(ref_be.lib.cah_state_get_device(ref.handle, ctypes.c_void_p(theirs.data_ptr()), nbytes))
+                err = torch.maximum(err, (mine.double() - theirs.double()).abs().max())
+        final = fb.current_state(st).astype(np.float64)
         fb.set_state(st, init)
         ms = timed(ev, fb.stream(st), lambda: check(
             fb.lib.cah_advance(st.handle, 0, steps, ctypes.byref(cp))))
-        err = 0.0
-        for smp, u in saved.items():  # one step of each from the same state
-            fb.set_state(st, u)
-            ref_be.set_state(ref, u)
-            check(fb.lib.cah_advance(st.handle, smp, 1, ctypes.byref(cp)))
-            check(ref_be.lib.cah_advance(ref.handle, smp, 1, ctypes.byref(cp)))
-            d = np.abs(fb.current_state(st).astype(np.float64) - ref_be.current_state(ref))
-            err = max(err, float(d.max()))
         fb.free_state(st)
+        if field == "seq":
+            seq_final = final
         out[field] = {"ms_per_step": round(ms / steps, 4),
                       "gcups": round(n * n * steps / (ms * 1e-3) / 1e9, 2),
-                      "per_step_max_abs_vs_fp64": err, "final_mass": float(final.sum())}
+                      "per_step_max_abs_vs_fp64": float(err.item()),
+                      "final_mass": float(final.sum()),
+                      "final_max_abs_vs_fp64": float(np.abs(final - seq_final).max()),
+                      "final_mass_rel_diff_vs_fp64": abs(float(final.sum()) - float(seq_final.sum())) /
+                      float(seq_final.sum())}
     ref_be.free_state(ref)
-    seq_final = out["seq"]["final_mass"]
-    for f in out:
-        out[f]["final_mass_rel_diff_vs_fp64"] = abs(out[f]["final_mass"] - seq_final) / seq_final
     return {"workload": "Lenia R=13 Float32 Torus 2048^2, central 256^2 uniform patch, 500 steps",
             "methods": out}
 
