@@ -27,12 +27,12 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, in T lane-instructions/s
 VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
-# k_forest_vn1<1, true, 0> (gfx950 ISA): 187 VALU instructions in the walk's row body of 4 cells
-# (4 draws, the rule, the store; the branch taken on a tie of the draw's high word left out)
-FOREST_VALU_PER_DRAW = 187 / 4
+# k_forest_vn1<1, true, 0, true> (gfx950 ISA): 181 VALU instructions in the walk's row body of 4
+# cells (4 draws, the rule, the store; the branch taken on a tie of the draw's high word left out)
+FOREST_VALU_PER_DRAW = 181 / 4
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_ltl8192_forest16384.json")
 COPY_N = 16384 * 16384  # bytes of the calibration copy (one 16384^2 Int8 lattice)
-KERNELS = {"ltl": "k_life_swar<5, 2, true, 1>", "forest": "k_forest_vn1<1, true, 0>",
+KERNELS = {"ltl": "k_life_swar<5, 2, true, 1>", "forest": "k_forest_vn1<1, true, 0, true>",
            "copy": "k_copy_rows<16>"}
 ALGO_BYTES = {"ltl": 2 * 8192 * 8192, "forest": 2 * 16384 * 16384, "copy": 2 * COPY_N}
 

@@ -259,6 +259,7 @@ struct cah_state {
     Path path = P_GENERIC;
     int32_t life_R = 0;
     bool bin_clean = false;  // every cell is 0 or 1 (Life only)
+    bool tri_clean = false;  // every cell is 0, 1 or 2 (the forest fire's k_forest_vn1 variant)
     LifeArgs life{};
     int32_t tune_vb = 0, tune_rows = 0;
     struct Arena* arena = nullptr;        // device memory source (nullptr: hipMalloc)
@@ -595,7 +596,7 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
     } else if (plain2d && d->rule == CAH_RULE_FOREST_FIRE && is_vn1(s->offsets)) {
         s->path = P_FOREST;
     }
-    s->bin_clean = true;  // the buffers start all zero
+    s->bin_clean = s->tri_clean = true;  // the buffers start all zero
     return CAH_OK;
 }
 
@@ -637,6 +638,12 @@ static int sparse_refresh(cah_state* s);
 static bool scan_binary(const int8_t* h, size_t n) {
     for (size_t i = 0; i < n; ++i)
         if ((uint8_t)h[i] > 1) return false;
+    return true;
+}
+
+static bool scan_012(const int8_t* h, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if ((uint8_t)h[i] > 2) return false;
     return true;
 }
 
@@ -741,6 +748,7 @@ extern "C" int cah_state_set(cah_state* s, const void* host, size_t nbytes) {
     HIPCHK(hipSetDevice(s->device));
     if (int rc = copy_in(s, host)) return rc;
     if (s->es == 1) s->bin_clean = scan_binary((const int8_t*)host, nbytes);
+    if (s->es == 1) s->tri_clean = s->bin_clean || scan_012((const int8_t*)host, nbytes);
     if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
@@ -762,7 +770,7 @@ extern "C" int cah_state_set_device(cah_state* s, const void* dptr, size_t nbyte
                            (int64_t)(s->n1 * s->es), s->n2, s->stream))
         return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (s->es == 1) s->bin_clean = false;  // not scanned
+    if (s->es == 1) s->bin_clean = s->tri_clean = false;  // not scanned
     if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
@@ -896,6 +904,7 @@ extern "C" int cah_state_write_cell(cah_state* s, int64_t i1, int64_t i2, const 
     if (s->sparse) s->sp_dirty = true;  // the lists are rebuilt before the next step
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->es == 1 && *(const uint8_t*)value > 1) s->bin_clean = false;
+    if (s->es == 1 && *(const uint8_t*)value > 2) s->tri_clean = false;
     return CAH_OK;
 }
 
@@ -1081,7 +1090,12 @@ static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
     a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
     const int blocks = waves_to_blocks(a.nwaves);
     const bool small = (uint64_t)(s->n1 * s->n2) < 0xFFFFFFFFull;
-    launch_forest(vb, small, blocks, a, s->stream);
+    auto edge012 = [&](int k) {
+        const double v = s->d.boundary_value[k];
+        return s->d.boundary[k] != CAH_BOUNDARY_CONSTANT || v == 0 || v == 1 || v == 2;
+    };
+    launch_forest(vb, small, s->tri_clean && edge012(0) && edge012(1), blocks, a, s->stream);
+    s->tri_clean = true;  // the rule writes 0, 1 and 2 only
     return CAH_OK;
 }
 
@@ -1477,7 +1491,7 @@ extern "C" int cah_state_fill_bernoulli(cah_state* s, uint64_t seed, double p) {
     k_fill_bernoulli<<<grid, 256, 0, s->stream>>>((int8_t*)s->buf[s->cur], lat_of(s), seed, p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->bin_clean = true;
+    s->bin_clean = s->tri_clean = true;
     if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
@@ -2218,6 +2232,7 @@ static int admit(cah_server* s, SrvReq* r) {
                                    st->n2, s->up))
                 return rc;
         if (st->es == 1) st->bin_clean = scan_binary((const int8_t*)h, n);
+        if (st->es == 1) st->tri_clean = st->bin_clean || scan_012((const int8_t*)h, n);
         std::vector<char>().swap(r->u0);
     }
     switch (st->path) {

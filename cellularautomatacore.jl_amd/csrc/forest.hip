@@ -1,25 +1,31 @@
 // forest.hip -- instantiations and launchers of the forest-fire kernels.
 #include "launch.hpp"
 
-template <int VB, bool SMALL>
+template <int VB, bool SMALL, bool TRI>
 static void forest_vs(int prec, int blocks, const ForestArgs& a, hipStream_t st) {
     switch (prec) {
-    case 0: k_forest_vn1<VB, SMALL, 0><<<blocks, 256, 0, st>>>(a); break;
-    case 1: k_forest_vn1<VB, SMALL, 1><<<blocks, 256, 0, st>>>(a); break;
-    case 2: k_forest_vn1<VB, SMALL, 2><<<blocks, 256, 0, st>>>(a); break;
-    default: k_forest_vn1<VB, SMALL, 3><<<blocks, 256, 0, st>>>(a); break;
+    case 0: k_forest_vn1<VB, SMALL, 0, TRI><<<blocks, 256, 0, st>>>(a); break;
+    case 1: k_forest_vn1<VB, SMALL, 1, TRI><<<blocks, 256, 0, st>>>(a); break;
+    case 2: k_forest_vn1<VB, SMALL, 2, TRI><<<blocks, 256, 0, st>>>(a); break;
+    default: k_forest_vn1<VB, SMALL, 3, TRI><<<blocks, 256, 0, st>>>(a); break;
     }
 }
 
-void launch_forest(int vb, bool small, int blocks, const ForestArgs& a, hipStream_t st) {
-    const int prec = (a.p_f32 ? 1 : 0) | (a.f_f32 ? 2 : 0);
+template <bool TRI>
+static void forest_t(int vb, bool small, int prec, int blocks, const ForestArgs& a, hipStream_t st) {
     if (vb == 2) {
-        if (small) forest_vs<2, true>(prec, blocks, a, st);
-        else forest_vs<2, false>(prec, blocks, a, st);
+        if (small) forest_vs<2, true, TRI>(prec, blocks, a, st);
+        else forest_vs<2, false, TRI>(prec, blocks, a, st);
     } else {
-        if (small) forest_vs<1, true>(prec, blocks, a, st);
-        else forest_vs<1, false>(prec, blocks, a, st);
+        if (small) forest_vs<1, true, TRI>(prec, blocks, a, st);
+        else forest_vs<1, false, TRI>(prec, blocks, a, st);
     }
+}
+
+void launch_forest(int vb, bool small, bool tri, int blocks, const ForestArgs& a, hipStream_t st) {
+    const int prec = (a.p_f32 ? 1 : 0) | (a.f_f32 ? 2 : 0);
+    if (tri) forest_t<true>(vb, small, prec, blocks, a, st);
+    else forest_t<false>(vb, small, prec, blocks, a, st);
 }
 
 template <bool SMALL, int PREC>

@@ -74,10 +74,25 @@ CAH_DEV uint32_t forest_bytes(uint32_t e4, uint32_t f4, uint32_t nf4, uint32_t h
     return ((em & hit4) | (~em & tree)) & ~fm;
 }
 
+// Burning (== 2) and empty (== 0) cells of four bytes, as 0x01 per byte.  TRI: the
+// walk reads only 0, 1 and 2 (the lattice has held nothing else since its last
+// step or scan, and no Constant edge holds another value), so bit 1 alone marks a
+// fire and a zero pair of low bits an empty cell; otherwise the exact byte test.
+template <bool TRI>
+CAH_DEV uint32_t forest_fire4(uint32_t x) {
+    if constexpr (TRI) return (x >> 1) & 0x01010101u;
+    else return bytes_eq(x, 0x02020202u);
+}
+template <bool TRI>
+CAH_DEV uint32_t forest_empty4(uint32_t x) {
+    if constexpr (TRI) return ~(x | (x >> 1)) & 0x01010101u;
+    else return bytes_eq(x, 0u);
+}
+
 // The walk of one wave down rows [ys, ye) of its band, as life_walk: rows in a
 // ring of 6 register slots (row r in slot (r - ys + 1) mod 6), each load writing
 // its final slot, 3 rows in flight.
-template <int VB, bool SMALL, int PREC, bool FAST>
+template <int VB, bool SMALL, int PREC, bool FAST, bool TRI>
 CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer) {
     constexpr int S = 6;
     const int8_t cv1 = (int8_t)a.cv1, cv2 = (int8_t)a.cv2;
@@ -102,8 +117,8 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
     });
 #pragma unroll
     for (int i = 0; i < VB; ++i) {
-        fire[0][i] = bytes_eq(raw[0][i], 0x02020202u);
-        fire[1][i] = bytes_eq(raw[1][i], 0x02020202u);
+        fire[0][i] = forest_fire4<TRI>(raw[0][i]);
+        fire[1][i] = forest_fire4<TRI>(raw[1][i]);
     }
 
     int32_t y = ys;
@@ -111,7 +126,7 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
         constexpr int ph = decltype(PH)::value;
         constexpr int s_dn = (ph + 2) % S, s_c = (ph + 1) % S, s_up = ph % S;
 #pragma unroll
-        for (int i = 0; i < VB; ++i) fire[s_dn][i] = bytes_eq(raw[s_dn][i], 0x02020202u);
+        for (int i = 0; i < VB; ++i) fire[s_dn][i] = forest_fire4<TRI>(raw[s_dn][i]);
         uint32_t E[VB + 2];
         E[0] = from_prev_lane(fire[s_c][VB - 1]);
         E[VB + 1] = from_next_lane(fire[s_c][0]);
@@ -122,7 +137,7 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
         bool tie = false;
         Unroll<0, VB>::run([&](auto I) {
             constexpr int i = decltype(I)::value;
-            e4[i] = bytes_eq(raw[s_c][i], 0u);
+            e4[i] = forest_empty4<TRI>(raw[s_c][i]);
             const uint32_t k4 = 0x02020202u - e4[i];  // the draw's k: 1 empty, 2 otherwise
             hit4[i] = 0;
             Unroll<0, 4>::run([&](auto B) {
@@ -177,7 +192,7 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
     });
 }
 
-template <int VB, bool SMALL, int PREC>
+template <int VB, bool SMALL, int PREC, bool TRI>
 CAH_DEV void forest_band(const ForestArgs& a, int wid) {
     constexpr int OW = 62 * 4 * VB;
     const int lane = threadIdx.x & 63;
@@ -186,15 +201,15 @@ CAH_DEV void forest_band(const ForestArgs& a, int wid) {
     const int32_t ye = min(ys + a.rows, a.L.n2);
     const int32_t x = band * OW - 4 * VB + lane * 4 * VB;
     const bool writer = lane >= 1 && lane <= 62 && x < a.L.n1;
-    if (mapped_ok<VB>(a.L, 3)) forest_walk<VB, SMALL, PREC, true>(a, x, ys, ye, writer);
-    else forest_walk<VB, SMALL, PREC, false>(a, x, ys, ye, writer);
+    if (mapped_ok<VB>(a.L, 3)) forest_walk<VB, SMALL, PREC, true, TRI>(a, x, ys, ye, writer);
+    else forest_walk<VB, SMALL, PREC, false, TRI>(a, x, ys, ye, writer);
 }
 
-template <int VB, bool SMALL, int PREC>
+template <int VB, bool SMALL, int PREC, bool TRI>
 __global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
     const int wid = wave_id();
     if (wid >= a.nwaves) return;
-    forest_band<VB, SMALL, PREC>(a, wid);
+    forest_band<VB, SMALL, PREC, TRI>(a, wid);
 }
 
 // ---------------------------------------------------------------- batched ensembles
@@ -223,5 +238,5 @@ __global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restri
         a.next = const_cast<int8_t*>(c);
     }
     a.t += dt;
-    forest_band<VB, SMALL, PREC>(a, wid - wave_base[k]);
+    forest_band<VB, SMALL, PREC, false>(a, wid - wave_base[k]);
 }

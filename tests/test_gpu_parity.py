@@ -168,6 +168,31 @@ def test_forest_band_walk(oracle, i, vb):
     assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t))
 
 
+@pytest.mark.parametrize("edge", [M.Reflecting(), M.Constant(3), M.Constant(2)])
+def test_forest_values_outside_the_rule(oracle, backend, edge):
+    """set_state! may bring values the forest fire never makes (3, -1, 5): the
+    rule reads them as trees.  The walk's short byte tests for lattices holding
+    only 0, 1 and 2 must not see them, nor a Constant(3) edge (bit 1 set, not a
+    fire): the first step and every step beside such an edge take the exact
+    byte test; once a step has run, a lattice with other edges takes the short one."""
+    dims = (300, 77)
+    u = rng_state(6, dims, [0, 1, 2, 3, -1, 5, 1, 1])
+    prob = forest_problem(dims, p=0.2, f=0.02, seed=7, boundary=edge, tspan=(2, 10))
+    st = backend.allocate_state(prob, M.Synchronous())
+    try:
+        backend.set_state(st, u)
+        backend.advance(st, prob, M.Synchronous(), prob.p, 2, 3)
+        got3 = backend.current_state(st)
+        backend.advance(st, prob, M.Synchronous(), prob.p, 5, 5)
+        got = backend.current_state(st)
+    finally:
+        backend.free_state(st)
+    d, keep = desc_for(prob)
+    ref3 = oracle.advance_plain(d, u, 2, 3, params_for(prob))
+    assert np.array_equal(got3, ref3)
+    assert np.array_equal(got, oracle.advance_plain(d, ref3, 5, 5, params_for(prob)))
+
+
 def test_forest_2048_from_empty(oracle, backend):
     dims = (2048, 1536)
     prob = forest_problem(dims, p=0.03, f=1e-4, seed=7, u0=np.zeros(dims, np.int8), tspan=60)
