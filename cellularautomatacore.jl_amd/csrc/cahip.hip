@@ -641,11 +641,6 @@ static bool scan_binary(const int8_t* h, size_t n) {
     return true;
 }
 
-static bool scan_012(const int8_t* h, size_t n) {
-    for (size_t i = 0; i < n; ++i)
-        if ((uint8_t)h[i] > 2) return false;
-    return true;
-}
 
 // Rows of rb bytes from a pitched to a dense layout or back, on a stream.
 static int copy_rows(const void* src, int64_t sp, void* dst, int64_t dp, int64_t rb, int64_t rows,
@@ -748,7 +743,9 @@ extern "C" int cah_state_set(cah_state* s, const void* host, size_t nbytes) {
     HIPCHK(hipSetDevice(s->device));
     if (int rc = copy_in(s, host)) return rc;
     if (s->es == 1) s->bin_clean = scan_binary((const int8_t*)host, nbytes);
-    if (s->es == 1) s->tri_clean = s->bin_clean || scan_012((const int8_t*)host, nbytes);
+    // a state with 2s takes the forest fire's exact byte test for its first step
+    // (a scan for them here would cost a full pass over the host copy)
+    if (s->es == 1) s->tri_clean = s->bin_clean;
     if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
@@ -2232,7 +2229,7 @@ static int admit(cah_server* s, SrvReq* r) {
                                    st->n2, s->up))
                 return rc;
         if (st->es == 1) st->bin_clean = scan_binary((const int8_t*)h, n);
-        if (st->es == 1) st->tri_clean = st->bin_clean || scan_012((const int8_t*)h, n);
+        if (st->es == 1) st->tri_clean = st->bin_clean;
         std::vector<char>().swap(r->u0);
     }
     switch (st->path) {
