@@ -94,7 +94,7 @@ void lenia_fft_destroy(LeniaFFT* f) {
     if (f->fwd) hipfftDestroy(f->fwd);
     if (f->inv) hipfftDestroy(f->inv);
     for (void* p : {(void*)f->spec, (void*)f->kspec, (void*)f->field})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);  // teardown: nothing to report to
     delete f;
 }
 
@@ -136,11 +136,14 @@ int lenia_fft_create(LeniaFFT** out, int n1, int n2, int64_t pitch, const int32_
         for (int a = 0; a < n1; ++a) kpad[(size_t)b * pitch + a] = (float)acc[(size_t)b * n1 + a];
     float* kdev = nullptr;
     if (hipMalloc(&kdev, kpad.size() * sizeof(float)) != hipSuccess) return bail(-4);
-    hipMemcpy(kdev, kpad.data(), kpad.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (hipMemcpy(kdev, kpad.data(), kpad.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(kdev);
+        return bail(-7);
+    }
     hipfftResult r = hipfftExecR2C(f->fwd, kdev, (hipfftComplex*)f->kspec);
-    hipStreamSynchronize(st);
-    hipFree(kdev);
-    if (r != HIPFFT_SUCCESS) return bail(-7);
+    const hipError_t sync = hipStreamSynchronize(st);
+    (void)hipFree(kdev);
+    if (r != HIPFFT_SUCCESS || sync != hipSuccess) return bail(-7);
     // conj(rfft(kpad)) / (n1 n2): the inverse transform is unnormalised
     const float scale = 1.0f / ((float)n1 * (float)n2);
     k_spec_conj_scale<<<(unsigned)((f->nspec + 255) / 256), 256, 0, st>>>(f->kspec, f->nspec, scale);
