@@ -1731,13 +1731,15 @@ extern "C" int cah_batch_fill_bernoulli(cah_batch* b, int32_t i, uint64_t seed, 
 
 static int forest_prec(const cah_params& p) { return (p.ff_p_is_f32 ? 1 : 0) | (p.ff_f_is_f32 ? 2 : 0); }
 
-static void launch_batch_step(cah_batch* b, int64_t t, bool bin) {
+// clean: every lattice and Constant edge holds only 0 and 1 (Life) or only 0, 1
+// and 2 (the forest fire), which lets the kernels skip the exact byte tests.
+static void launch_batch_step(cah_batch* b, int64_t t, bool clean) {
     const int blocks = waves_to_blocks(b->nwaves);
     if (b->kind == B_FOREST)
-        launch_forest_batch(b->vb, b->small, forest_prec(b->fp), blocks, (const ForestArgs*)b->items_d,
+        launch_forest_batch(b->vb, b->small, clean, forest_prec(b->fp), blocks, (const ForestArgs*)b->items_d,
                             b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, b->stream);
     else
-        launch_life_batch(b->life_R, b->vb, bin, b->niv, blocks, (const LifeArgs*)b->items_d,
+        launch_life_batch(b->life_R, b->vb, clean, b->niv, blocks, (const LifeArgs*)b->items_d,
                           b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, b->stream);
     b->parity ^= 1;
 }
@@ -1773,14 +1775,16 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
         b->fp = *p;
         b->fp_set = true;
     }
-    bool bin = true;
+    bool bin = true, tri = true;
     for (auto* s : b->members) {
-        const bool e1 = s->d.boundary[0] != CAH_BOUNDARY_CONSTANT ||
-                        s->d.boundary_value[0] == 0 || s->d.boundary_value[0] == 1;
-        const bool e2 = s->d.boundary[1] != CAH_BOUNDARY_CONSTANT ||
-                        s->d.boundary_value[1] == 0 || s->d.boundary_value[1] == 1;
-        bin &= s->bin_clean && e1 && e2;
+        auto edge = [&](int k, double top) {
+            const double v = s->d.boundary_value[k];
+            return s->d.boundary[k] != CAH_BOUNDARY_CONSTANT || (v >= 0 && v <= top && v == (double)(int)v);
+        };
+        bin &= s->bin_clean && edge(0, 1) && edge(1, 1);
+        tri &= s->tri_clean && edge(0, 2) && edge(1, 2);
     }
+    if (b->kind == B_FOREST) bin = tri;  // the forest kernels' short byte tests
     // Moore(1) Life: up to TB_K steps a launch (the items run TB_K - step0 steps),
     // a last single step through the band walk, as a single lattice advances
     auto queue = [&]() {
@@ -1813,7 +1817,10 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
     HIPCHK(hipGetLastError());
     for (auto* s : b->members) {
         s->cur = b->parity;
-        if (b->kind == B_FOREST) s->bin_clean = false;
+        if (b->kind == B_FOREST && nsteps > 0) {
+            s->bin_clean = false;
+            s->tri_clean = true;  // the forest fire writes 0, 1 and 2 only
+        }
         if (b->kind == B_LIFE && nsteps > 0) s->bin_clean = true;  // Life writes 0 and 1 only
     }
     return CAH_OK;
@@ -1856,7 +1863,7 @@ extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const
     // every trajectory's lattice, to undo the trial steps
     const int par0 = b->parity;
     std::vector<void*> saved(b->members.size(), nullptr);
-    std::vector<bool> bin0;
+    std::vector<bool> bin0, tri0;
     auto release = [&]() {
         for (void* q : saved)
             if (q) hipFree(q);
@@ -1873,6 +1880,7 @@ extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const
         }
         HIPCHK(hipMemcpyAsync(saved[i], s->buf[s->cur], bytes, hipMemcpyDeviceToDevice, b->stream));
         bin0.push_back(s->bin_clean);
+        tri0.push_back(s->tri_clean);
     }
     auto restore = [&]() -> int {
         HIPCHK(hipStreamSynchronize(b->stream));
@@ -1881,6 +1889,7 @@ extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const
             cah_state* s = b->members[i];
             s->cur = par0;
             s->bin_clean = bin0[i];
+            s->tri_clean = tri0[i];
             HIPCHK(hipMemcpyAsync(s->buf[par0], saved[i], (size_t)(s->pitch * s->n2) * s->es,
                                   hipMemcpyDeviceToDevice, b->stream));
         }
@@ -2507,7 +2516,7 @@ static int build_tick(cah_server* s) {
                 launch_life_batch(R, R == 5 ? 2 : 1, g.key % 2, niv, waves_to_blocks(nw),
                                   (const LifeArgs*)items, map, base, nw, k & 1, s->st[SC_LIFE]);
             } else if (g.cls == SC_FOREST) {
-                launch_forest_batch(1, g.key % 2, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
+                launch_forest_batch(1, g.key % 2, false, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
                                     map, base, nw, k & 1, k, s->st[SC_FOREST]);
             } else {
                 launch_lenia_mfma_batch(g.key, nw, (const LeniaMfmaArgs*)items, map, base, k & 1, k,

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
 // only the items those waves belong to, so items ordered by steps left can stop
 // at different steps of one sequence of launches.
 
-template <int VB, bool SMALL, int PREC>
+template <int VB, bool SMALL, int PREC, bool TRI>
 __global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restrict__ items,
                                                       const int32_t* __restrict__ wave_item,
                                                       const int32_t* __restrict__ wave_base,
@@ -238,5 +238,5 @@ __global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restri
         a.next = const_cast<int8_t*>(c);
     }
     a.t += dt;
-    forest_band<VB, SMALL, PREC, false>(a, wid - wave_base[k]);
+    forest_band<VB, SMALL, PREC, TRI>(a, wid - wave_base[k]);
 }

@@ -21,8 +21,8 @@ void launch_life_batch_r(int vb, bool bin, int niv, int blocks, const LifeArgs* 
 // the lattice and its Constant edges hold only 0, 1 and 2.
 void launch_forest(int vb, bool small, bool tri, int blocks, const ForestArgs& a, hipStream_t st);
 // The batch items carry p, f and their time base; prec = p_f32 | f_f32 << 1;
-// vb in {1, 2}.
-void launch_forest_batch(int vb, bool small, int prec, int blocks, const ForestArgs* items,
+// vb in {1, 2}; tri: every item's lattice and Constant edges hold only 0, 1 and 2.
+void launch_forest_batch(int vb, bool small, bool tri, int prec, int blocks, const ForestArgs* items,
                          const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,
                          int32_t parity, int64_t dt, hipStream_t st);
 

@@ -371,6 +371,44 @@ def test_batch_tiles(vb, rows):
             assert np.array_equal(out, final_state(solve(p, backend=be, save_everystep=False)))
 
 
+def test_batch_forest_values_outside_the_rule():
+    """A batch whose lattices hold values the forest fire never makes (3, -1, 5)
+    runs the exact byte tests until a step has cleaned them, then the short
+    ones: each trajectory equals its single run over two advances."""
+    from cahip import _lib as L
+    from cahip.backend import check
+    be = HIPBackend()
+    dims_list = [(150, 97), (64, 64), (517, 33)]
+    probs = [forest_problem(d, p=0.2, f=0.02, seed=3 + i, tspan=(2, 10)) for i, d in enumerate(dims_list)]
+    us = [rng_state(30 + i, d, [0, 1, 2, 3, -1, 5, 1, 1]) for i, d in enumerate(dims_list)]
+    descs, keeps = zip(*[make_desc(p, M.Synchronous()) for p in probs])
+    arr = (L.CahDesc * len(descs))(*descs)
+    h = ctypes.c_void_p()
+    check(be.lib.cah_batch_create(be.ctx(), len(descs), arr, ctypes.byref(h)))
+    outs = []
+    try:
+        for i, u in enumerate(us):
+            uf = np.asfortranarray(u)
+            check(be.lib.cah_batch_set(h, i, uf.ctypes.data, uf.nbytes))
+        cp = make_params(probs[0], probs[0].p)
+        check(be.lib.cah_batch_advance(h, 2, 3, ctypes.byref(cp), 1))
+        check(be.lib.cah_batch_advance(h, 5, 5, ctypes.byref(cp), 1))
+        for i, p in enumerate(probs):
+            out = np.empty(p.space.dims, np.int8, order="F")
+            check(be.lib.cah_batch_get(h, i, out.ctypes.data, out.nbytes))
+            outs.append(out)
+    finally:
+        be.lib.cah_batch_destroy(h)
+    for p, u, out in zip(probs, us, outs):
+        st = be.allocate_state(p, M.Synchronous())
+        try:
+            be.set_state(st, u)
+            be.advance(st, p, M.Synchronous(), p.p, 2, 8)
+            assert np.array_equal(out, be.current_state(st))
+        finally:
+            be.free_state(st)
+
+
 def test_batch_autotune_keeps_results():
     """cah_batch_autotune times each candidate tile (cells a lane, rows a wave)
     on the batch's own lattices and undoes the trial steps: the state at t0 (saved after tuning) is
