@@ -220,6 +220,25 @@ def test_forest_draw_matches_reference_printed_value(k):
         assert out[41, 0] == want, (k, q, out[41, 0])
 
 
+@pytest.mark.parametrize("k", [1, 2])
+def test_forest_draw_reference_value_on_300x200(oracle, k):
+    """The check above on the 300 x 200 lattice and the probability ranges of
+    the forest tests that run on the GPU (cell 42 is (41, 0) on any lattice at
+    least 42 cells wide; no probability is zero): the kernel's draw at cell 42,
+    t = 10, seed 2026 sits exactly where the reference prints it, and the whole
+    step equals the oracle's."""
+    u_ref = {1: 0.038246647972453474, 2: 0.403975113636703}[k]
+    dims = (300, 200)
+    for q, hit in [(float(np.nextafter(u_ref, 1.0)), True), (u_ref, False)]:
+        u0 = np.full(dims, 0 if k == 1 else 1, np.int8)  # empty (growth) or trees without fire (lightning)
+        p, f = (q, 1e-4) if k == 1 else (0.03, q)
+        prob = forest_problem(dims, p=p, f=f, seed=2026, u0=u0, tspan=(10, 11))
+        sol = solve(prob, M.Synchronous(), backend=HIPBackend(), saveat=1)
+        want = (1 if hit else 0) if k == 1 else (2 if hit else 1)
+        assert sol.u[-1][41, 0] == want, (k, q, sol.u[-1][41, 0])
+        assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t))
+
+
 @pytest.mark.parametrize("which", ["growth", "lightning"])
 @pytest.mark.parametrize("hit", [True, False])
 def test_forest_draw_high_word_ties(oracle, which, hit):
