@@ -2199,6 +2199,15 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
     return CAH_OK;
 }
 
+static bool edges012(const cah_state* st) {
+    const cah_desc& d = st->d;
+    auto ok = [&](int k) {
+        const double v = d.boundary_value[k];
+        return d.boundary[k] != CAH_BOUNDARY_CONSTANT || v == 0 || v == 1 || v == 2;
+    };
+    return ok(0) && ok(1);
+}
+
 static bool edges01(const cah_state* st) {
     const cah_desc& d = st->d;
     return (d.boundary[0] != CAH_BOUNDARY_CONSTANT || d.boundary_value[0] == 0 || d.boundary_value[0] == 1) &&
@@ -2304,6 +2313,7 @@ struct SrvGroup {
     std::vector<int32_t> base;                         // first wave/block of each item, + total
     bool tb = false;                                   // Moore(1) Life through k_life_tb
     int shape = 0;                                     // its tile shape
+    bool tri_edges = false, tri_cells = false;         // forest fire: edges / lattices hold only 0, 1, 2
 };
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -2359,6 +2369,9 @@ static int build_tick(cah_server* s) {
         g.cls = r->cls;
         g.key = r->key;
         g.tb = r->tb = r->cls == SC_LIFE && r->key == 4;
+        if (g.reqs.empty()) g.tri_edges = g.tri_cells = true;
+        g.tri_edges &= edges012(r->st);
+        g.tri_cells &= r->st->tri_clean;
         g.reqs.push_back(r);
     }
     // table layout: per group items, wave (block) -> item map, item bases
@@ -2516,7 +2529,10 @@ static int build_tick(cah_server* s) {
                 launch_life_batch(R, R == 5 ? 2 : 1, g.key % 2, niv, waves_to_blocks(nw),
                                   (const LifeArgs*)items, map, base, nw, k & 1, s->st[SC_LIFE]);
             } else if (g.cls == SC_FOREST) {
-                launch_forest_batch(1, g.key % 2, false, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
+                // the short byte tests once every lattice holds only 0, 1 and 2: from
+                // the tick's second step on, every item has taken a step of the rule
+                const bool tri = g.tri_edges && (g.tri_cells || k > 0);
+                launch_forest_batch(1, g.key % 2, tri, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
                                     map, base, nw, k & 1, k, s->st[SC_FOREST]);
             } else {
                 launch_lenia_mfma_batch(g.key, nw, (const LeniaMfmaArgs*)items, map, base, k & 1, k,
@@ -2537,7 +2553,10 @@ static int build_tick(cah_server* s) {
             r->st->bin_clean = true;                             // Life writes 0 and 1 only
         } else if (r->cls != SC_LOOP) {
             r->st->cur ^= r->steps & 1;
-            if (r->cls == SC_FOREST && r->steps > 0) r->st->bin_clean = false;
+            if (r->cls == SC_FOREST && r->steps > 0) {
+                r->st->bin_clean = false;
+                r->st->tri_clean = true;  // the forest fire writes 0, 1 and 2 only
+            }
         }
         r->queued += r->steps;
         if (r->queued == r->t1) {
