@@ -282,9 +282,10 @@ int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* counts, int6
  * 2, radius-5 Life 8 only; 4, 8, 16, 32 or 64 rows a wave; one tile for every
  * trajectory) with HIP events, keeps the fastest, and leaves every trajectory's
  * state and time as it was (the trial steps run on the lattices and are undone
- * from a device copy, which needs device memory for one copy of every lattice).  *vb and *rows are the choice, or 0 when the batch has no
- * band walk to tune (trajectories that advance one by one, Moore(1) Life, which
- * runs k_life_tb).  The layout holds until the next cah_batch_autotune. */
+ * from a device copy, which needs device memory for one copy of every lattice).
+ * *vb and *rows are the choice, or 0 when the batch has no band walk to tune
+ * (trajectories that advance one by one, Moore(1) Life, which runs k_life_tb).
+ * The layout holds until the next cah_batch_autotune. */
 int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const cah_params* p, int32_t* vb,
                        int32_t* rows);
 /* No reference counterpart: sets the band walk's tile (vb = 1 or 2 dwords a lane,
