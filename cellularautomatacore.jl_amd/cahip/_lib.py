@@ -116,6 +116,8 @@ SIGNATURES = {
     "cah_batch_sync": (_I, [_P]),
     "cah_batch_stream": (_I, [_P, ctypes.POINTER(_P)]),
     "cah_batch_destroy": (_I, [_P]),
+    "cah_check_launches": (_I, [_PD, _I64, _I64, _PP, ctypes.c_char_p, _SZ]),
+    "cah_check_batch_launches": (_I, [_I32, _PD, _I64, _I64, _PP, _I32, _I32, ctypes.c_char_p, _SZ]),
     "cah_server_create": (_I, [_P, _SZ, _I32, ctypes.POINTER(_P)]),
     "cah_server_submit": (_I, [_P, _PD, _P, _SZ, _I64, _I64, _PP, ctypes.POINTER(_U64)]),
     "cah_server_tune": (_I, [_P, _I32, _D]),

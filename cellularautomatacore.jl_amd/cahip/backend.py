@@ -131,6 +131,16 @@ def make_desc(prob, scheme, field_method=L.FIELD_DIRECT_F32):
     return d, keep
 
 
+def _parse_report(text):
+    """{kernel: (launches, checks)} from a launch-check report."""
+    out = {}
+    for line in text.splitlines():
+        name, rest = line.rsplit(": ", 1)
+        launches, checks = (int(x.split()[0]) for x in rest.split(", "))
+        out[name] = (launches, checks)
+    return out
+
+
 def count_dict(counts):
     """{Int8 value: count} over the values present, from counts by byte value."""
     return {(v if v < 128 else v - 256): int(c) for v, c in enumerate(counts) if c}
@@ -352,6 +362,37 @@ class HIPBackend:
         co = count_dict(self._counts(obs)[0])
         cells = int(np.prod(obs.dims))
         return [MS.kappa_from_counts(count_dict(c), co, int(e), cells) for c, e in zip(counts, eq)]
+
+    # -------------------------------------------------------------- launch checks
+
+    def check_launches(self, prob, scheme=None, t0=None, nsteps=None):
+        """The host-side launch checks (cah_check_launches) of prob's launches:
+        every kernel launch of `nsteps` steps from t0 (the whole tspan by
+        default), checked against the buffers it may read and write, without a
+        device.  Returns {kernel: (launches, checks)}; raises DeviceError naming
+        the first failed check."""
+        scheme = M.Synchronous() if scheme is None else scheme
+        d, _keep = make_desc(prob, scheme, self.lenia_field)
+        t0 = prob.tspan[0] if t0 is None else t0
+        nsteps = prob.tspan[1] - prob.tspan[0] if nsteps is None else nsteps
+        cp = make_params(prob, prob.p)
+        rep = ctypes.create_string_buffer(1 << 14)
+        check(self.lib.cah_check_launches(ctypes.byref(d), int(t0), int(nsteps), ctypes.byref(cp), rep, len(rep)))
+        return _parse_report(rep.value.decode())
+
+    def check_batch_launches(self, probs, scheme=None, tile=(0, 0)):
+        """The launch checks (cah_check_batch_launches) of a batch of probs over
+        their common tspan, with the band walk's tile (dwords a lane, rows a wave;
+        0 for the default)."""
+        scheme = M.Synchronous() if scheme is None else scheme
+        descs, keeps = zip(*[make_desc(pr, scheme, self.lenia_field) for pr in probs])
+        arr = (L.CahDesc * len(descs))(*descs)
+        t0, t1 = probs[0].tspan
+        cp = make_params(probs[0], probs[0].p)
+        rep = ctypes.create_string_buffer(1 << 14)
+        check(self.lib.cah_check_batch_launches(len(descs), arr, int(t0), int(t1 - t0), ctypes.byref(cp),
+                                                int(tile[0]), int(tile[1]), rep, len(rep)))
+        return _parse_report(rep.value.decode())
 
     def __del__(self):
         if getattr(self, "_ctx", None) is not None:

@@ -140,7 +140,7 @@ struct LaneMap {
 };
 
 template <int VB>
-CAH_DEV LaneMap lane_map(const Lat& L, int32_t x) {
+CAH_HD LaneMap lane_map(const Lat& L, int32_t x) {
     constexpr int W = 4 * VB;
     LaneMap m{x, false, false};
     if (x >= 0 && x + W <= L.n1) return m;
@@ -152,19 +152,42 @@ CAH_DEV LaneMap lane_map(const Lat& L, int32_t x) {
     } else {
         m.fill = true;
     }
-    m.src = min(max(m.src, 0), L.n1 - W);  // lanes far past an edge only need a valid address
+    // lanes far past an edge only need a valid address
+    m.src = m.src < 0 ? 0 : (m.src > L.n1 - W ? L.n1 - W : m.src);
     return m;
 }
 
 // The row a (wave-uniform) index y reads, and whether it lies beyond a
 // Constant axis-2 edge.  Valid for -n2 < y < 2 n2.
-CAH_DEV int32_t row_map(const Lat& L, int32_t y, bool& fill) {
+CAH_HD int32_t row_map(const Lat& L, int32_t y, bool& fill) {
     fill = false;
     if (y >= 0 && y < L.n2) return y;
     if (L.b2 == CAH_BOUNDARY_PERIODIC) return y < 0 ? y + L.n2 : y - L.n2;
     if (L.b2 == CAH_BOUNDARY_REFLECTING) return y < 0 ? -1 - y : 2 * L.n2 - 1 - y;
     fill = true;
     return 0;
+}
+
+// The part of a band walk one wave owns: wave `wid` walks band wid % nbands over
+// the rows [ys, ye) of chunk wid / nbands, and its lane reads the 4*VB bytes
+// from column x; lanes 1..62 inside the lattice write them (the end lanes are
+// guards).  Bands are 62 * 4 * VB bytes wide.
+struct BandPart {
+    int32_t x, ys, ye;
+    bool writer, up;
+};
+
+template <int VB>
+CAH_HD BandPart band_part(int32_t nbands, int32_t rows, const Lat& L, int wid, int lane) {
+    constexpr int OW = 62 * 4 * VB;
+    const int band = wid % nbands, chunk = wid / nbands;
+    BandPart b;
+    b.ys = chunk * rows;
+    b.ye = b.ys + rows < L.n2 ? b.ys + rows : L.n2;
+    b.x = band * OW - 4 * VB + lane * 4 * VB;
+    b.writer = lane >= 1 && lane <= 62 && b.x < L.n1;
+    b.up = chunk & 1;
+    return b;
 }
 
 CAH_DEV uint32_t bswap32(uint32_t v) { return __builtin_amdgcn_perm(v, v, 0x00010203u); }
@@ -233,7 +256,7 @@ CAH_DEV void store_mapped(__amdgpu_buffer_rsrc_t u, const Lat& L, int32_t x, int
 
 // Whether a lattice's rows and lanes can all use load_mapped.
 template <int VB>
-CAH_DEV bool mapped_ok(const Lat& L, int32_t halo_rows) {
+CAH_HD bool mapped_ok(const Lat& L, int32_t halo_rows) {
     return L.n1 % (4 * VB) == 0 && L.n1 >= 64 * 4 * VB && L.n2 > halo_rows &&
            L.pitch * (int64_t)L.n2 < (int64_t(1) << 31);
 }

@@ -31,6 +31,7 @@
 #include "kernels_generic.hpp"
 #include "launch.hpp"
 #include "kernels_lenia.hpp"
+#include "launch_check.hpp"
 
 // ---------------------------------------------------------------- Life dispatch
 
@@ -80,6 +81,28 @@ static int fail(int code, const char* fmt, ...) {
     } while (0)
 
 extern "C" const char* cah_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------- launch checks
+//
+// The launches the checks have seen on this thread since the last reset, by
+// kernel: (launches, individual checks), for the dry entry points' report.
+static thread_local std::map<std::string, std::pair<int64_t, int64_t>> g_tally;
+
+// Runs the launch checks `body` (a function of an lchk::Chk) when they are on:
+// for a dry state or batch, or everywhere with CAH_CHECK_LAUNCHES set.  The
+// launch must not run when this returns an error.
+template <class F>
+static int checked(bool dry, const char* kernel, F&& body) {
+    if (!dry && !lchk::enabled()) return CAH_OK;
+    lchk::Chk c(kernel);
+    body(c);
+    auto& t = g_tally[kernel];
+    t.first += 1;
+    t.second += c.checks;
+    if (!c.ok()) return fail(CAH_ERR_DEVICE, "launch check failed: %s", c.err.c_str());
+    return CAH_OK;
+}
+
 
 // ---------------------------------------------------------------- context
 
@@ -266,6 +289,7 @@ struct cah_state {
     struct Arena* staging = nullptr;      // pinned host memory of a server's uploads
     bool own_stream = true;               // false: the stream belongs to a server
     std::vector<void*> keep;              // pinned sources of queued uploads
+    bool dry = false;  // fake device addresses, no HIP call: launches are checked, not run
 };
 
 static Lat lat_of(const cah_state* s) {
@@ -358,6 +382,7 @@ struct Arena {
                 if (b.size > n) free_[p + n] = Blk{b.size - n, b.slab};
                 used[p] = Blk{n, b.slab};
                 in_use += n;
+                if (!pinned) lchk::reg(p, n, "server arena block");
                 return p;
             }
             const size_t sz = std::max(slab_min, n);
@@ -378,6 +403,7 @@ struct Arena {
         Blk b = u->second;
         used.erase(u);
         in_use -= b.size;
+        if (!pinned) lchk::unreg(p);
         auto nx = free_.find(p + b.size);
         if (nx != free_.end() && nx->second.slab == b.slab) {
             b.size += nx->second.size;
@@ -401,20 +427,37 @@ struct Arena {
     }
 };
 
+// hipMalloc, and the span the launch checks know the block by.
+static hipError_t dmalloc(void** p, size_t n, const char* what) {
+    const hipError_t e = hipMalloc(p, std::max<size_t>(n, 1));
+    if (e == hipSuccess) lchk::reg(*p, std::max<size_t>(n, 1), what);
+    return e;
+}
+
+static void dfree(void* p, bool dry = false) {
+    if (!p) return;
+    lchk::unreg(p);
+    if (!dry) (void)hipFree(p);  // teardown: nothing to report to
+}
+
 static int dev_alloc(cah_state* s, void** p, size_t n) {
+    if (s->dry) {
+        *p = lchk::dry_alloc(n, "state buffer (dry)");
+        return CAH_OK;
+    }
     if (s->arena) {
         *p = s->arena->alloc(n);
         if (!*p) return fail(CAH_ERR_OUT_OF_MEMORY, "the device arena cannot hold %zu more bytes", n);
         return CAH_OK;
     }
-    HIPCHK(hipMalloc(p, std::max<size_t>(n, 1)));
+    HIPCHK(dmalloc(p, n, "state buffer"));
     return CAH_OK;
 }
 
 static void dev_free(cah_state* s, void* p) {
     if (!p) return;
     if (s->arena) s->arena->release(p);
-    else hipFree(p);
+    else dfree(p, s->dry);
 }
 
 // The pinned sources of a server state's uploads, once those have completed.
@@ -425,7 +468,7 @@ static void release_staging(cah_state* s) {
 
 static void free_state(cah_state* s) {
     if (!s) return;
-    hipSetDevice(s->device);
+    if (!s->dry) hipSetDevice(s->device);
     if (s->staging) release_staging(s);
     if (s->stream && !s->arena) hipStreamSynchronize(s->stream);
     for (void* p : {s->buf[0], s->buf[1], (void*)s->mask_d, (void*)s->ofs_d, (void*)s->taps_d,
@@ -434,14 +477,21 @@ static void free_state(cah_state* s) {
                     (void*)s->sp_cnt, (void*)s->sp_stamp, (void*)s->sp_edgef, (void*)s->sp_edges,
                     (void*)s->sp_tmp, (void*)s->sp_act})
         dev_free(s, p);
-    lenia_fft_destroy(s->fft);
+    if (s->fft) {
+        lchk::unreg(lenia_fft_buffer(s->fft));
+        lenia_fft_destroy(s->fft);
+    }
     if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
     delete s;
 }
 
 template <typename T>
-static int upload(T** dst, const T* src, size_t n) {
-    HIPCHK(hipMalloc((void**)dst, std::max<size_t>(n, 1) * sizeof(T)));
+static int upload(T** dst, const T* src, size_t n, bool dry = false) {
+    if (dry) {
+        *dst = (T*)lchk::dry_alloc(std::max<size_t>(n, 1) * sizeof(T), "table (dry)");
+        return CAH_OK;
+    }
+    HIPCHK(dmalloc((void**)dst, std::max<size_t>(n, 1) * sizeof(T), "table"));
     if (n) HIPCHK(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
     return CAH_OK;
 }
@@ -460,7 +510,7 @@ static int stage(cah_state* s, const void* src, size_t n, void** out) {
 template <typename T>
 static int upload(cah_state* s, T** dst, const T* src, size_t n) {
     if (int rc = dev_alloc(s, (void**)dst, std::max<size_t>(n, 1) * sizeof(T))) return rc;
-    if (!n) return CAH_OK;
+    if (!n || s->dry) return CAH_OK;
     if (s->staging) {
         void* h;
         if (int rc = stage(s, src, n * sizeof(T), &h)) return rc;
@@ -476,11 +526,13 @@ static int upload(cah_state* s, T** dst, const T* src, size_t n) {
 static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* arena = nullptr,
                         Arena* staging = nullptr, hipStream_t stream = nullptr) {
     s->device = ctx->device;
-    HIPCHK(hipSetDevice(s->device));
+    if (!s->dry) HIPCHK(hipSetDevice(s->device));
     s->arena = arena;
     s->staging = staging;
     if (stream) {
         s->stream = stream;
+        s->own_stream = false;
+    } else if (s->dry) {
         s->own_stream = false;
     } else {
         HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -505,7 +557,7 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
     const size_t bytes = (size_t)s->pitch * (size_t)s->n2 * s->es;
     for (int b = 0; b < 2; ++b) {
         if (int rc = dev_alloc(s, &s->buf[b], bytes)) return rc;
-        HIPCHK(hipMemsetAsync(s->buf[b], 0, bytes, s->stream));
+        if (!s->dry) HIPCHK(hipMemsetAsync(s->buf[b], 0, bytes, s->stream));
     }
     if (d->mask) {
         int rc = upload(s, &s->mask_d, d->mask, (size_t)(s->n1 * s->n2));
@@ -535,9 +587,17 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
         const int fm = d->field_method;
         if (fm == CAH_FIELD_FFT_F32) {
             s->path = P_LENIA_FFT;
+            const size_t nspec = (size_t)(s->n1 / 2 + 1) * (size_t)s->n2;
+            if (s->dry) {
+                s->fft = lenia_fft_make_dry((int)s->n1, (int)s->n2, lchk::dry_alloc(nspec * 8, "FFT spectrum (dry)"),
+                                            lchk::dry_alloc(nspec * 8, "FFT kernel spectrum (dry)"),
+                                            (float*)lchk::dry_alloc((size_t)(s->n1 * s->n2) * 4, "FFT field (dry)"));
+                return CAH_OK;
+            }
             if (int rc = lenia_fft_create(&s->fft, (int)s->n1, (int)s->n2, s->pitch, s->taps.data(),
                                           s->weights.data(), d->n_taps, s->stream))
                 return fail(rc, "cannot set up the FFT field (hipFFT)");
+            lchk::reg(lenia_fft_buffer(s->fft), (size_t)(s->n1 * s->n2) * 4, "FFT field");
             return CAH_OK;
         }
         if ((fm == CAH_FIELD_MFMA_F16 || fm == CAH_FIELD_MFMA_BF16 || fm == CAH_FIELD_MFMA_F16X2) &&
@@ -600,10 +660,11 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
     return CAH_OK;
 }
 
-extern "C" int cah_state_create(cah_ctx* ctx, const cah_desc* d, cah_state** out) {
+static int new_state(cah_ctx* ctx, const cah_desc* d, cah_state** out, bool dry) {
     if (!ctx || !out) return fail(CAH_ERR_ARGUMENT, "cah_state_create: NULL argument");
     if (int rc = cah_check_problem(d)) return rc;
     cah_state* s = new cah_state();
+    s->dry = dry;
     int rc = create_state(ctx, d, s);
     if (rc) {
         std::string keep = g_err;
@@ -613,6 +674,10 @@ extern "C" int cah_state_create(cah_ctx* ctx, const cah_desc* d, cah_state** out
     }
     *out = s;
     return CAH_OK;
+}
+
+extern "C" int cah_state_create(cah_ctx* ctx, const cah_desc* d, cah_state** out) {
+    return new_state(ctx, d, out, false);
 }
 
 extern "C" int cah_state_destroy(cah_state* s) {
@@ -653,6 +718,12 @@ static int copy_rows(const void* src, int64_t sp, void* dst, int64_t dp, int64_t
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 16384);
     const uint8_t* s8 = (const uint8_t*)src;
     uint8_t* d8 = (uint8_t*)dst;
+    if (int rc = checked(false, "k_copy_rows", [&](lchk::Chk& c) {
+            c.need(rb % W == 0 && sp % W == 0 && dp % W == 0, "rows of %lld bytes in %d-byte pieces", (long long)rb, W);
+            c.range(src, 0, (rows - 1) * sp + rb, "source rows");
+            c.range(dst, 0, (rows - 1) * dp + rb, "destination rows");
+        }))
+        return rc;
     if (W == 16) k_copy_rows<16><<<blocks, 256, 0, st>>>(s8, sp, d8, dp, rb, rows);
     else if (W == 4) k_copy_rows<4><<<blocks, 256, 0, st>>>(s8, sp, d8, dp, rb, rows);
     else k_copy_rows<1><<<blocks, 256, 0, st>>>(s8, sp, d8, dp, rb, rows);
@@ -763,9 +834,11 @@ extern "C" int cah_state_set_device(cah_state* s, const void* dptr, size_t nbyte
     if (nbytes != state_bytes(s))
         return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
     HIPCHK(hipSetDevice(s->device));
-    if (int rc = copy_rows(dptr, (int64_t)(s->n1 * s->es), s->buf[s->cur], (int64_t)(s->pitch * s->es),
-                           (int64_t)(s->n1 * s->es), s->n2, s->stream))
-        return rc;
+    lchk::reg(dptr, nbytes, "caller's device array");
+    const int rc = copy_rows(dptr, (int64_t)(s->n1 * s->es), s->buf[s->cur], (int64_t)(s->pitch * s->es),
+                             (int64_t)(s->n1 * s->es), s->n2, s->stream);
+    lchk::unreg(dptr);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->es == 1) s->bin_clean = s->tri_clean = false;  // not scanned
     if (s->sparse) return sparse_refresh(s);
@@ -777,9 +850,11 @@ extern "C" int cah_state_get_device(cah_state* s, void* dptr, size_t nbytes) {
     if (nbytes != state_bytes(s))
         return fail(CAH_ERR_STATE_SHAPE, "state has %zu bytes, got %zu", state_bytes(s), nbytes);
     HIPCHK(hipSetDevice(s->device));
-    if (int rc = copy_rows(s->buf[s->cur], (int64_t)(s->pitch * s->es), dptr, (int64_t)(s->n1 * s->es),
-                           (int64_t)(s->n1 * s->es), s->n2, s->stream))
-        return rc;
+    lchk::reg(dptr, nbytes, "caller's device array");
+    const int rc = copy_rows(s->buf[s->cur], (int64_t)(s->pitch * s->es), dptr, (int64_t)(s->n1 * s->es),
+                             (int64_t)(s->n1 * s->es), s->n2, s->stream);
+    lchk::unreg(dptr);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
     return CAH_OK;
 }
@@ -826,6 +901,18 @@ static int queue_counts(const cah_state* s, const cah_state* o, unsigned long lo
     const uint8_t* a = (const uint8_t*)s->buf[s->cur];
     const uint8_t* b = o ? (const uint8_t*)o->buf[o->cur] : nullptr;
     const int K = rule_values(s);
+    if (int rc = checked(false, "k_counts", [&](lchk::Chk& c) {
+            // each lane reads 16 bytes from a column inside the row: to the row's
+            // n1 rounded up to 16, within the pitch
+            const int64_t end = (s->n2 - 1) * s->pitch + (s->n1 + 15) / 16 * 16;
+            c.need(s->pitch % 16 == 0, "pitch %lld", (long long)s->pitch);
+            c.range(a, 0, end, "lattice");
+            if (b) c.range(b, 0, end, "second lattice");
+            c.range(part_d, 0, 8LL * CNT_BLOCKS * CNT_PART, "partial sums");
+            c.range(out_d, 0, 8LL * CNT_SLOTS, "counters");
+            c.need(blocks >= 1 && blocks <= CNT_BLOCKS, "%d blocks", blocks);
+        }))
+        return rc;
     if (K <= 2) launch_counts<2>(blocks, a, b, s, part_d, out_d, st);
     else if (K <= 3) launch_counts<3>(blocks, a, b, s, part_d, out_d, st);
     else if (K <= 4) launch_counts<4>(blocks, a, b, s, part_d, out_d, st);
@@ -845,6 +932,11 @@ static int finish_counts(const cah_state* s, unsigned long long* out_d, unsigned
     if (sum == (unsigned long long)n) return CAH_OK;
     HIPCHK(hipMemsetAsync(out_d, 0, 256 * sizeof(unsigned long long), st));
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+    if (int rc = checked(false, "k_hist256", [&](lchk::Chk& c) {
+            lchk::lattice(c, s->buf[s->cur], lat_of(s), 1, "lattice");
+            c.range(out_d, 0, 8LL * 256, "histogram");
+        }))
+        return rc;
     hipLaunchKernelGGL(k_hist256, blocks, 256, 0, st, (const uint8_t*)s->buf[s->cur], s->pitch, s->n1, s->n2,
                        out_d);
     HIPCHK(hipGetLastError());
@@ -985,9 +1077,15 @@ static int step_life(cah_state* s) {
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
     a.rows = band_rows(s, a.nbands, s->life_R);
     a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
-    launch_life(s->life_R, vb, s->bin_clean, life_niv(a), waves_to_blocks(a.nwaves), a, s->stream);
+    const int blocks = waves_to_blocks(a.nwaves);
+    const int R = s->life_R;
+    int rc = checked(s->dry, "k_life_swar", [&](lchk::Chk& c) {
+        if (vb == 1) lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, R, 2 * R + 1);
+        else lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, R, 2 * R + 1);
+    });
+    if (!rc && !s->dry) launch_life(R, vb, s->bin_clean, life_niv(a), blocks, a, s->stream);
     s->bin_clean = saved;
-    return CAH_OK;
+    return rc;
 }
 
 // k_life_tb over several lattices (batches, the server's Moore(1) Life family).
@@ -1043,14 +1141,22 @@ static LifeTbArgs tb_item(const cah_state* s, int cur, int32_t steps) {
 static int step_life_tb(cah_state* s, int steps) {
     const LifeTbArgs a = tb_item(s, s->cur, steps);
     const int64_t cells = s->n1 * s->n2;
-    auto launch = [&](auto kernel, int txw, int ty) {
+    auto launch = [&](auto kernel, int txw, int ty) -> int {
         dim3 grid((unsigned)((s->n1 + 32 * txw - 1) / (32 * txw)), (unsigned)((s->n2 + ty - 1) / ty));
-        kernel<<<grid, 256, 0, s->stream>>>(a);
+        if (int rc = checked(s->dry, "k_life_tb", [&](lchk::Chk& c) {
+                lchk::grid_y(c, grid.y);
+                lchk::life_tb(c, a.L, a.cur, a.next, txw, ty, TB_K, a.steps, grid.x, grid.y);
+            }))
+            return rc;
+        if (!s->dry) kernel<<<grid, 256, 0, s->stream>>>(a);
+        return CAH_OK;
     };
-    if (cells <= ((int64_t)1 << 20)) launch(k_life_tb<2, 16, TB_K>, 2, 16);
-    else if (cells <= ((int64_t)1 << 24)) launch(k_life_tb<4, 32, TB_K>, 4, 32);
-    else launch(k_life_tb<16, 128, TB_K>, 16, 128);
-    HIPCHK(hipGetLastError());
+    int rc;
+    if (cells <= ((int64_t)1 << 20)) rc = launch(k_life_tb<2, 16, TB_K>, 2, 16);
+    else if (cells <= ((int64_t)1 << 24)) rc = launch(k_life_tb<4, 32, TB_K>, 4, 32);
+    else rc = launch(k_life_tb<16, 128, TB_K>, 16, 128);
+    if (rc) return rc;
+    if (!s->dry) HIPCHK(hipGetLastError());
     s->cur = 1 - s->cur;
     return CAH_OK;
 }
@@ -1091,7 +1197,12 @@ static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
         const double v = s->d.boundary_value[k];
         return s->d.boundary[k] != CAH_BOUNDARY_CONSTANT || v == 0 || v == 1 || v == 2;
     };
-    launch_forest(vb, small, s->tri_clean && edge012(0) && edge012(1), blocks, a, s->stream);
+    if (int rc = checked(s->dry, "k_forest_vn1", [&](lchk::Chk& c) {
+            if (vb == 1) lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, 1, 3);
+            else lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, 1, 3);
+        }))
+        return rc;
+    if (!s->dry) launch_forest(vb, small, s->tri_clean && edge012(0) && edge012(1), blocks, a, s->stream);
     s->tri_clean = true;  // the rule writes 0, 1 and 2 only
     return CAH_OK;
 }
@@ -1137,11 +1248,29 @@ static GenArgs generic_args(cah_state* s, int64_t t, int gate, const cah_params*
     return a;
 }
 
+// The checks of a general sweep's arguments beyond its tile: the neighbourhood
+// within the halo, the offsets, mask, field and covariate buffers, the output.
+static void check_generic_args(lchk::Chk& c, const cah_state* s, const GenArgs& a) {
+    const int64_t cells = s->n1 * s->n2;
+    lchk::offsets_within(c, s->offsets.data(), a.nofs, a.h, "neighbourhood");
+    if (a.nofs) c.range(a.ofs, 0, 8LL * a.nofs, "offsets");
+    lchk::lattice(c, a.next, a.L, 1, "state written");
+    if (a.mask) c.range(a.mask, 0, cells, "mask");
+    if (a.field) c.range(a.field, 0, cells * (a.field32 ? 4 : 8), "field");
+    if (a.cov) c.range(a.cov, 0, cells * 8 * (int64_t)a.cov_slices, "covariate");
+}
+
 static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p) {
     const GenArgs a = generic_args(s, t, gate, p);
     dim3 grid((unsigned)((s->n1 + GEN_TX - 1) / GEN_TX), (unsigned)((s->n2 + GEN_TY - 1) / GEN_TY));
     const size_t lds = (size_t)(GEN_TX + 2 * a.h) * (GEN_TY + 2 * a.h);
-    k_generic_i8<<<grid, 256, lds, s->stream>>>(a);
+    if (int rc = checked(s->dry, "k_generic_i8", [&](lchk::Chk& c) {
+            lchk::grid_y(c, grid.y);
+            lchk::tile_kernel(c, a.L, a.cur, 1, GEN_TX, GEN_TY, a.h, grid.x, grid.y, lds);
+            check_generic_args(c, s, a);
+        }))
+        return rc;
+    if (!s->dry) k_generic_i8<<<grid, 256, lds, s->stream>>>(a);
     return CAH_OK;
 }
 
@@ -1179,6 +1308,16 @@ static int sparse_refresh(cah_state* s) {
     HIPCHK(hipMemsetAsync(s->sp_cnt, 0, 6 * sizeof(int32_t), s->stream));
     HIPCHK(hipMemsetAsync(s->sp_stamp, 0xFF, (size_t)nt * sizeof(int32_t), s->stream));  // -1
     dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
+    if (int rc = checked(false, "k_sparse_flags", [&](lchk::Chk& c) {
+            lchk::lattice(c, s->buf[s->cur], lat_of(s), 1, "state");
+            c.range(s->sp_tmp, 0, nt, "tile flags");
+            c.range(s->sp_act, 0, 4LL * nt, "tile list");
+            c.range(s->sp_edges, 0, 4LL * s->sp_nedge, "edge tiles");
+            c.range(s->sp_cand[0], 0, 4LL * nt, "candidate list");
+            c.range(s->sp_stamp, 0, 4LL * nt, "stamps");
+            c.range(s->sp_cnt, 0, 24, "counters");
+        }))
+        return rc;
     k_sparse_flags<<<grid, 256, 0, s->stream>>>((const int8_t*)s->buf[s->cur], lat_of(s), s->sp_ts, s->sp_tsy,
                                                 s->sp_ntx, s->sp_bg, s->sp_tmp);
     k_sparse_compact<<<(unsigned)((nt + 255) / 256), 256, 0, s->stream>>>(s->sp_tmp, nt, s->sp_act, s->sp_cnt + 5);
@@ -1315,6 +1454,17 @@ static int sparse_step(cah_state* s, int64_t t, const cah_params* p) {
     const size_t lds = (size_t)(s->sp_ts + 2 * a.h) * (size_t)(s->sp_tsy + 2 * a.h);
     // persistent blocks: the number of listed tiles is known only on the device
     const int blocks = (int)std::min<int64_t>((int64_t)s->sp_ntx * s->sp_nty, 4 * device_cus(s->device));
+    if (int rc = checked(false, "k_sparse_sweep", [&](lchk::Chk& c) {
+            const int64_t nt = (int64_t)s->sp_ntx * s->sp_nty;
+            c.need((int64_t)s->sp_ntx * s->sp_ts >= s->n1 && (int64_t)s->sp_nty * s->sp_tsy >= s->n2,
+                   "tiles do not cover the lattice");
+            lchk::tile_kernel(c, a.L, a.cur, 1, s->sp_ts, s->sp_tsy, a.h, s->sp_ntx, s->sp_nty, lds);
+            check_generic_args(c, s, a);
+            for (const int32_t* l : {sa.list, (const int32_t*)sa.list_next, (const int32_t*)sa.stamp})
+                c.range(l, 0, 4 * nt, "tile list");
+            c.range(sa.edge, 0, nt, "edge flags");
+        }))
+        return rc;
     k_sparse_sweep<<<blocks, 256, lds, s->stream>>>(a, sa, s->sp_ts, s->sp_tsy, s->sp_bg);
     HIPCHK(hipGetLastError());
     ++s->sp_step;
@@ -1323,18 +1473,30 @@ static int sparse_step(cah_state* s, int64_t t, const cah_params* p) {
 }
 
 // The field epidemic's kernel field of the step-start state, read by every gate.
-static void launch_epi_field(cah_state* s) {
+static int launch_epi_field(cah_state* s) {
     const Lat L = lat_of(s);
     dim3 grid((unsigned)((s->n1 + EPI_TX - 1) / EPI_TX), (unsigned)((s->n2 + EPI_TY - 1) / EPI_TY));
     const size_t lds = (size_t)(EPI_TX + 2 * s->epi_h) * (EPI_TY + 2 * s->epi_h);
     const int8_t* cur = (const int8_t*)s->buf[s->cur];
     const int8_t cv1 = (int8_t)s->d.boundary_value[0], cv2 = (int8_t)s->d.boundary_value[1];
+    const int64_t es = s->d.lenia_precision == 32 ? 4 : 8;
+    if (int rc = checked(s->dry, "k_epi_field", [&](lchk::Chk& c) {
+            lchk::grid_y(c, grid.y);
+            lchk::tile_kernel(c, L, cur, 1, EPI_TX, EPI_TY, s->epi_h, grid.x, grid.y, lds);
+            lchk::offsets_within(c, s->taps.data(), s->d.n_taps, s->epi_h, "kernel field taps");
+            c.range(s->taps_d, 0, 8LL * s->d.n_taps, "taps");
+            c.range(es == 4 ? (const void*)s->w32_d : (const void*)s->w64_d, 0, es * s->d.n_taps, "weights");
+            c.range(s->field_d, 0, es * s->n1 * s->n2, "field");
+        }))
+        return rc;
+    if (s->dry) return CAH_OK;
     if (s->d.lenia_precision == 32)
         k_epi_field<float><<<grid, 256, lds, s->stream>>>(cur, L, cv1, cv2, s->taps_d, s->w32_d, s->d.n_taps,
                                                           s->epi_h, (float*)s->field_d);
     else
         k_epi_field<double><<<grid, 256, lds, s->stream>>>(cur, L, cv1, cv2, s->taps_d, s->w64_d, s->d.n_taps,
                                                            s->epi_h, (double*)s->field_d);
+    return CAH_OK;
 }
 
 static LeniaArgs lenia_args(cah_state* s, int64_t t, int gate) {
@@ -1368,6 +1530,18 @@ static int launch_lenia_f32col(cah_state* s, int64_t t) {
     LeniaArgs a = lenia_args(s, t, 0);
     a.w32 = s->wtab_d;
     dim3 grid((unsigned)((s->n1 + 63) / 64), (unsigned)((s->n2 + 31) / 32));
+    const int H = s->lenia_H;
+    if (int rc = checked(s->dry, "k_lenia_f32col", [&](lchk::Chk& c) {
+            c.need(H == 4 || H == 8 || H == 12 || H == 16, "H = %d", H);
+            lchk::grid_y(c, grid.y);
+            lchk::tile_kernel(c, a.L, a.cur, 4, 64, 32, H, grid.x, grid.y, (size_t)(64 + 2 * H) * (32 + 2 * H) * 4);
+            lchk::offsets_within(c, s->taps.data(), s->d.n_taps, H, "kernel taps");
+            c.range(a.w32, 0, 4LL * (2 * H + 1) * (2 * H + 1), "weight table");
+            lchk::lattice(c, a.next, a.L, 4, "state written");
+            if (a.mask) c.range(a.mask, 0, s->n1 * s->n2, "mask");
+        }))
+        return rc;
+    if (s->dry) return CAH_OK;
     switch (s->lenia_H) {
     case 4: k_lenia_f32col<4><<<grid, 256, 0, s->stream>>>(a); break;
     case 8: k_lenia_f32col<8><<<grid, 256, 0, s->stream>>>(a); break;
@@ -1381,6 +1555,17 @@ static int launch_lenia_field_seq(cah_state* s, int64_t t) {
     LeniaArgs a = lenia_args(s, t, 0);
     dim3 grid((unsigned)((s->n1 + LEN_TX - 1) / LEN_TX), (unsigned)((s->n2 + LEN_TY - 1) / LEN_TY));
     const size_t lds = (size_t)(LEN_TX + 2 * a.h) * (LEN_TY + 2 * a.h) * sizeof(float);
+    if (int rc = checked(s->dry, "k_lenia_field_seq", [&](lchk::Chk& c) {
+            const int64_t es = a.prec32 ? 4 : 8;
+            lchk::grid_y(c, grid.y);
+            lchk::tile_kernel(c, a.L, a.cur, 4, LEN_TX, LEN_TY, a.h, grid.x, grid.y, lds);
+            lchk::offsets_within(c, s->taps.data(), a.ntaps, a.h, "kernel taps");
+            c.range(a.taps, 0, 8LL * a.ntaps, "taps");
+            c.range(a.prec32 ? (const void*)a.w32 : (const void*)a.w64, 0, es * a.ntaps, "weights");
+            c.range(a.field, 0, es * s->n1 * s->n2, "field");
+        }))
+        return rc;
+    if (s->dry) return CAH_OK;
     if (a.prec32) k_lenia_field_seq<float><<<grid, 256, lds, s->stream>>>(a);
     else k_lenia_field_seq<double><<<grid, 256, lds, s->stream>>>(a);
     return CAH_OK;
@@ -1393,7 +1578,16 @@ static int launch_lenia_growth(cah_state* s, int64_t t, int gate) {
         a.field32 = 1;
     }
     dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
-    k_lenia_growth<<<grid, 256, 0, s->stream>>>(a);
+    if (int rc = checked(s->dry, "k_lenia_growth", [&](lchk::Chk& c) {
+            c.need((int64_t)grid.x * 256 >= s->n1, "grid.x = %u", grid.x);
+            lchk::grid_y(c, grid.y);
+            lchk::lattice(c, a.cur, a.L, 4, "state read");
+            lchk::lattice(c, a.next, a.L, 4, "state written");
+            c.range(a.field, 0, (a.prec32 || a.field32 ? 4 : 8) * s->n1 * s->n2, "field");
+            if (a.mask) c.range(a.mask, 0, s->n1 * s->n2, "mask");
+        }))
+        return rc;
+    if (!s->dry) k_lenia_growth<<<grid, 256, 0, s->stream>>>(a);
     return CAH_OK;
 }
 
@@ -1417,7 +1611,12 @@ static int launch_lenia_mfma_step(cah_state* s, int64_t t) {
     a.sp = s->d.scheme_p;
     a.sp_f32 = s->d.scheme_p_is_f32;
     dim3 grid((unsigned)((s->n1 + LM_BX - 1) / LM_BX), (unsigned)((s->n2 + LM_BY - 1) / LM_BY));
-    launch_lenia_mfma(s->lenia_mode, grid, a, s->stream);
+    if (int rc = checked(s->dry, "k_lenia_mfma", [&](lchk::Chk& c) {
+            lchk::grid_y(c, grid.y);
+            lchk::lenia_mfma(c, a, s->lenia_mode, grid.x, grid.y);
+        }))
+        return rc;
+    if (!s->dry) launch_lenia_mfma(s->lenia_mode, grid, a, s->stream);
     return CAH_OK;
 }
 
@@ -1430,7 +1629,7 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
         return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
     if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC && !p)
         return fail(CAH_ERR_ARGUMENT, "the field epidemic needs its parameters (beta, gamma)");
-    HIPCHK(hipSetDevice(s->device));
+    if (!s->dry) HIPCHK(hipSetDevice(s->device));
     if (s->sparse) {
         if (!quiescent(s, s->sp_bg, p))
             return fail(CAH_ERR_ARGUMENT,
@@ -1455,24 +1654,35 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
     }
     for (int64_t k = 0; k < nsteps; ++k) {
         const int64_t t = t0 + k;
-        if (s->path == P_LENIA_SEQ) launch_lenia_field_seq(s, t);
-        if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC) launch_epi_field(s);
-        if (s->path == P_LENIA_FFT && lenia_fft_field(s->fft, (const float*)s->buf[s->cur], s->stream))
-            return fail(CAH_ERR_DEVICE, "hipFFT failed in the Lenia field");
+        if (s->path == P_LENIA_SEQ)
+            if (int rc = launch_lenia_field_seq(s, t)) return rc;
+        if (s->d.rule == CAH_RULE_FIELD_EPIDEMIC)
+            if (int rc = launch_epi_field(s)) return rc;
+        if (s->path == P_LENIA_FFT) {
+            // the R2C transform reads the pitched state (n1 of every pitch floats)
+            if (int rc = checked(s->dry, "hipfftExecR2C", [&](lchk::Chk& c) {
+                    lchk::lattice(c, s->buf[s->cur], lat_of(s), 4, "state transformed");
+                }))
+                return rc;
+            if (!s->dry && lenia_fft_field(s->fft, (const float*)s->buf[s->cur], s->stream))
+                return fail(CAH_ERR_DEVICE, "hipFFT failed in the Lenia field");
+        }
         for (int g = 0; g < ngates(s->d); ++g) {
+            int rc = CAH_OK;
             switch (s->path) {
-            case P_LIFE_SWAR: step_life(s); break;
-            case P_FOREST: step_forest(s, t, p); break;
+            case P_LIFE_SWAR: rc = step_life(s); break;
+            case P_FOREST: rc = step_forest(s, t, p); break;
             case P_LENIA_SEQ:
-            case P_LENIA_FFT: launch_lenia_growth(s, t, g); break;
-            case P_LENIA_F32COL: launch_lenia_f32col(s, t); break;
-            case P_LENIA_MFMA: launch_lenia_mfma_step(s, t); break;
-            default: launch_generic(s, t, g, p); break;
+            case P_LENIA_FFT: rc = launch_lenia_growth(s, t, g); break;
+            case P_LENIA_F32COL: rc = launch_lenia_f32col(s, t); break;
+            case P_LENIA_MFMA: rc = launch_lenia_mfma_step(s, t); break;
+            default: rc = launch_generic(s, t, g, p); break;
             }
+            if (rc) return rc;
             s->cur = 1 - s->cur;
         }
     }
-    HIPCHK(hipGetLastError());
+    if (!s->dry) HIPCHK(hipGetLastError());
     // Life maps {0,1} to {0,1}; any other rule may leave other values
     if (s->d.rule != CAH_RULE_LIFE && nsteps > 0) s->bin_clean = false;
     return CAH_OK;
@@ -1485,6 +1695,10 @@ extern "C" int cah_state_fill_bernoulli(cah_state* s, uint64_t seed, double p) {
     if (s->es != 1) return fail(CAH_ERR_ARGUMENT, "fill_bernoulli needs an Int8 state");
     HIPCHK(hipSetDevice(s->device));
     dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
+    if (int rc = checked(false, "k_fill_bernoulli", [&](lchk::Chk& c) {
+            lchk::lattice(c, s->buf[s->cur], lat_of(s), 1, "state");
+        }))
+        return rc;
     k_fill_bernoulli<<<grid, 256, 0, s->stream>>>((int8_t*)s->buf[s->cur], lat_of(s), seed, p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1499,6 +1713,10 @@ extern "C" int cah_state_fill_uniform_box(cah_state* s, uint64_t seed, int64_t l
     if (s->es != 4) return fail(CAH_ERR_ARGUMENT, "fill_uniform_box needs a Float32 state");
     HIPCHK(hipSetDevice(s->device));
     dim3 grid((unsigned)((s->n1 + 255) / 256), (unsigned)std::min<int64_t>(s->n2, 65535));
+    if (int rc = checked(false, "k_fill_uniform_box", [&](lchk::Chk& c) {
+            lchk::lattice(c, s->buf[s->cur], lat_of(s), 4, "state");
+        }))
+        return rc;
     k_fill_uniform_box<<<grid, 256, 0, s->stream>>>((float*)s->buf[s->cur], lat_of(s), seed,
                                                     (int32_t)lo1, (int32_t)len1, (int32_t)lo2,
                                                     (int32_t)len2);
@@ -1541,16 +1759,24 @@ struct cah_batch {
     LifeTbArgs* tb_items_d = nullptr;
     int32_t* tb_item_d = nullptr;
     int32_t* tb_base_d = nullptr;
+    // host copies of the tables, for the launch checks (made once per layout and
+    // buffer parity)
+    std::vector<LifeArgs> litems;
+    std::vector<int32_t> wave_item_h, wave_base_h;
+    std::vector<LifeTbArgs> tb_items_h;
+    std::vector<int32_t> tb_item_h, tb_base_h;
+    bool checked_parity[2] = {false, false};
+    bool dry = false;  // members are dry states: launches are checked, not run
 };
 
 static void free_batch(cah_batch* b) {
     if (!b) return;
-    hipSetDevice(b->device);
+    if (!b->dry) hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
     for (void* p : {b->items_d, (void*)b->wave_item_d, (void*)b->wave_base_d, (void*)b->counts_d,
                     (void*)b->tb_items_d, (void*)b->tb_item_d, (void*)b->tb_base_d})
-        if (p) hipFree(p);
+        dfree(p, b->dry);
     for (auto* m : b->members) free_state(m);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -1605,14 +1831,18 @@ static int batch_layout(cah_batch* b) {
         }
     }
     b->nwaves = (int32_t)wave_item.size();
+    b->checked_parity[0] = b->checked_parity[1] = false;
     if (b->kind == B_FOREST) {
         b->fitems = fitems;
-        if (int rc = upload((ForestArgs**)&b->items_d, fitems.data(), fitems.size())) return rc;
+        if (int rc = upload((ForestArgs**)&b->items_d, fitems.data(), fitems.size(), b->dry)) return rc;
     } else {
-        if (int rc = upload((LifeArgs**)&b->items_d, litems.data(), litems.size())) return rc;
+        b->litems = litems;
+        if (int rc = upload((LifeArgs**)&b->items_d, litems.data(), litems.size(), b->dry)) return rc;
     }
-    if (int rc = upload(&b->wave_item_d, wave_item.data(), wave_item.size())) return rc;
-    if (int rc = upload(&b->wave_base_d, wave_base.data(), wave_base.size())) return rc;
+    if (int rc = upload(&b->wave_item_d, wave_item.data(), wave_item.size(), b->dry)) return rc;
+    if (int rc = upload(&b->wave_base_d, wave_base.data(), wave_base.size(), b->dry)) return rc;
+    b->wave_item_h = wave_item;
+    b->wave_base_h = wave_base;
     if (b->kind == B_LIFE && b->life_R == 1) {  // several steps a launch (k_life_tb)
         std::vector<const cah_state*> m(b->members.begin(), b->members.end());
         b->tb = true;
@@ -1625,30 +1855,37 @@ static int batch_layout(cah_batch* b) {
             block_item.insert(block_item.end(), (size_t)tb_tiles(m[k], b->tb_shape), (int32_t)k);
         }
         b->tb_blocks = (int32_t)block_item.size();
-        if (int rc = upload(&b->tb_items_d, items.data(), items.size())) return rc;
-        if (int rc = upload(&b->tb_item_d, block_item.data(), block_item.size())) return rc;
-        if (int rc = upload(&b->tb_base_d, block_base.data(), block_base.size())) return rc;
+        if (int rc = upload(&b->tb_items_d, items.data(), items.size(), b->dry)) return rc;
+        if (int rc = upload(&b->tb_item_d, block_item.data(), block_item.size(), b->dry)) return rc;
+        if (int rc = upload(&b->tb_base_d, block_base.data(), block_base.size(), b->dry)) return rc;
+        b->tb_items_h = items;
+        b->tb_item_h = block_item;
+        b->tb_base_h = block_base;
     }
     return CAH_OK;
 }
 
-extern "C" int cah_batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, cah_batch** out) {
+static int new_state(cah_ctx* ctx, const cah_desc* d, cah_state** out, bool dry);
+
+static int batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, cah_batch** out, bool dry) {
     if (!ctx || !descs || !out || n < 1) return fail(CAH_ERR_ARGUMENT, "cah_batch_create: bad arguments");
     cah_batch* b = new cah_batch();
     b->device = ctx->device;
+    b->dry = dry;
     auto bail = [&](int rc) {
         std::string keep = g_err;
         free_batch(b);
         g_err = keep;
         return rc;
     };
-    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    if (!dry && (hipSetDevice(b->device) != hipSuccess ||
+                 hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess))
         return bail(fail(CAH_ERR_DEVICE, "cannot create the batch stream"));
     for (int32_t i = 0; i < n; ++i) {
         if (descs[i].rule != descs[0].rule || descs[i].dtype != CAH_DTYPE_I8)
             return bail(fail(CAH_ERR_ARGUMENT, "a batch holds trajectories of one Int8 rule"));
         cah_state* s = nullptr;
-        if (int rc = cah_state_create(ctx, &descs[i], &s)) return bail(rc);
+        if (int rc = new_state(ctx, &descs[i], &s, dry)) return bail(rc);
         b->members.push_back(s);
     }
     bool all_forest = true, all_life = true;
@@ -1671,9 +1908,14 @@ extern "C" int cah_batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, 
     }
     if (b->kind != B_LOOP)
         if (int rc = batch_layout(b)) return bail(rc);
-    for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
+    if (!dry)
+        for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
     *out = b;
     return CAH_OK;
+}
+
+extern "C" int cah_batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, cah_batch** out) {
+    return batch_create(ctx, n, descs, out, false);
 }
 
 static int member(cah_batch* b, int32_t i, cah_state** s) {
@@ -1704,7 +1946,7 @@ extern "C" int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* c
     HIPCHK(hipSetDevice(b->device));
     if (other && other->stream != b->stream) HIPCHK(hipStreamSynchronize(other->stream));
     const size_t n = b->members.size();
-    if (!b->counts_d) HIPCHK(hipMalloc((void**)&b->counts_d, counts_words(n) * sizeof(unsigned long long)));
+    if (!b->counts_d) HIPCHK(dmalloc((void**)&b->counts_d, counts_words(n) * sizeof(unsigned long long), "batch counters"));
     // every member's counts on the batch stream, then one copy back
     for (size_t i = 0; i < n; ++i)
         if (int rc = queue_counts(b->members[i], other, b->counts_d + i * CNT_SLOTS, b->counts_d + n * CNT_SLOTS,
@@ -1733,8 +1975,62 @@ static int forest_prec(const cah_params& p) { return (p.ff_p_is_f32 ? 1 : 0) | (
 
 // clean: every lattice and Constant edge holds only 0 and 1 (Life) or only 0, 1
 // and 2 (the forest fire), which lets the kernels skip the exact byte tests.
+// The checks of a batch's launches at buffer parity `par` (every member's
+// geometry, both kernels' tables), once per layout and parity.
+static int check_batch(cah_batch* b, int par) {
+    if ((!b->dry && !lchk::enabled()) || b->checked_parity[par]) return CAH_OK;
+    const int64_t n = (int64_t)b->members.size();
+    auto swap = [&](auto a) {
+        if (par) {
+            const auto c = a.cur;
+            a.cur = a.next;
+            a.next = const_cast<decltype(a.next)>(c);
+        }
+        return a;
+    };
+    if (b->tb) {
+        const TbShape sh = TB_SHAPES[b->tb_shape];
+        if (int rc = checked(b->dry, "k_life_tb_batch", [&](lchk::Chk& c) {
+                lchk::item_table(c, b->tb_item_h.data(), b->tb_base_h.data(), b->tb_blocks, n,
+                                 [&](int64_t k) { return (int64_t)tb_tiles(b->members[k], b->tb_shape); });
+                for (int64_t k = 0; k < n && c.ok(); ++k) {
+                    const LifeTbArgs a = swap(b->tb_items_h[k]);
+                    const int64_t gx = (a.L.n1 + 32 * sh.txw - 1) / (32 * sh.txw), gy = (a.L.n2 + sh.ty - 1) / sh.ty;
+                    lchk::life_tb(c, a.L, a.cur, a.next, sh.txw, sh.ty, TB_K, 1, gx, gy);
+                }
+            }))
+            return rc;
+    }
+    if (int rc = checked(b->dry, b->kind == B_FOREST ? "k_forest_batch" : "k_life_batch", [&](lchk::Chk& c) {
+            lchk::item_table(c, b->wave_item_h.data(), b->wave_base_h.data(), b->nwaves, n, [&](int64_t k) {
+                return (int64_t)(b->kind == B_FOREST ? b->fitems[k].nwaves : b->litems[k].nwaves);
+            });
+            for (int64_t k = 0; k < n && c.ok(); ++k) {
+                const int R = b->kind == B_FOREST ? 1 : b->life_R;
+                if (b->kind == B_FOREST) {
+                    const ForestArgs a = swap(b->fitems[k]);
+                    if (b->vb == 1) lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, 1, 3);
+                    else lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, 1, 3);
+                } else {
+                    const LifeArgs a = swap(b->litems[k]);
+                    if (b->vb == 1)
+                        lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, R, 2 * R + 1);
+                    else
+                        lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, R, 2 * R + 1);
+                }
+            }
+        }))
+        return rc;
+    b->checked_parity[par] = true;
+    return CAH_OK;
+}
+
 static void launch_batch_step(cah_batch* b, int64_t t, bool clean) {
     const int blocks = waves_to_blocks(b->nwaves);
+    if (b->dry) {
+        b->parity ^= 1;
+        return;
+    }
     if (b->kind == B_FOREST)
         launch_forest_batch(b->vb, b->small, clean, forest_prec(b->fp), blocks, (const ForestArgs*)b->items_d,
                             b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, b->stream);
@@ -1748,7 +2044,7 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
                                  int32_t use_graph) {
     if (!b) return fail(CAH_ERR_ARGUMENT, "batch is NULL");
     if (nsteps < 0 || t0 < 0) return fail(CAH_ERR_ARGUMENT, "t0 and nsteps must be >= 0");
-    HIPCHK(hipSetDevice(b->device));
+    if (!b->dry) HIPCHK(hipSetDevice(b->device));
     if (b->kind == B_LOOP) {
         for (auto* s : b->members)
             if (int rc = cah_advance(s, t0, nsteps, p)) return rc;
@@ -1757,21 +2053,23 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
     if (b->kind == B_FOREST && !p)
         return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
     // members may have been set through their own streams
-    for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
+    if (!b->dry)
+        for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
     if (b->kind == B_FOREST &&
         (!b->fp_set || b->fp.ff_p != p->ff_p || b->fp.ff_f != p->ff_f ||
          b->fp.ff_p_is_f32 != p->ff_p_is_f32 || b->fp.ff_f_is_f32 != p->ff_f_is_f32)) {
         // the items carry p and f; the host copy is rewritten only once no
         // queued copy still reads it
-        HIPCHK(hipStreamSynchronize(b->stream));
+        if (!b->dry) HIPCHK(hipStreamSynchronize(b->stream));
         for (ForestArgs& a : b->fitems) {
             a.p = p->ff_p;
             a.f = p->ff_f;
             a.p_f32 = p->ff_p_is_f32;
             a.f_f32 = p->ff_f_is_f32;
         }
-        HIPCHK(hipMemcpyAsync(b->items_d, b->fitems.data(), b->fitems.size() * sizeof(ForestArgs),
-                              hipMemcpyHostToDevice, b->stream));
+        if (!b->dry)
+            HIPCHK(hipMemcpyAsync(b->items_d, b->fitems.data(), b->fitems.size() * sizeof(ForestArgs),
+                                  hipMemcpyHostToDevice, b->stream));
         b->fp = *p;
         b->fp_set = true;
     }
@@ -1787,18 +2085,22 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
     if (b->kind == B_FOREST) bin = tri;  // the forest kernels' short byte tests
     // Moore(1) Life: up to TB_K steps a launch (the items run TB_K - step0 steps),
     // a last single step through the band walk, as a single lattice advances
+    // the launch checks of both buffer parities, before anything is queued
+    for (int par = 0; par < 2; ++par)
+        if (int rc = check_batch(b, par)) return rc;
     auto queue = [&]() {
         int64_t k = 0;
         if (b->tb)
             for (; nsteps - k >= 2; b->parity ^= 1) {
                 const int32_t n = (int32_t)std::min<int64_t>(nsteps - k, TB_K);
-                launch_life_tb_batch(b->tb_shape, b->tb_blocks, b->tb_items_d, b->tb_item_d, b->tb_base_d,
-                                     b->parity, TB_K - n, b->stream);
+                if (!b->dry)
+                    launch_life_tb_batch(b->tb_shape, b->tb_blocks, b->tb_items_d, b->tb_item_d, b->tb_base_d,
+                                         b->parity, TB_K - n, b->stream);
                 k += n;
             }
         for (; k < nsteps; ++k) launch_batch_step(b, t0 + k, bin);
     };
-    if (use_graph && nsteps > 1) {
+    if (use_graph && nsteps > 1 && !b->dry) {
         if (b->exec) {
             HIPCHK(hipStreamSynchronize(b->stream));
             HIPCHK(hipGraphExecDestroy(b->exec));
@@ -1814,7 +2116,7 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
     } else {
         queue();
     }
-    HIPCHK(hipGetLastError());
+    if (!b->dry) HIPCHK(hipGetLastError());
     for (auto* s : b->members) {
         s->cur = b->parity;
         if (b->kind == B_FOREST && nsteps > 0) {
@@ -1828,11 +2130,11 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
 
 // The band walk's tables, freed before batch_layout builds them again.
 static int free_layout(cah_batch* b) {
-    HIPCHK(hipStreamSynchronize(b->stream));
+    if (!b->dry) HIPCHK(hipStreamSynchronize(b->stream));
     if (b->exec) HIPCHK(hipGraphExecDestroy(b->exec));
     b->exec = nullptr;
     for (void** q : {&b->items_d, (void**)&b->wave_item_d, (void**)&b->wave_base_d}) {
-        if (*q) HIPCHK(hipFree(*q));
+        dfree(*q, b->dry);
         *q = nullptr;
     }
     b->fp_set = false;  // the new forest items carry no p and f yet
@@ -1844,7 +2146,7 @@ extern "C" int cah_batch_tune(cah_batch* b, int32_t vb, int32_t rows) {
     if (b->kind == B_LOOP || b->tb) return CAH_OK;  // no band walk
     if (b->kind == B_LIFE && b->life_R == 5 && vb == 1)
         return fail(CAH_ERR_ARGUMENT, "radius-5 Life needs 8 cells a lane (vb = 2)");
-    HIPCHK(hipSetDevice(b->device));
+    if (!b->dry) HIPCHK(hipSetDevice(b->device));
     b->vb = vb ? vb : (b->kind == B_LIFE && b->life_R == 5 ? 2 : 1);
     b->rows = rows;
     if (int rc = free_layout(b)) return rc;
@@ -1967,6 +2269,52 @@ extern "C" int cah_batch_stream(cah_batch* b, void** stream) {
 extern "C" int cah_batch_destroy(cah_batch* b) {
     free_batch(b);
     return CAH_OK;
+}
+
+// ---------------------------------------------------------------- launch checks without a device
+//
+// A dry state or batch has fake device addresses and makes no HIP call: its
+// advance runs the launch sequence through the launch checks only.
+
+static cah_ctx g_dry_ctx{0};
+
+static void tally_report(char* report, size_t cap) {
+    if (!report || !cap) return;
+    std::string r;
+    for (const auto& kv : g_tally) {
+        char line[256];
+        snprintf(line, sizeof line, "%s: %lld launches, %lld checks\n", kv.first.c_str(), (long long)kv.second.first,
+                 (long long)kv.second.second);
+        r += line;
+    }
+    snprintf(report, cap, "%s", r.c_str());
+}
+
+extern "C" int cah_check_launches(const cah_desc* d, int64_t t0, int64_t nsteps, const cah_params* p, char* report,
+                                  size_t cap) {
+    g_tally.clear();
+    cah_state* s = nullptr;
+    int rc = new_state(&g_dry_ctx, d, &s, true);
+    if (!rc) rc = cah_advance(s, t0, nsteps, p);
+    const std::string keep = g_err;
+    free_state(s);
+    g_err = keep;
+    tally_report(report, cap);
+    return rc;
+}
+
+extern "C" int cah_check_batch_launches(int32_t n, const cah_desc* descs, int64_t t0, int64_t nsteps,
+                                        const cah_params* p, int32_t vb, int32_t rows, char* report, size_t cap) {
+    g_tally.clear();
+    cah_batch* b = nullptr;
+    int rc = batch_create(&g_dry_ctx, n, descs, &b, true);
+    if (!rc && (vb || rows)) rc = cah_batch_tune(b, vb, rows);
+    if (!rc) rc = cah_batch_advance(b, t0, nsteps, p, 0);
+    const std::string keep = g_err;
+    free_batch(b);
+    g_err = keep;
+    tally_report(report, cap);
+    return rc;
 }
 
 // ---------------------------------------------------------------- serving
@@ -2501,6 +2849,62 @@ static int build_tick(cah_server* s) {
         std::memcpy(sl.host + g.off_map, maps[gi].data(), 4 * maps[gi].size());
         std::memcpy(sl.host + g.off_base, g.base.data(), 4 * g.base.size());
         ++gi;
+    }
+    if (lchk::enabled()) {
+        // every group's tables and every item's geometry, at both buffer parities
+        // (the launch of step k swaps an item's buffers when k is odd)
+        size_t gj = 0;
+        for (auto& kv : groups) {
+            SrvGroup& g = kv.second;
+            const int64_t n = (int64_t)g.reqs.size();
+            const int32_t* map = maps[gj++].data();
+            const char* name = g.tb ? "k_life_tb_batch (server)" : g.cls == SC_LIFE ? "k_life_batch (server)"
+                               : g.cls == SC_FOREST ? "k_forest_batch (server)" : "k_lenia_mfma_batch (server)";
+            int rc = checked(false, name, [&](lchk::Chk& c) {
+                lchk::item_table(c, map, g.base.data(), g.base.back(), n,
+                                 [&](int64_t k) { return (int64_t)(g.base[k + 1] - g.base[k]); });
+                for (int64_t k = 0; k < n && c.ok(); ++k)
+                    for (int par = 0; par < 2; ++par) {
+                        const char* item = sl.host + g.off_items;
+                        auto swap = [&](auto a) {
+                            if (par) {
+                                const auto q = a.cur;
+                                a.cur = a.next;
+                                a.next = const_cast<decltype(a.next)>(q);
+                            }
+                            return a;
+                        };
+                        if (g.tb) {
+                            LifeTbArgs a;
+                            std::memcpy(&a, item + k * sizeof a, sizeof a);
+                            a = swap(a);
+                            const TbShape sh = TB_SHAPES[g.shape];
+                            lchk::life_tb(c, a.L, a.cur, a.next, sh.txw, sh.ty, TB_K, std::min(a.steps, TB_K),
+                                          (a.L.n1 + 32 * sh.txw - 1) / (32 * sh.txw), (a.L.n2 + sh.ty - 1) / sh.ty);
+                        } else if (g.cls == SC_LIFE) {
+                            LifeArgs a;
+                            std::memcpy(&a, item + k * sizeof a, sizeof a);
+                            a = swap(a);
+                            const int R = g.key / 4;
+                            if (R == 5)
+                                lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, R, 2 * R + 1);
+                            else
+                                lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, R, 2 * R + 1);
+                        } else if (g.cls == SC_FOREST) {
+                            ForestArgs a;
+                            std::memcpy(&a, item + k * sizeof a, sizeof a);
+                            a = swap(a);
+                            lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, 1, 3);
+                        } else {
+                            LeniaMfmaArgs a;
+                            std::memcpy(&a, item + k * sizeof a, sizeof a);
+                            a = swap(a);
+                            lchk::lenia_mfma(c, a, g.key, a.nbx, (a.L.n2 + LM_BY - 1) / LM_BY);
+                        }
+                    }
+            });
+            if (rc) return rc;
+        }
     }
     if (off) HIPCHK(hipMemcpyAsync(sl.dev, sl.host, off, hipMemcpyHostToDevice, s->up));
     HIPCHK(hipEventRecord(sl.up, s->up));

@@ -6,6 +6,8 @@
 #include "../../include/cahip.h"
 
 #define CAH_DEV __device__ __forceinline__
+// the index arithmetic that the host-side launch checks share with the kernels
+#define CAH_HD __host__ __device__ __forceinline__
 
 // Philox4x32-10 (src/random.jl:6-37).  The key schedule is the same for every
 // lane of a wave, so the compiler keeps it in scalar registers; counter words
@@ -101,7 +103,7 @@ CAH_DEV bool bernoulli(uint64_t seed, uint64_t cellid, int64_t t, uint32_t k, do
 
 // _map_axis (src/spaces.jl:46-51) on 0-based indices.  `out` is set only for a
 // Constant axis and an index off the lattice.
-CAH_DEV int32_t map_axis0(int32_t kind, int32_t g, int32_t n, bool& out) {
+CAH_HD int32_t map_axis0(int32_t kind, int32_t g, int32_t n, bool& out) {
     out = false;
     if (g >= 0 && g < n) return g;
     if (kind == CAH_BOUNDARY_PERIODIC) {

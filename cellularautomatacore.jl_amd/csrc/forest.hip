@@ -1,4 +1,6 @@
-// forest.hip -- instantiations and launchers of the forest-fire kernels.
+// forest.hip -- instantiations and launchers of the single-lattice forest-fire
+// kernels (the batched ones are in forest_batch.hip: the two units compile in
+// parallel).
 #include "launch.hpp"
 
 template <int VB, bool SMALL, bool TRI>
@@ -26,32 +28,4 @@ void launch_forest(int vb, bool small, bool tri, int blocks, const ForestArgs& a
     const int prec = (a.p_f32 ? 1 : 0) | (a.f_f32 ? 2 : 0);
     if (tri) forest_t<true>(vb, small, prec, blocks, a, st);
     else forest_t<false>(vb, small, prec, blocks, a, st);
-}
-
-template <bool SMALL, int PREC, bool TRI>
-static void forest_batch_go(int vb, int blocks, const ForestArgs* it, const int32_t* wi, const int32_t* wb,
-                            int32_t nw, int32_t parity, int64_t dt, hipStream_t st) {
-    if (vb == 2) k_forest_batch<2, SMALL, PREC, TRI><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, dt);
-    else k_forest_batch<1, SMALL, PREC, TRI><<<blocks, 256, 0, st>>>(it, wi, wb, nw, parity, dt);
-}
-
-void launch_forest_batch(int vb, bool small, bool tri, int prec, int blocks, const ForestArgs* it,
-                         const int32_t* wi, const int32_t* wb, int32_t nw, int32_t parity, int64_t dt,
-                         hipStream_t st) {
-    auto go = [&](auto S, auto T) {
-        constexpr bool SM = decltype(S)::value, TR = decltype(T)::value;
-        switch (prec) {
-        case 0: forest_batch_go<SM, 0, TR>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
-        case 1: forest_batch_go<SM, 1, TR>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
-        case 2: forest_batch_go<SM, 2, TR>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
-        default: forest_batch_go<SM, 3, TR>(vb, blocks, it, wi, wb, nw, parity, dt, st); break;
-        }
-    };
-    if (small) {
-        if (tri) go(std::true_type{}, std::true_type{});
-        else go(std::true_type{}, std::false_type{});
-    } else {
-        if (tri) go(std::false_type{}, std::true_type{});
-        else go(std::false_type{}, std::false_type{});
-    }
 }

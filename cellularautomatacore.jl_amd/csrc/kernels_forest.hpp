@@ -194,15 +194,9 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
 
 template <int VB, bool SMALL, int PREC, bool TRI>
 CAH_DEV void forest_band(const ForestArgs& a, int wid) {
-    constexpr int OW = 62 * 4 * VB;
-    const int lane = threadIdx.x & 63;
-    const int band = wid % a.nbands, chunk = wid / a.nbands;
-    const int32_t ys = chunk * a.rows;
-    const int32_t ye = min(ys + a.rows, a.L.n2);
-    const int32_t x = band * OW - 4 * VB + lane * 4 * VB;
-    const bool writer = lane >= 1 && lane <= 62 && x < a.L.n1;
-    if (mapped_ok<VB>(a.L, 3)) forest_walk<VB, SMALL, PREC, true, TRI>(a, x, ys, ye, writer);
-    else forest_walk<VB, SMALL, PREC, false, TRI>(a, x, ys, ye, writer);
+    const BandPart b = band_part<VB>(a.nbands, a.rows, a.L, wid, threadIdx.x & 63);
+    if (mapped_ok<VB>(a.L, 3)) forest_walk<VB, SMALL, PREC, true, TRI>(a, b.x, b.ys, b.ye, b.writer);
+    else forest_walk<VB, SMALL, PREC, false, TRI>(a, b.x, b.ys, b.ye, b.writer);
 }
 
 template <int VB, bool SMALL, int PREC, bool TRI>

@@ -170,16 +170,9 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 // at the same time, so the second read finds them in L2.
 template <int R, int VB, bool BIN, int NIV>
 CAH_DEV void life_band(const LifeArgs& a, int wid) {
-    constexpr int OW = 62 * 4 * VB;  // output bytes per band
-    const int lane = threadIdx.x & 63;
-    const int band = wid % a.nbands, chunk = wid / a.nbands;
-    const int32_t ys = chunk * a.rows;
-    const int32_t ye = min(ys + a.rows, a.L.n2);
-    const int32_t x = band * OW - 4 * VB + lane * 4 * VB;
-    const bool writer = lane >= 1 && lane <= 62 && x < a.L.n1;
-    const bool up = chunk & 1;
-    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true>(a, x, ys, ye, writer, up);
-    else life_walk<R, VB, BIN, NIV, false>(a, x, ys, ye, writer, up);
+    const BandPart b = band_part<VB>(a.nbands, a.rows, a.L, wid, threadIdx.x & 63);
+    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true>(a, b.x, b.ys, b.ye, b.writer, b.up);
+    else life_walk<R, VB, BIN, NIV, false>(a, b.x, b.ys, b.ye, b.writer, b.up);
 }
 
 template <int R, int VB, bool BIN, int NIV>

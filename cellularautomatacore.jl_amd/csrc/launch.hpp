@@ -48,3 +48,6 @@ int lenia_fft_create(LeniaFFT** out, int n1, int n2, int64_t pitch, const int32_
 int lenia_fft_field(LeniaFFT* f, const float* cur, hipStream_t st);
 float* lenia_fft_buffer(LeniaFFT* f);
 void lenia_fft_destroy(LeniaFFT* f);
+// A plan-less stand-in with the given (fake) buffers, for launch checks without
+// a device: its field is never computed.
+LeniaFFT* lenia_fft_make_dry(int n1, int n2, void* spec, void* kspec, float* field);

@@ -74,6 +74,7 @@ struct LeniaFFT {
     float* field = nullptr;
     int n1 = 0, n2 = 0;
     int64_t nspec = 0;
+    bool dry = false;  // buffers at fake addresses, no plans
 };
 
 __global__ void k_spec_conj_scale(float2* k, int64_t n, float scale) {
@@ -89,8 +90,24 @@ __global__ void k_spec_mul(float2* s, const float2* k, int64_t n) {
     }
 }
 
+LeniaFFT* lenia_fft_make_dry(int n1, int n2, void* spec, void* kspec, float* field) {
+    LeniaFFT* f = new LeniaFFT();
+    f->n1 = n1;
+    f->n2 = n2;
+    f->nspec = (int64_t)(n1 / 2 + 1) * n2;
+    f->spec = (float2*)spec;
+    f->kspec = (float2*)kspec;
+    f->field = field;
+    f->dry = true;
+    return f;
+}
+
 void lenia_fft_destroy(LeniaFFT* f) {
     if (!f) return;
+    if (f->dry) {
+        delete f;
+        return;
+    }
     if (f->fwd) hipfftDestroy(f->fwd);
     if (f->inv) hipfftDestroy(f->inv);
     for (void* p : {(void*)f->spec, (void*)f->kspec, (void*)f->field})

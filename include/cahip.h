@@ -297,6 +297,26 @@ int cah_batch_sync(cah_batch* b);
 int cah_batch_stream(cah_batch* b, void** stream);
 int cah_batch_destroy(cah_batch* b);
 
+/* ---------------------------------------------------------------- launch checks
+ * No reference counterpart: a debugging aid.  Every kernel launch the library
+ * makes can be checked on the host before it is queued: a model of the
+ * kernel's addressing (the kernel's own index arithmetic, over every wave, lane,
+ * block and row of the grid) gives the lowest and highest byte each buffer
+ * argument can be read or written at, which must lie inside that buffer's
+ * allocation; the grid must cover the work and the LDS size must fit.  A failed
+ * check refuses the launch with CAH_ERR_DEVICE.  With the environment variable
+ * CAH_CHECK_LAUNCHES set (to anything but 0) when the library loads, every
+ * launch is checked against the device allocations.  The two calls below need
+ * no device: they build the state (or batch) with fake device addresses and
+ * run the launch sequence of cah_advance (cah_batch_advance, without a graph)
+ * through the checks only.  `report` (may be NULL) receives one line per kernel:
+ * its launches and checks made. */
+int cah_check_launches(const cah_desc* d, int64_t t0, int64_t nsteps, const cah_params* p, char* report,
+                       size_t cap);
+/* vb, rows: the band walk's tile as cah_batch_tune takes it (0, 0: the default). */
+int cah_check_batch_launches(int32_t n, const cah_desc* descs, int64_t t0, int64_t nsteps, const cah_params* p,
+                             int32_t vb, int32_t rows, char* report, size_t cap);
+
 /* ---------------------------------------------------------------- serving
  * No single reference counterpart: a simulation server for the traffic that
  * ensembles and abc_rejection generate (src/calibration.jl:66-91), with
