@@ -7,8 +7,12 @@ Workload (one "step"): one synchronous step of each of
     p = 0.03, f = 1e-4, Philox seed 7;
 both lattices resident in HBM.  `value` = cell updates per second over both, in
 GCUPS.  Kernel durations come from HIP events recorded on each state's own
-stream around its launches.  The CPU baseline is the oracle's restatement of
-the reference's threaded ArrayBackend, timed on a few steps of each lattice.
+stream around its launches.  Each lattice first runs `--warmup` steps and then
+at least WARM_MS of further untimed steps (the device's clocks settle over
+tens of milliseconds of load), then WINDOWS timed windows of exactly `--steps`
+steps each; the figures are the median window's, and the spread over the
+windows is reported.  The CPU baseline is the oracle's restatement of the
+reference's threaded ArrayBackend, timed on a few steps of each lattice.
 """
 import argparse
 import ctypes
@@ -25,6 +29,9 @@ sys.path.insert(0, os.path.join(ROOT, "cellularautomatacore.jl_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP16_PEAK_TFLOPS = 2500.0  # dense FP16 / BF16 MFMA
+WARM_MS = 200.0            # untimed device work per lattice before the timed windows
+WINDOWS = 5                # timed windows of --steps steps each; the median is reported
 # VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, in T lane-instructions/s
 VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
 # k_forest_vn1<1, true, 0, true> (gfx950 ISA): 181 VALU instructions in the walk's row body of 4
@@ -92,6 +99,34 @@ def pmc_summary(fetch_csv, write_csv, cmd):
     return out
 
 
+def trace_summary(trace_csv, out_md, cmd, top=12):
+    """Kernel durations per (kernel, grid size) from a rocprofv3 --kernel-trace
+    CSV (one row per dispatch), so that the benchmark's lattices, which share
+    kernels with the other configurations, are told apart by their launch
+    size: calls, mean, median, min and max in microseconds, and the share of
+    the traced kernel time.  Writes a Markdown table to out_md."""
+    import csv
+    groups, total = {}, 0.0
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            dims = lambda k: (int(r.get(k + "_X") or 1) * int(r.get(k + "_Y") or 1) *  # noqa: E731
+                              int(r.get(k + "_Z") or 1)) if k + "_X" in r else int(r.get(k) or 0)
+            key = (r["Kernel_Name"], dims("Grid_Size"), dims("Workgroup_Size"))
+            groups.setdefault(key, []).append(d)
+            total += d
+    rows = sorted(groups.items(), key=lambda kv: -sum(kv[1]))[:top]
+    lines = ["| Kernel | Grid (work-items) | Block | Calls | Mean (us) | Median (us) | Min (us) | Max (us) | Share |",
+             "|---|---|---|---|---|---|---|---|---|"]
+    for (name, grid, wg), ds in rows:
+        ds = sorted(ds)
+        lines.append(f"| `{name[:90]}` | {grid} | {wg} | {len(ds)} | {sum(ds) / len(ds):.2f} | "
+                     f"{ds[len(ds) // 2]:.2f} | {ds[0]:.2f} | {ds[-1]:.2f} | {100 * sum(ds) / total:.1f}% |")
+    with open(out_md, "w") as fh:
+        fh.write("# Kernel durations by launch size (rocprofv3 --kernel-trace)\n\n"
+                 f"Command:\n\n```\n{cmd}\n```\n\n" + "\n".join(lines) + "\n")
+
+
 class HipEvents:
     """hipEvent timing on an explicit stream (libamdhip64 through ctypes)."""
 
@@ -129,6 +164,30 @@ def make_states(be):
     check(be.lib.cah_state_fill_bernoulli(s_ltl.handle, 1, 0.5))
     s_for = be.allocate_state(forest, M.Synchronous())
     return (ltl, s_ltl), (forest, s_for)
+
+
+def host_cores():
+    """The host cores this job may use: the CPUs of its affinity mask, capped by
+    a cgroup CPU quota and by OMP_NUM_THREADS when they are set (a multi-GPU
+    host grants each GPU's job a share of its cores and says so there).
+    Returns (cores, description)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    n = aff
+    if quota:
+        n = min(n, quota)
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return n, {"cpus_visible": os.cpu_count(), "affinity": aff, "cgroup_cpu_quota": quota,
+               "omp_num_threads": omp or None, "threads_used": n}
 
 
 def cpu_baseline(nthreads, steps=2):
@@ -435,6 +494,26 @@ def bench_lenia(be, ev, steps=500):
                       "final_max_abs_vs_fp64": float(np.abs(final - seq_final).max()),
                       "final_mass_rel_diff_vs_fp64": abs(float(final.sum()) - float(seq_final.sum())) /
                       float(seq_final.sum())}
+        taps = len(prob.taps[0])
+        sec = ms / steps * 1e-3
+        algo_tflops = 2 * taps * n * n / sec / 1e12
+        out[field]["hbm_gbs_algorithmic"] = round(8 * n * n / sec / 1e9, 1)
+        if field.startswith("mfma"):
+            # one 64 x 64 block: (2H+1) row offsets x 2 K-chunks x 4 row tiles x 4
+            # waves of 16x16x32 MFMAs (16384 flop each), three per product when split
+            H = 12
+            per_block = (2 * H + 1) * 2 * 4 * 4 * 2 * 16 * 16 * 32 * (3 if field == "mfma_f16x2" else 1)
+            exec_tflops = per_block * (n // 64) ** 2 / sec / 1e12
+            out[field]["roofline"] = {
+                "bound": "mfma", "kernel": "k_lenia_mfma", "achieved": round(algo_tflops, 1),
+                "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(algo_tflops / FP16_PEAK_TFLOPS, 4),
+                "algorithmic_flop_per_cell": 2 * taps,
+                "executed_tflops": round(exec_tflops, 1),
+                "executed_frac": round(exec_tflops / FP16_PEAK_TFLOPS, 4),
+                "note": "achieved counts the direct sum's 2 x 516 flop a cell; executed counts every MFMA "
+                        "the implicit GEMM issues, the Toeplitz band's zeros included"}
+        else:
+            out[field]["tflops_algorithmic"] = round(algo_tflops, 2)
     ref_be.free_state(ref)
     return {"workload": "Lenia R=13 Float32 Torus 2048^2, central 256^2 uniform patch, 500 steps",
             "methods": out}
@@ -591,6 +670,13 @@ def bench_serving():
            "requests": n,
            "cell_updates": int(sum(p.u0.size * (p.tspan[1] - p.tspan[0]) for p in probs))}
     out.update(latency_pair(probs, arr))
+    # capacity: the same requests all present at t = 0, served and plain
+    cap = latency_pair(probs, np.zeros(n))
+    out["capacity"] = {"served_sims_per_s": cap["served"]["sims_per_s"],
+                       "plain_sims_per_s": cap["plain"]["sims_per_s"],
+                       "ratio": round(cap["served"]["sims_per_s"] / cap["plain"]["sims_per_s"], 2),
+                       "served_p50_ms": cap["served"]["p50_ms"], "served_p99_ms": cap["served"]["p99_ms"],
+                       "identical_results": cap["identical_results"]}
     return out
 
 
@@ -631,9 +717,14 @@ def main():
                     help="also run the plain copy that scales the HBM counters")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "COMMAND"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
+    ap.add_argument("--trace-summary", nargs=3, metavar=("TRACE_CSV", "OUT_MD", "COMMAND"),
+                    help="summarise a rocprofv3 --kernel-trace CSV by kernel and launch size and exit")
     args = ap.parse_args()
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary)))
+        return
+    if args.trace_summary:
+        trace_summary(*args.trace_summary)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the sweep on a single MI355X "
@@ -652,27 +743,42 @@ def main():
     def run(st, params, t0, n):
         check(be.lib.cah_advance(st.handle, t0, n, ctypes.byref(params)))
 
-    run(s_ltl, pl, 0, args.warmup)
-    run(s_for, pf, 0, args.warmup)
-    ev.sync()
+    K, W = args.steps, args.warmup
 
-    K = args.steps
-    # each lattice timed on its own (the other idle), with HIP events recorded on
-    # the state's own stream around its K launches
-    str_ltl, str_for = be.stream(s_ltl), be.stream(s_for)
-    a1, b1, a2, b2 = ev.event(), ev.event(), ev.event(), ev.event()
-    t_wall = time.perf_counter()
-    ev.record(a1, str_ltl)
-    run(s_ltl, pl, args.warmup, K)
-    ev.record(b1, str_ltl)
-    ev.sync()
-    ev.record(a2, str_for)
-    run(s_for, pf, args.warmup, K)
-    ev.record(b2, str_for)
-    ev.sync()
-    wall = time.perf_counter() - t_wall
-    ms_ltl = ev.elapsed_ms(a1, b1) / K
-    ms_for = ev.elapsed_ms(a2, b2) / K
+    def reset(st, kind):
+        """The lattice's initial state again: Bernoulli(0.5) from seed 1 (LtL), all empty (forest)."""
+        check(be.lib.cah_state_fill_bernoulli(st.handle, *((1, 0.5) if kind == "ltl" else (7, 0.0))))
+
+    def measure(st, params, kind):
+        """At least WARM_MS of untimed steps (the clocks settle under load), then
+        WINDOWS windows, each from the initial state: W untimed steps, then
+        exactly K steps between HIP events on the state's own stream.  Per
+        window: (device ms per step, host wall ms per step)."""
+        stream, spent, t = be.stream(st), 0.0, 0
+        while spent < WARM_MS:
+            spent += timed(ev, stream, lambda: run(st, params, t, 20))
+            t += 20
+        out = []
+        for _ in range(WINDOWS):
+            reset(st, kind)
+            run(st, params, 0, W)
+            a, b = ev.event(), ev.event()
+            ev.sync()
+            t_wall = time.perf_counter()
+            ev.record(a, stream)
+            run(st, params, W, K)
+            ev.record(b, stream)
+            ev.sync()
+            wall = time.perf_counter() - t_wall
+            out.append((ev.elapsed_ms(a, b) / K, wall * 1e3 / K))
+        return out
+
+    # each lattice timed on its own (the other idle)
+    win_ltl = measure(s_ltl, pl, "ltl")
+    win_for = measure(s_for, pf, "forest")
+    med = lambda xs: float(np.median(xs))  # noqa: E731
+    ms_ltl, ms_for = med([w[0] for w in win_ltl]), med([w[0] for w in win_for])
+    wall_ms = med([w[1] for w in win_ltl]) + med([w[1] for w in win_for])
 
     c_ltl, c_for = 8192 * 8192, 16384 * 16384
     ms_step = ms_ltl + ms_for
@@ -705,7 +811,7 @@ def main():
         "metric": "GCUPS + % HBM roofline (synchronous sweep)",
         "value": round(value, 3), "unit": "GCUPS", "n_gpus": 1, "steps": K,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "wall_ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True,
+        "wall_ms_per_step": round(wall_ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int8", "data": "synthetic",
         "config": {"workload": "LtL Moore(5) B34..45/S34..58 Torus 8192^2 Bernoulli(0.5) seed 1 "
                                "+ forest fire VN(1) Reflecting 16384^2 all-empty p=0.03 f=1e-4 seed 7",
@@ -717,19 +823,28 @@ def main():
                              "ms_per_step": round(ms_for, 5),
                              "hbm_gbs_algorithmic": round(2 * c_for / ms_for / 1e6, 1)},
         },
+        "timing": {"warm_ms_per_lattice": WARM_MS, "windows": WINDOWS,
+                   "note": f"each window: the initial state, {W} untimed steps, then {K} timed steps; "
+                           "the median window is reported",
+                   "ltl_ms_per_step": [round(w[0], 5) for w in win_ltl],
+                   "forest_ms_per_step": [round(w[0], 5) for w in win_for],
+                   "spread": round(max(max(w[0] for w in ws) / min(w[0] for w in ws) - 1
+                                       for ws in (win_ltl, win_for)), 4)},
         "pcie_inclusive": {"host_copy_ms": round(host_ms, 2),
                            "gcups": round((c_ltl + c_for) * K / (ms_step * K + host_ms) / 1e6, 3),
                            "note": f"{K} steps of each lattice with the initial state set from and "
                                    "the final state read back to pageable host memory"},
-        "roofline": {"bound": "hbm", "kernel": KERNELS["forest"], "achieved": round(achieved, 1),
+        "roofline": {"bound": "valu", "kernel": KERNELS["forest"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("forest", {}).get("bytes_per_launch"),
                      "copy_gbs": round(cp_for, 1), "frac_of_copy": round(achieved / cp_for, 4),
                      "valu": {"instructions_per_draw": FOREST_VALU_PER_DRAW,
                               "achieved_t_lane_instr_s": round(valu_t, 2), "peak": round(VALU_PEAK_T, 2),
                               "frac": round(valu_t / VALU_PEAK_T, 4)},
-                     "note": "one Philox4x32-10 draw per cell (17 v_mad_u64_u32 / v_mul_*_u32 "
-                             "and 17-18 v_bitop3_b32): bound by the VALU, not by HBM"},
+                     "note": "achieved/peak/frac: the HBM view (2 B per cell update against 8 TB/s); the "
+                             "kernel is bound by the VALU (valu: 45.25 lane-instructions per cell for "
+                             "one Philox4x32-10 draw, 17 v_mad_u64_u32 / v_mul_*_u32 and 17-18 "
+                             "v_bitop3_b32, the rule and the store, against the issue peak)"},
         "roofline_ltl": {"bound": "hbm", "kernel": KERNELS["ltl"], "achieved": round(achieved_ltl, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved_ltl / HBM_PEAK_GBS, 4),
@@ -753,8 +868,9 @@ def main():
             sys.exit(f"unknown configs {unknown}; known: {list(configs)}")
         line["configs"] = {k: configs[k]() for k in names}
     if not args.no_cpu:
-        n = min(16, os.cpu_count() or 1)
+        n, host = host_cores()
         line["cpu_baseline"] = cpu_baseline(n)
+        line["cpu_baseline"]["host"] = host
     print(json.dumps(line), flush=True)
 
 

@@ -30,11 +30,15 @@ class Server:
         self.h = h
         self.meta = {}
 
-    def submit(self, prob, scheme=None):
+    def submit(self, prob, scheme=None, u0=None):
+        """Queue prob; u0, when given, replaces prob.u0 as the initial state (any
+        contents of the state's element type, as set_state! takes them)."""
         scheme = M.Synchronous() if scheme is None else scheme
         d, keep = make_desc(prob, scheme, self.be.lenia_field)
         cp = make_params(prob, prob.p)
-        u0 = np.asfortranarray(prob.u0)
+        u0 = np.asfortranarray(prob.u0 if u0 is None else np.asarray(u0, dtype=prob.u0.dtype))
+        if u0.shape != tuple(prob.space.dims):
+            raise M.StateShapeError(f"expected size {prob.space.dims}, got {u0.shape}")
         rid = ctypes.c_uint64()
         check(self.lib.cah_server_submit(self.h, ctypes.byref(d), u0.ctypes.data, u0.nbytes,
                                          prob.tspan[0], prob.tspan[1], ctypes.byref(cp),

@@ -1945,6 +1945,9 @@ extern "C" int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* c
         if (int rc = counts_check(m, other)) return rc;
     HIPCHK(hipSetDevice(b->device));
     if (other && other->stream != b->stream) HIPCHK(hipStreamSynchronize(other->stream));
+    // a batch without a batched kernel advances its members on their own streams
+    for (cah_state* m : b->members)
+        if (m->stream != b->stream) HIPCHK(hipStreamSynchronize(m->stream));
     const size_t n = b->members.size();
     if (!b->counts_d) HIPCHK(dmalloc((void**)&b->counts_d, counts_words(n) * sizeof(unsigned long long), "batch counters"));
     // every member's counts on the batch stream, then one copy back

@@ -61,7 +61,34 @@ def forest():
             "saves": run(prob, [100, 500])}
 
 
-JOBS = {"ltl_8192": ltl, "forest_16384": forest}
+def batch_sides(n=256, seed=2026):
+    """The batched workload's documented lattice sides: round(64 * 32^u), u from
+    numpy's default_rng(2026), square."""
+    u = np.random.default_rng(seed).random(n)
+    return [int(round(64 * 32 ** x)) for x in u]
+
+
+def batched_forest():
+    """256 forest fires, trajectory i (1-based) with seed i (prob.seed = 0 and
+    seed_trajectory), sides batch_sides(), Reflecting, all empty, p = 0.03,
+    f = 1e-4, 300 steps: the digest of every final state."""
+    out = []
+    for i, s in enumerate(batch_sides()):
+        z = np.zeros((s, s), np.int8, order="F")
+        prob = CAProblem(M.ForestFire(), z, 300, {"p": 0.03, "f": 1e-4},
+                         space=M.Grid(s, s, boundary=M.Reflecting()), seed=i + 1)
+        d, keep = make_desc(prob, M.Synchronous())
+        u = caref.advance_halo(d, z, 0, 300, make_params(prob, prob.p), NT)
+        out.append({"side": s, "seed": i + 1, "sha256": digest(u), "sum": int(u.astype(np.int64).sum()),
+                    "count2": int((u == 2).sum())})
+        if i % 32 == 0:
+            print(f"  trajectory {i + 1}: side {s}", flush=True)
+    return {"desc": "256 forest fires VN(1) Reflecting, sides round(64*32^u) (u ~ default_rng(2026)), "
+                    "seeds 1..256, all empty, p=0.03 f=1e-4, 300 steps",
+            "final": out}
+
+
+JOBS = {"ltl_8192": ltl, "forest_16384": forest, "batched_forest_256": batched_forest}
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(JOBS)

@@ -33,6 +33,16 @@ def params_for(prob, p=None):
     return make_params(prob, prob.p if p is None else p)
 
 
+def oracle_final(oracle, prob, scheme=None, nthreads=None):
+    """The oracle's state at the end of prob's tspan (the threaded ArrayBackend
+    restatement, for Int8 rules)."""
+    import os
+    d, keep = desc_for(prob, scheme, field_method=0)
+    t0, t1 = prob.tspan
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    return oracle.advance_halo(d, prob.u0, t0, t1 - t0, params_for(prob), nt)
+
+
 def naive_generations(birth, survive, C, u):
     """naive_generations (test/test_rules.jl:23-32): Moore(1) on a torus, C = 2 is Life."""
     m, n = u.shape

@@ -10,7 +10,7 @@ import pytest
 from cahip import model as M
 from cahip.backend import HIPBackend, make_desc, make_params
 from cahip.problem import CAProblem, solve
-from helpers import desc_for, forest_problem, ltl_rule, params_for, rng_state
+from helpers import desc_for, forest_problem, ltl_rule, oracle_final, params_for, rng_state
 from test_oracle import CASES, SCHEMES
 
 pytestmark = pytest.mark.gpu
@@ -372,10 +372,10 @@ def test_resume_is_exact(oracle, backend):
 
 
 @pytest.mark.parametrize("vb,rows", [(1, 4), (1, 64), (2, 4), (2, 64)])
-def test_batch_tiles(vb, rows):
+def test_batch_tiles(oracle, vb, rows):
     """Every tile the autotuner may pick (4 or 8 cells a lane, 4..64 rows a wave)
-    gives each trajectory its single run: forest fires of ragged sizes and
-    Larger-than-Life of radius 2."""
+    gives each trajectory its single run and the oracle's: forest fires of
+    ragged sizes and Larger-than-Life of radius 2."""
     from cahip.problem import final_state, seed_trajectory
     be = HIPBackend()
     probs = []
@@ -388,9 +388,10 @@ def test_batch_tiles(vb, rows):
         outs = be.solve_batch(group, M.Synchronous(), tile=(vb, rows))
         for p, out in zip(group, outs):
             assert np.array_equal(out, final_state(solve(p, backend=be, save_everystep=False)))
+            assert np.array_equal(out, oracle_final(oracle, p))
 
 
-def test_batch_forest_values_outside_the_rule():
+def test_batch_forest_values_outside_the_rule(oracle):
     """A batch whose lattices hold values the forest fire never makes (3, -1, 5)
     runs the exact byte tests until a step has cleaned them, then the short
     ones: each trajectory equals its single run over two advances."""
@@ -426,9 +427,11 @@ def test_batch_forest_values_outside_the_rule():
             assert np.array_equal(out, be.current_state(st))
         finally:
             be.free_state(st)
+        d, keep = desc_for(p)  # u holds values the rule's validation refuses: the oracle takes it as is
+        assert np.array_equal(out, oracle.advance_halo(d, u, 2, 8, params_for(p), NT))
 
 
-def test_batch_autotune_keeps_results():
+def test_batch_autotune_keeps_results(oracle):
     """cah_batch_autotune times each candidate tile (cells a lane, rows a wave)
     on the batch's own lattices and undoes the trial steps: the state at t0 (saved after tuning) is
     still u0, and every trajectory equals its single run (forest fire with
@@ -448,6 +451,7 @@ def test_batch_autotune_keeps_results():
         for p, ser in zip(group, series):
             assert np.array_equal(ser[0], p.u0)
             assert np.array_equal(ser[-1], final_state(solve(p, backend=be, save_everystep=False)))
+            assert np.array_equal(ser[-1], oracle_final(oracle, p))
     conway = [CAProblem(M.Life("conway"), rng_state(20 + i, (90, 70), [0, 1]), 9) for i in range(2)]
     be.solve_batch(conway, M.Synchronous(), autotune=True)
     assert be.batch_tile == (0, 0)
@@ -459,6 +463,8 @@ def test_batch_equals_single_runs(oracle, backend):
     ep = EnsembleProblem(base)
     batched = solve_ensemble(ep, backend=backend, trajectories=5, batched=True)
     single = solve_ensemble(ep, backend=backend, trajectories=5, batched=False)
-    for a, b in zip(batched, single):
+    for i, (a, b) in enumerate(zip(batched, single)):
         assert np.array_equal(a, b)
+        # trajectory i + 1 runs with seed prob.seed + i + 1 (seed_trajectory)
+        assert np.array_equal(a, oracle_final(oracle, base.remake(seed=i + 1)))
     assert len({a.tobytes() for a in batched}) == 5

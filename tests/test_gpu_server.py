@@ -1,5 +1,8 @@
 """The serving form: requests of mixed rules, lattices and tspans arriving while
-others run give, each, exactly the result of solving that problem alone."""
+others run give, each, exactly the result of solving that problem alone, and
+for the Int8 rules exactly the CPU oracle's (the threaded ArrayBackend
+restatement); Lenia requests equal their single GPU run, which
+test_gpu_lenia.py holds to the FP64 oracle per step."""
 import numpy as np
 import pytest
 
@@ -8,7 +11,7 @@ from cahip import model as M
 from cahip.backend import HIPBackend
 from cahip.problem import CAProblem, final_state, solve
 from cahip.server import Server
-from helpers import ltl_rule
+from helpers import ltl_rule, oracle_final
 
 pytestmark = pytest.mark.gpu
 
@@ -46,8 +49,21 @@ def mixed_requests(n, seed=4):
     return probs
 
 
+def assert_exact(oracle, results, ids, probs, schemes=None, lenia_field="mfma_f16"):
+    """Every served result equals the problem solved alone on the GPU and, for an
+    Int8 rule, the oracle's final state."""
+    be = HIPBackend(lenia_field=lenia_field)
+    schemes = schemes or [M.Synchronous()] * len(probs)
+    for rid, p, sc in zip(ids, probs, schemes):
+        alone = final_state(solve(p, sc, backend=be, save_everystep=False))
+        assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs from its single run"
+        if not isinstance(p.rule, M.Lenia):
+            assert np.array_equal(results[rid], oracle_final(oracle, p, sc)), \
+                f"request {rid} ({p.rule}) differs from the oracle"
+
+
 @pytest.mark.parametrize("chunk,tick_us", [(0, 0), (7, 0), (7, 400)])
-def test_mix_equals_alone(chunk, tick_us):
+def test_mix_equals_alone(oracle, chunk, tick_us):
     probs = mixed_requests(30)
     # a budget that holds only a few lattices at once, so requests queue
     srv = Server(budget_bytes=3_000_000, chunk=chunk, lenia_field="mfma_f16")
@@ -71,16 +87,14 @@ def test_mix_equals_alone(chunk, tick_us):
     # poll reports requests in the order their ticks finished
     t_done = [srv.times(rid)[2] for rid in order]
     assert t_done == sorted(t_done)
-    be = HIPBackend(lenia_field="mfma_f16")
-    for rid, p in zip(ids, probs):
-        alone = final_state(solve(p, backend=be, save_everystep=False))
-        assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs"
+    assert_exact(oracle, results, ids, probs)
+    for rid in ids:
         t_sub, t_adm, t_done = srv.times(rid)
         assert t_sub <= t_adm <= t_done
     srv.close()
 
 
-def test_fft_and_stochastic_requests():
+def test_fft_and_stochastic_requests(oracle):
     """Requests on the FFT field and the stochastic scheme (the loop family)."""
     rng = np.random.default_rng(9)
     probs, schemes = [], []
@@ -102,10 +116,7 @@ def test_fft_and_stochastic_requests():
                 results[rid] = srv.fetch(rid)
         if len(results) == len(ids):
             break
-    be = HIPBackend(lenia_field="mfma_f16")
-    for rid, p, sc in zip(ids, probs, schemes):
-        alone = final_state(solve(p, sc, backend=be, save_everystep=False))
-        assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs"
+    assert_exact(oracle, results, ids, probs, schemes)
     srv.close()
 
 
@@ -116,7 +127,7 @@ def test_budget_refuses_a_request_larger_than_it():
     srv.close()
 
 
-def test_fetch_from_threads_while_ticking():
+def test_fetch_from_threads_while_ticking(oracle):
     """Client threads copy results out (lattices of 5 MB and more, which the host
     copy splits over its helper threads) while the serving thread submits and
     ticks; the budget holds a few lattices, so fetches free room for waiting
@@ -168,9 +179,47 @@ def test_fetch_from_threads_while_ticking():
             t.join()
     assert not errors, errors
     assert polled == len(ids) and len(results) == len(ids)
-    be = HIPBackend(lenia_field="mfma_f16")
-    for rid, p in zip(ids, probs):
-        alone = final_state(solve(p, backend=be, save_everystep=False))
-        assert np.array_equal(results[rid], alone), f"request {rid} ({p.rule}) differs"
+    assert_exact(oracle, results, ids, probs)
+    for rid in ids:
         assert srv.status(rid) == L.REQ_FETCHED
+    srv.close()
+
+
+@pytest.mark.parametrize("chunk", [1, 3])
+def test_forest_values_outside_the_rule(oracle, chunk):
+    """Forest-fire requests whose states hold values the rule never makes (3, -1,
+    5), and requests whose Constant edge is 3 or 2, served over several ticks
+    next to requests holding only 0, 1 and 2: the short byte tests must wait
+    until a step has cleaned a lattice, and never run beside a Constant(3)
+    edge.  Each result equals the oracle's."""
+    from cahip.backend import make_desc, make_params
+    from cahip.server import Server as S
+    rng = np.random.default_rng(17)
+    cases = []  # (dims, edge, values)
+    for i, (dims, edge, vals) in enumerate([
+            ((150, 97), M.Reflecting(), [0, 1, 2, 3, -1, 5, 1, 1]),
+            ((300, 64), M.Constant(3), [0, 1, 2, 1]),
+            ((97, 130), M.Constant(2), [0, 1, 1, 2]),
+            ((64, 64), M.Reflecting(), [0, 1, 1, 2]),
+            ((200, 33), M.Constant(3), [0, 1, 2, 3, -1, 5])]):
+        cases.append((dims, edge, rng.choice(np.array(vals, np.int8), dims)))
+    srv = S(budget_bytes=50_000_000, chunk=chunk)
+    ids, refs = [], []
+    for i, (dims, edge, u) in enumerate(cases):
+        space = M.Grid(*dims, boundary=edge)
+        prob = CAProblem(M.ForestFire(), np.zeros(dims, np.int8), (2, 12), {"p": 0.2, "f": 0.02}, space=space,
+                         seed=30 + i)
+        ids.append(srv.submit(prob, u0=u))
+        d, keep = make_desc(prob, M.Synchronous())
+        refs.append(oracle.advance_halo(d, np.asfortranarray(u), 2, 10, make_params(prob, prob.p), 4))
+        srv.tick()
+    results = {}
+    for _ in range(100000):
+        srv.tick()
+        for rid in srv.poll():
+            results[rid] = srv.fetch(rid)
+        if len(results) == len(ids):
+            break
+    for rid, ref in zip(ids, refs):
+        assert np.array_equal(results[rid], ref), f"request {rid} differs from the oracle"
     srv.close()
