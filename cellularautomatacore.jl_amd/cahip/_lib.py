@@ -111,6 +111,7 @@ SIGNATURES = {
     "cah_batch_fill_bernoulli": (_I, [_P, _I32, _U64, _D]),
     "cah_batch_advance": (_I, [_P, _I64, _I64, _PP, _I32]),
     "cah_batch_counts": (_I, [_P, _P, _P, _P]),
+    "cah_batch_counts_noalloc": (_I, [_P, _P, _P, _P]),
     "cah_batch_autotune": (_I, [_P, _I64, _I32, _PP, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "cah_batch_tune": (_I, [_P, _I32, _I32]),
     "cah_batch_sync": (_I, [_P]),
@@ -132,6 +133,7 @@ SIGNATURES = {
 }
 
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FETCHED = 0, 1, 2, 3
+COUNTS_NOALLOC_MAX = 512
 
 _lib = None
 

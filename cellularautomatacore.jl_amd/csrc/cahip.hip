@@ -110,6 +110,31 @@ struct cah_ctx {
     int device;
 };
 
+// The counters of cah_batch_counts_noalloc: a fixed device array (each device
+// the library opens holds its own copy until the process ends), laid out as
+// the partial sums of one k_counts launch, then CAH_COUNTS_NOALLOC_MAX lattices'
+// CNT_SLOTS counters, then their counts and agreements in the caller's layout.
+// Host code reaches it only through the address hipGetSymbolAddress gives for
+// the current device (g_scratch_ptr), never through the symbol's name.
+constexpr size_t SCRATCH_PART = (size_t)CNT_BLOCKS * CNT_PART;
+constexpr size_t SCRATCH_WORDS = SCRATCH_PART + (size_t)CAH_COUNTS_NOALLOC_MAX * (CNT_SLOTS + 256 + 1);
+static __device__ __align__(256) unsigned long long scratch[SCRATCH_WORDS];
+constexpr int SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The scratch's address on `device` (the current device), looked up once.
+static int scratch_setup(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return CAH_OK;  // the no-alloc counts refuse it
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CAH_OK;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (unsigned long long*)p;
+    lchk::reg(p, SCRATCH_WORDS * sizeof(unsigned long long), "no-alloc counts scratch");
+    return CAH_OK;
+}
+
 extern "C" int cah_open(int device, cah_ctx** out) {
     if (!out) return fail(CAH_ERR_ARGUMENT, "cah_open: out is NULL");
     int n = 0;
@@ -117,6 +142,7 @@ extern "C" int cah_open(int device, cah_ctx** out) {
     if (device < 0 || device >= n)
         return fail(CAH_ERR_ARGUMENT, "device %d does not exist (%d visible)", device, n);
     HIPCHK(hipSetDevice(device));
+    if (int rc = scratch_setup(device)) return rc;
     *out = new cah_ctx{device};
     return CAH_OK;
 }
@@ -1963,6 +1989,65 @@ extern "C" int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* c
         unsigned long long* hi = h.data() + i * CNT_SLOTS;
         if (int rc = finish_counts(b->members[i], b->counts_d + i * CNT_SLOTS, hi, b->stream)) return rc;
         store_counts(hi, counts + i * 256, equal ? equal + i : nullptr);
+    }
+    return CAH_OK;
+}
+
+extern "C" int cah_batch_counts_noalloc(cah_batch* b, const cah_state* other, int64_t* counts, int64_t* equal) {
+    if (!b || !counts || (other && !equal))
+        return fail(CAH_ERR_ARGUMENT, "cah_batch_counts_noalloc: NULL argument");
+    const int64_t n = (int64_t)b->members.size();
+    if (n > CAH_COUNTS_NOALLOC_MAX)
+        return fail(CAH_ERR_ARGUMENT,
+                    "cah_batch_counts_noalloc counts at most %d trajectories (its fixed device array holds "
+                    "their counters); this batch has %lld: use cah_batch_counts",
+                    CAH_COUNTS_NOALLOC_MAX, (long long)n);
+    for (cah_state* m : b->members)
+        if (int rc = counts_check(m, other)) return rc;
+    if (b->device < 0 || b->device >= SCRATCH_DEVICES || !g_scratch_ptr[b->device])
+        return fail(CAH_ERR_DEVICE, "device %d has no counts scratch (it is set up by cah_open)", b->device);
+    HIPCHK(hipSetDevice(b->device));
+    if (other && other->stream != b->stream) HIPCHK(hipStreamSynchronize(other->stream));
+    for (cah_state* m : b->members)
+        if (m->stream != b->stream) HIPCHK(hipStreamSynchronize(m->stream));
+    unsigned long long* part = g_scratch_ptr[b->device];
+    unsigned long long* slots = part + SCRATCH_PART;
+    unsigned long long* packed = slots + (size_t)CAH_COUNTS_NOALLOC_MAX * CNT_SLOTS;
+    unsigned long long* peq = packed + (size_t)CAH_COUNTS_NOALLOC_MAX * 256;
+    // every member's counters (k_counts_reduce writes all CNT_SLOTS of a member's
+    // slot), then the outputs' layout, then one copy of each output
+    for (int64_t i = 0; i < n; ++i)
+        if (int rc = queue_counts(b->members[i], other, slots + i * CNT_SLOTS, part, b->stream)) return rc;
+    const int pblocks = (int)std::min<int64_t>((n * 257 + 255) / 256, 1024);
+    if (int rc = checked(false, "k_counts_pack", [&](lchk::Chk& c) {
+            c.range(slots, 0, 8 * n * CNT_SLOTS, "counters");
+            c.range(packed, 0, 8 * n * 256, "packed counts");
+            c.range(peq, 0, 8 * n, "packed agreements");
+        }))
+        return rc;
+    k_counts_pack<<<pblocks, 256, 0, b->stream>>>(slots, (int32_t)n, packed, peq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(counts, packed, (size_t)n * 256 * sizeof(int64_t), hipMemcpyDeviceToHost, b->stream));
+    if (equal) HIPCHK(hipMemcpyAsync(equal, peq, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    // a lattice holding values outside the rule's own: all 256 byte values
+    for (int64_t i = 0; i < n; ++i) {
+        const cah_state* m = b->members[i];
+        int64_t sum = 0;
+        for (int v = 0; v < 256; ++v) sum += counts[i * 256 + v];
+        if (sum == m->n1 * m->n2) continue;
+        unsigned long long* row = packed + i * 256;
+        const int hb = (int)std::max<int64_t>(1, std::min<int64_t>((m->n1 * m->n2 + 255) / 256, 4096));
+        if (int rc = checked(false, "k_hist256", [&](lchk::Chk& c) {
+                lchk::lattice(c, m->buf[m->cur], lat_of(m), 1, "lattice");
+                c.range(row, 0, 8LL * 256, "histogram");
+            }))
+            return rc;
+        k_zero_u64<<<1, 256, 0, b->stream>>>(row, 256);
+        k_hist256<<<hb, 256, 0, b->stream>>>((const uint8_t*)m->buf[m->cur], m->pitch, m->n1, m->n2, row);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(counts + i * 256, row, 256 * sizeof(int64_t), hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
     }
     return CAH_OK;
 }

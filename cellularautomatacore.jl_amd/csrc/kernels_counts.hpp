@@ -167,3 +167,24 @@ __global__ __launch_bounds__(256) void k_hist256(const uint8_t* __restrict__ s, 
     __syncthreads();
     if (h[threadIdx.x]) atomicAdd(out + threadIdx.x, (unsigned long long)h[threadIdx.x]);
 }
+
+// The counters of n lattices (CNT_SLOTS words each: counts of the 256 byte
+// values, then the agreement) in the layout of cah_batch_counts' outputs:
+// counts[i * 256 + v] and equal[i].
+__global__ __launch_bounds__(256) void k_counts_pack(const unsigned long long* __restrict__ slots, int32_t n,
+                                                     unsigned long long* __restrict__ counts,
+                                                     unsigned long long* __restrict__ equal) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < (int64_t)n * 257;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = j / 257, v = j - i * 257;
+        if (v < 256) counts[i * 256 + v] = slots[i * CNT_SLOTS + v];
+        else equal[i] = slots[i * CNT_SLOTS + 256];
+    }
+}
+
+// n words set to 0 (a kernel rather than a memset, so that the calls that use
+// it can be captured in a graph).
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* __restrict__ p, int64_t n) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        p[j] = 0;
+}

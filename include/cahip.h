@@ -277,6 +277,16 @@ int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const cah_params
  * output_func that reduces each trajectory to its state counts or its agreement
  * with an observed map (src/calibration.jl:76) would compute them. */
 int cah_batch_counts(cah_batch* b, const cah_state* other, int64_t* counts, int64_t* equal);
+/* No counterpart in the reference: cah_batch_counts for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same outputs, and it calls no hipMalloc
+ * or hipFree: its counters live in a fixed device array of about 2.3 MiB that
+ * every device the library opens holds from cah_open until the process ends.
+ * That array holds the counters of CAH_COUNTS_NOALLOC_MAX trajectories; a batch
+ * of more is refused with CAH_ERR_ARGUMENT.  Every call on one device shares the
+ * array: calls that overlap on one device race, so the caller serializes them. */
+#define CAH_COUNTS_NOALLOC_MAX 512
+int cah_batch_counts_noalloc(cah_batch* b, const cah_state* other, int64_t* counts, int64_t* equal);
 /* No reference counterpart: the autotuner of the batched band walk.  Times
  * `steps` steps from t0 for each candidate tile (4 or 8 cells a lane, *vb = 1 or
  * 2, radius-5 Life 8 only; 4, 8, 16, 32 or 64 rows a wave; one tile for every
