@@ -99,7 +99,7 @@ def pmc_summary(fetch_csv, write_csv, cmd):
     return out
 
 
-def trace_summary(trace_csv, out_md, cmd, top=12):
+def trace_summary(trace_csv, out_md, cmd, top=30):
     """Kernel durations per (kernel, grid size) from a rocprofv3 --kernel-trace
     CSV (one row per dispatch), so that the benchmark's lattices, which share
     kernels with the other configurations, are told apart by their launch
@@ -703,6 +703,52 @@ def bench_serving_ensembles():
     return out
 
 
+PROFILE_WORKLOADS = ("ltl", "forest", "lenia_mfma_f16", "lenia_mfma_f16x2", "lenia_mfma_bf16", "lenia_fft_f32",
+                     "brians_brain", "forest_stochastic", "copy")
+
+
+def profile_workload(name, steps):
+    """One workload alone, for profiler passes: `steps` steps (after 3 untimed
+    ones) of Larger-than-Life Moore(5) 8192^2 (ltl), the forest fire 16384^2
+    (forest), Lenia R=13 2048^2 with a field method (lenia_*), Brian's Brain
+    Moore(1) Torus 8192^2 (brians_brain), a Stochastic(0.5) forest fire
+    16384^2 (forest_stochastic), or the calibration copy (copy); prints the mean
+    ms per step from HIP events."""
+    from cahip import model as M
+    from cahip.backend import HIPBackend, check, make_params
+    from cahip.problem import CAProblem
+    ev = HipEvents()
+    if name in ("ltl", "forest", "copy"):
+        be = HIPBackend(0)
+        (ltl, s_ltl), (forest, s_for) = make_states(be)
+        if name == "copy":
+            copy_calibration(be, s_for, reps=steps)
+            return
+        prob, st, scheme = (ltl, s_ltl, M.Synchronous()) if name == "ltl" else (forest, s_for, M.Synchronous())
+    else:
+        field = name[len("lenia_"):] if name.startswith("lenia_") else "f32"
+        be = HIPBackend(0, lenia_field=field)
+        scheme = M.Synchronous()
+        if name.startswith("lenia_"):
+            prob = CAProblem(M.Lenia(), np.zeros((2048, 2048), np.float32), steps, space=M.Torus(2048, 2048))
+            st = be.allocate_state(prob, scheme)
+            check(be.lib.cah_state_fill_uniform_box(st.handle, 1, 896, 256, 896, 256))
+        elif name == "brians_brain":
+            prob = CAProblem(M.Generations("brians_brain"), np.zeros((8192, 8192), np.int8), steps,
+                             space=M.Torus(8192, 8192))
+            st = be.allocate_state(prob, scheme)
+            check(be.lib.cah_state_fill_bernoulli(st.handle, 1, 0.3))
+        else:
+            scheme = M.Stochastic(0.5)
+            prob = CAProblem(M.ForestFire(), np.zeros((16384, 16384), np.int8), steps, {"p": 0.03, "f": 1e-4},
+                             space=M.Grid(16384, 16384, boundary=M.Reflecting()), seed=7)
+            st = be.allocate_state(prob, scheme)
+    cp = make_params(prob, prob.p)
+    check(be.lib.cah_advance(st.handle, 0, 3, ctypes.byref(cp)))
+    ms = timed(ev, be.stream(st), lambda: check(be.lib.cah_advance(st.handle, 3, steps, ctypes.byref(cp))))
+    print(json.dumps({"workload": name, "steps": steps, "ms_per_step": round(ms / steps, 5)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -719,7 +765,12 @@ def main():
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) and exit")
     ap.add_argument("--trace-summary", nargs=3, metavar=("TRACE_CSV", "OUT_MD", "COMMAND"),
                     help="summarise a rocprofv3 --kernel-trace CSV by kernel and launch size and exit")
+    ap.add_argument("--profile", choices=PROFILE_WORKLOADS,
+                    help="run one workload alone for --steps steps (for profiler passes) and exit")
     args = ap.parse_args()
+    if args.profile:
+        profile_workload(args.profile, args.steps)
+        return
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary)))
         return

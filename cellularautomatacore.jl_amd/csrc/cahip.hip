@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -30,6 +31,7 @@
 #include "kernels_life_tb.hpp"
 #include "kernels_generic.hpp"
 #include "launch.hpp"
+#include "lenia_launch.hpp"
 #include "kernels_lenia.hpp"
 #include "launch_check.hpp"
 
@@ -282,6 +284,7 @@ struct cah_state {
     float* wtab_d = nullptr;   // dense table for k_lenia_f32col
     void* field_d = nullptr;   // field buffer for the sequential path
     void* frag_d = nullptr;    // MFMA A fragments
+    size_t frag_bytes = 0;     //   ... bytes of one plane of them
     unsigned long long* counts_d = nullptr;  // CNT_SLOTS counters of cah_state_counts
     double* cov_d = nullptr;   // field epidemic: covariate slices
     // the sparse sweep (cah_state_sparse): two candidate lists used in alternate
@@ -637,6 +640,7 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
             s->lenia_H = ht;
             std::vector<char> frag;
             lenia_mfma_fragments(s->lenia_mode, ht, wd.data(), frag);
+            s->frag_bytes = frag.size() / 2;  // one plane
             return upload(s, (char**)&s->frag_d, frag.data(), frag.size());
         }
         const bool fused = fm == CAH_FIELD_DIRECT_F32 && !cb;
@@ -1636,13 +1640,14 @@ static int launch_lenia_mfma_step(cah_state* s, int64_t t) {
     a.scheme = s->d.scheme;
     a.sp = s->d.scheme_p;
     a.sp_f32 = s->d.scheme_p_is_f32;
-    dim3 grid((unsigned)((s->n1 + LM_BX - 1) / LM_BX), (unsigned)((s->n2 + LM_BY - 1) / LM_BY));
+    const int64_t nbx = (s->n1 + LM_BX - 1) / LM_BX, nby = (s->n2 + LM_BY - 1) / LM_BY;
+    a.nbx = (int32_t)nbx;
+    const int32_t ntiles = (int32_t)(nbx * nby);
     if (int rc = checked(s->dry, "k_lenia_mfma", [&](lchk::Chk& c) {
-            lchk::grid_y(c, grid.y);
-            lchk::lenia_mfma(c, a, s->lenia_mode, grid.x, grid.y);
+            lchk::lenia_mfma(c, a, s->lenia_mode, nbx, nby, LM_KCHUNKS, s->frag_bytes);
         }))
         return rc;
-    if (!s->dry) launch_lenia_mfma(s->lenia_mode, grid, a, s->stream);
+    if (!s->dry) launch_lenia_mfma(s->lenia_mode, ntiles, a, s->stream);
     return CAH_OK;
 }
 
@@ -2987,7 +2992,8 @@ static int build_tick(cah_server* s) {
                             LeniaMfmaArgs a;
                             std::memcpy(&a, item + k * sizeof a, sizeof a);
                             a = swap(a);
-                            lchk::lenia_mfma(c, a, g.key, a.nbx, (a.L.n2 + LM_BY - 1) / LM_BY);
+                            lchk::lenia_mfma(c, a, g.key, a.nbx, (a.L.n2 + LM_BY - 1) / LM_BY, LM_KCHUNKS,
+                                             g.reqs[k]->st->frag_bytes);
                         }
                     }
             });

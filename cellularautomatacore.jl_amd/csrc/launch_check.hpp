@@ -360,7 +360,10 @@ inline void tile_kernel(Chk& c, const Lat& L, const void* cur, int64_t es, int T
 // input tile's float4 chunks (or cells, when the lattice is too small for the
 // chunked fill), the A fragments, the LDS rows and columns the MFMAs read, and
 // the outputs.
-inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int64_t gy) {
+// gx x gy: the lattice's 64 x 64 tiles; kchunks: the K-chunks the launched kernel
+// runs with; plane: the bytes of one plane of the A fragments the state holds.
+inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int64_t gy, int kchunks,
+                       size_t plane) {
     const Lat& L = a.L;
     const int H = a.H;
     c.need(H >= 1 && H <= LM_HMAX, "H = %d", H);
@@ -370,10 +373,16 @@ inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int
     if (!c.ok()) return;
     const int rows = LM_BY + 2 * H;
     c.need(rows <= LM_ROWS, "%d tile rows of %d", rows, LM_ROWS);
-    // the MFMAs read tile rows 16 t + n + dy <= 63 + 2H and columns 16 w + 32 c + 8 g + 7 <= 111
-    c.need(63 + 2 * H < rows && 48 + 32 + 24 + 8 <= LM_COLS && LM_COLS <= LM_STRIDE, "MFMA reads beyond the tile");
+    // the MFMAs read tile rows up to 63 + 2H and columns up to 16 w + 32 c + 8 g + 7 <= 111
+    const int nch = kchunks;
+    c.need(nch == 2, "%d K-chunks", nch);
+    c.need(a.nbx == gx, "nbx %d for %lld tiles a row", a.nbx, (long long)gx);
+    const int cols = 112;
+    const int stride = mode == LM_F16X2 ? lm_stride<LM_F16X2>() : lm_stride<LM_F16>();
+    c.need(63 + 2 * H < rows && cols <= LM_COLS && LM_COLS <= stride, "MFMA reads beyond the tile");
     const int np = mode == LM_F16X2 ? 2 : 1;
-    c.range(a.afrag, 0, (int64_t)np * (2 * H + 1) * 2 * 64 * 16, "A fragments");
+    c.need(plane == (size_t)(2 * H + 1) * nch * 64 * 16, "fragment plane of %zu bytes for H = %d", plane, H);
+    c.range(a.afrag, 0, (int64_t)np * (int64_t)plane, "A fragments");
     constexpr int NCH = LM_COLS / 4 + 1;
     const bool chunked = L.n1 % 4 == 0 && L.n1 >= 4 * NCH && L.n2 >= rows;
     int64_t clo = INT64_MAX, chi = INT64_MIN;

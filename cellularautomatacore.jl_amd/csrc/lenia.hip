@@ -4,23 +4,23 @@
 #include <cstring>
 #include <vector>
 
-#include "launch.hpp"
+#include "lenia_launch.hpp"
 
-void launch_lenia_mfma(int mode, dim3 grid, const LeniaMfmaArgs& a, hipStream_t st) {
-    if (mode == LM_BF16) k_lenia_mfma<LM_BF16><<<grid, 256, 0, st>>>(a);
-    else if (mode == LM_F16X2) k_lenia_mfma<LM_F16X2><<<grid, 256, 0, st>>>(a);
-    else k_lenia_mfma<LM_F16><<<grid, 256, 0, st>>>(a);
+void launch_lenia_mfma(int mode, int32_t ntiles, const LeniaMfmaArgs& a, hipStream_t st) {
+    if (mode == LM_BF16) k_lenia_mfma<LM_BF16><<<ntiles, LM_NT, 0, st>>>(a);
+    else if (mode == LM_F16X2) k_lenia_mfma<LM_F16X2><<<ntiles, LM_NT, 0, st>>>(a);
+    else k_lenia_mfma<LM_F16><<<ntiles, LM_NT, 0, st>>>(a);
 }
 
 void launch_lenia_mfma_batch(int mode, int nblocks, const LeniaMfmaArgs* items,
                              const int32_t* block_item, const int32_t* block_base,
                              int32_t parity, int64_t dt, hipStream_t st) {
     if (mode == LM_BF16)
-        k_lenia_mfma_batch<LM_BF16><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, dt);
+        k_lenia_mfma_batch<LM_BF16><<<nblocks, LM_NT, 0, st>>>(items, block_item, block_base, parity, dt);
     else if (mode == LM_F16X2)
-        k_lenia_mfma_batch<LM_F16X2><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, dt);
+        k_lenia_mfma_batch<LM_F16X2><<<nblocks, LM_NT, 0, st>>>(items, block_item, block_base, parity, dt);
     else
-        k_lenia_mfma_batch<LM_F16><<<nblocks, 256, 0, st>>>(items, block_item, block_base, parity, dt);
+        k_lenia_mfma_batch<LM_F16><<<nblocks, LM_NT, 0, st>>>(items, block_item, block_base, parity, dt);
 }
 
 // Fragment f = (dyi * 2 + c) * 64 + lane holds A_dy[m][k] for m = lane % 16 and
@@ -30,38 +30,39 @@ void launch_lenia_mfma_batch(int mode, int nblocks, const LeniaMfmaArgs* items,
 template <typename E>
 static void fill_fragments(int H, const double* wd, std::vector<E>& hi, std::vector<E>& lo) {
     const int K = 2 * H + 1;
-    const size_t nfrag = (size_t)K * 2 * 64;
+    const size_t nfrag = (size_t)K * LM_KCHUNKS * 64;
     hi.assign(nfrag * 8, (E)0.0f);
     lo.assign(nfrag * 8, (E)0.0f);
     for (int dyi = 0; dyi < K; ++dyi)
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < LM_KCHUNKS; ++c)
             for (int lane = 0; lane < 64; ++lane)
                 for (int i = 0; i < 8; ++i) {
                     const int m = lane % 16, k = 32 * c + 8 * (lane / 16) + i;
                     const int dx = k - H - m;
                     double w = 0.0;
                     if (dx >= -H && dx <= H) w = wd[(size_t)dyi * K + (dx + H)];
-                    const size_t at = (((size_t)dyi * 2 + c) * 64 + lane) * 8 + i;
+                    const size_t at = (((size_t)dyi * LM_KCHUNKS + c) * 64 + lane) * 8 + i;
                     const E h = (E)(float)w;
                     hi[at] = h;
                     lo[at] = (E)(float)(w - (double)(float)h);
                 }
 }
 
+size_t lenia_mfma_fragment_bytes(int H) { return (size_t)(2 * H + 1) * LM_KCHUNKS * 64 * 8 * 2; }
+
 void lenia_mfma_fragments(int mode, int H, const double* wd, std::vector<char>& buf) {
-    const size_t nfrag = (size_t)(2 * H + 1) * 2 * 64 * 8;
-    const size_t es = 2;
-    buf.assign(nfrag * es * 2, 0);
+    const size_t plane = lenia_mfma_fragment_bytes(H);
+    buf.assign(plane * 2, 0);
     if (mode == LM_BF16) {
         std::vector<__bf16> hi, lo;
         fill_fragments(H, wd, hi, lo);
-        std::memcpy(buf.data(), hi.data(), nfrag * es);
-        std::memcpy(buf.data() + nfrag * es, lo.data(), nfrag * es);
+        std::memcpy(buf.data(), hi.data(), plane);
+        std::memcpy(buf.data() + plane, lo.data(), plane);
     } else {
         std::vector<_Float16> hi, lo;
         fill_fragments(H, wd, hi, lo);
-        std::memcpy(buf.data(), hi.data(), nfrag * es);
-        std::memcpy(buf.data() + nfrag * es, lo.data(), nfrag * es);
+        std::memcpy(buf.data(), hi.data(), plane);
+        std::memcpy(buf.data() + plane, lo.data(), plane);
     }
 }
 

@@ -7,6 +7,7 @@
 
 #include "kernels_life_tb.hpp"
 #include "launch.hpp"
+#include "lenia_launch.hpp"
 #include "launch_check.hpp"
 
 static int bad = 0;
@@ -119,16 +120,26 @@ int main() {
         a.next = fn;
         a.L = F;
         a.H = 12;
+        a.nbx = 32;
         a.afrag = lchk::dry_alloc(fb, "fragments");
         lchk::Chk c("k_lenia_mfma, right");
-        lchk::lenia_mfma(c, a, LM_F16, 32, 32);
+        lchk::lenia_mfma(c, a, LM_F16, 32, 32, 2, fb);
         expect("k_lenia_mfma, right", true, c);
         lchk::Chk d("k_lenia_mfma, split fragments missing their low half");
-        lchk::lenia_mfma(d, a, LM_F16X2, 32, 32);
+        lchk::lenia_mfma(d, a, LM_F16X2, 32, 32, 2, fb);
         expect(d.kernel.c_str(), false, d);
+        lchk::Chk f("k_lenia_mfma, a tile count a row that is not the lattice's");
+        a.nbx = 31;
+        lchk::lenia_mfma(f, a, LM_F16, 32, 32, 2, fb);
+        expect(f.kernel.c_str(), false, f);
+        a.nbx = 32;
+        lchk::Chk g("k_lenia_mfma, fragments for another H (H = 8)");
+        a.H = 8;
+        lchk::lenia_mfma(g, a, LM_F16, 32, 32, 2, fb);
+        expect(g.kernel.c_str(), false, g);
         a.H = 25;
         lchk::Chk e("k_lenia_mfma, H past the tile");
-        lchk::lenia_mfma(e, a, LM_F16, 32, 32);
+        lchk::lenia_mfma(e, a, LM_F16, 32, 32, 2, fb);
         expect(e.kernel.c_str(), false, e);
     }
     // a batch table whose entries skip an item
