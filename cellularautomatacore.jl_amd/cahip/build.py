@@ -17,8 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 OBJ = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libcahip.so")
-UNITS = ["cahip.hip", "forest.hip", "forest_batch.hip", "lenia.hip", "life_r1.hip", "life_r2.hip", "life_r3.hip",
-         "life_r4.hip", "life_r5.hip"]
+UNITS = ["cahip.hip", "forest.hip", "forest_batch.hip", "forest_gated.hip", "lenia.hip", "life_r1.hip",
+         "life_r2.hip", "life_r3.hip", "life_r4.hip", "life_r5.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

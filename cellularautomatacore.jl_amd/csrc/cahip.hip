@@ -48,6 +48,16 @@ inline void launch_life(int R, int vb, bool bin, int niv, int blocks, const Life
     }
 }
 
+inline void launch_life_gated(int R, int vb, bool bin, int niv, int blocks, const LifeArgs& a, hipStream_t st) {
+    switch (R) {
+    case 1: launch_life_gated_r<1>(vb, bin, niv, blocks, a, st); break;
+    case 2: launch_life_gated_r<2>(vb, bin, niv, blocks, a, st); break;
+    case 3: launch_life_gated_r<3>(vb, bin, niv, blocks, a, st); break;
+    case 4: launch_life_gated_r<4>(vb, bin, niv, blocks, a, st); break;
+    default: launch_life_gated_r<5>(vb, bin, niv, blocks, a, st); break;
+    }
+}
+
 inline void launch_life_batch(int R, int vb, bool bin, int niv, int blocks, const LifeArgs* it,
                               const int32_t* wi, const int32_t* wb, int32_t nw, int32_t parity,
                               hipStream_t st) {
@@ -263,8 +273,11 @@ extern "C" int cah_check_problem(const cah_desc* d) {
 
 // ---------------------------------------------------------------- state
 
+// P_LIFE_GATED and P_FOREST_GATED: the band walks of Life / Generations and of the
+// forest fire under a Stochastic or Checkerboard gate or a mask (Generations
+// under any scheme), one sweep a launch
 enum Path { P_GENERIC, P_LIFE_SWAR, P_FOREST, P_LENIA_SEQ, P_LENIA_F32COL, P_LENIA_MFMA,
-            P_LENIA_FFT };
+            P_LENIA_FFT, P_LIFE_GATED, P_FOREST_GATED };
 
 struct cah_state {
     int device = 0;
@@ -349,32 +362,6 @@ static bool is_vn1(const std::vector<int32_t>& o) {
     std::set<std::pair<int, int>> s;
     for (int k = 0; k < 4; ++k) s.insert({o[2 * k], o[2 * k + 1]});
     return s == std::set<std::pair<int, int>>{{0, -1}, {-1, 0}, {1, 0}, {0, 1}};
-}
-
-// Runs of set bits of a count mask -> [lo, hi] intervals.
-// Runs of set bits of a count mask -> [lo, hi] intervals, as the byte-parallel
-// constants of k_life_swar; slots left over are neutral (lo = 128 matches no
-// count).  -1 when the mask has more than LIFE_MAX_IV runs.
-static int intervals(uint64_t m, uint32_t* lo, uint32_t* hi) {
-    for (int k = 0; k < LIFE_MAX_IV; ++k) {
-        lo[k] = 0;
-        hi[k] = 0;
-    }
-    int k = 0;
-    for (int c = 0; c < 64;) {
-        if (!((m >> c) & 1u)) {
-            ++c;
-            continue;
-        }
-        int e = c;
-        while (e + 1 < 64 && ((m >> (e + 1)) & 1u)) ++e;
-        if (k == LIFE_MAX_IV) return -1;
-        lo[k] = (uint32_t)(128 - c) * 0x01010101u;
-        hi[k] = (uint32_t)(127 - e) * 0x01010101u;
-        ++k;
-        c = e + 1;
-    }
-    return k;
 }
 
 // ---------------------------------------------------------------- device memory
@@ -671,20 +658,27 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
         s->bin_clean = true;
         return CAH_OK;
     }
-    const bool plain2d = d->ndims == 2 && d->scheme == CAH_SCHEME_SYNCHRONOUS && !d->mask;
-    if (plain2d && d->rule == CAH_RULE_LIFE) {
+    // the band walks: two-dimensional lattices, every gated scheme (cah_supports)
+    const bool plain = d->scheme == CAH_SCHEME_SYNCHRONOUS && !d->mask;
+    if (d->ndims == 2 && (d->rule == CAH_RULE_LIFE || d->rule == CAH_RULE_GENERATIONS)) {
         const int R = moore_radius(s->offsets);
         LifeArgs& la = s->life;
-        int nb = intervals(d->birth, la.b_lo, la.b_hi);
-        int ns = intervals(d->survive, la.s_lo, la.s_hi);
+        int nb = life_intervals(d->birth, la.b_lo, la.b_hi);
+        int ns = life_intervals(d->survive, la.s_lo, la.s_hi);
         if (R >= 1 && R <= 5 && nb >= 0 && ns >= 0) {
             la.nb = nb;
             la.ns = ns;
+            if (d->rule == CAH_RULE_GENERATIONS) {
+                const int C = d->gen_states;
+                la.gen_C = C;
+                la.gen_k = (uint32_t)std::max(0, 129 - C) * 0x01010101u;
+                la.gen_dying = (C > 2 ? 2u : 0u) * 0x01010101u;
+            }
             s->life_R = R;
-            s->path = P_LIFE_SWAR;
+            s->path = plain && d->rule == CAH_RULE_LIFE ? P_LIFE_SWAR : P_LIFE_GATED;
         }
-    } else if (plain2d && d->rule == CAH_RULE_FOREST_FIRE && is_vn1(s->offsets)) {
-        s->path = P_FOREST;
+    } else if (d->ndims == 2 && d->rule == CAH_RULE_FOREST_FIRE && is_vn1(s->offsets)) {
+        s->path = plain ? P_FOREST : P_FOREST_GATED;
     }
     s->bin_clean = s->tri_clean = true;  // the buffers start all zero
     return CAH_OK;
@@ -1081,21 +1075,44 @@ static int32_t band_rows(const cah_state* s, int64_t nbands, int R) {
     return (int32_t)std::max<int64_t>(1, std::min<int64_t>(rows, s->n2));
 }
 
+// Dwords a lane of the Life walks.  Two on wide lattices; the gated Moore(1)
+// walk, bound by the VALU, keeps one (Brian's Brain 8192^2 on MI355X: 33.2 us a
+// step with one, 36.2 with two).
 static int life_vb(const cah_state* s) {
-    int vb = s->tune_vb ? std::min(s->tune_vb, 2) : (s->n1 >= 2048 ? 2 : 1);
+    const bool wide = s->n1 >= 2048 && !(s->path == P_LIFE_GATED && s->life_R == 1);
+    int vb = s->tune_vb ? std::min(s->tune_vb, 2) : (wide ? 2 : 1);
     if (s->life_R == 5 && vb < 2) vb = 2;
     return vb;
 }
 
 static int life_niv(const LifeArgs& a) { return std::max(1, std::max(a.nb, a.ns)); }
 
-static int step_life(cah_state* s) {
+// The gate and mask of sweep `gate` of a step (the gated band walks).
+static BandGate band_gate(const cah_state* s, int gate) {
+    BandGate g{};
+    g.mask = s->mask_d;
+    g.scheme = s->d.scheme;
+    g.gate = gate;
+    g.sp_f32 = s->d.scheme_p_is_f32;
+    g.sp = s->d.scheme_p;
+    return g;
+}
+
+// One sweep of a Moore(R) Life rule (k_life_swar), or of Life or Generations
+// under sweep `gate` of a gated scheme and the mask (k_life_gated), at step t.
+static int step_life(cah_state* s, int64_t t, int gate) {
     LifeArgs a = s->life;
     a.cur = (const int8_t*)s->buf[s->cur];
     a.next = (int8_t*)s->buf[1 - s->cur];
     a.L = lat_of(s);
     a.cv1 = (int32_t)s->d.boundary_value[0];
     a.cv2 = (int32_t)s->d.boundary_value[1];
+    const bool gated = s->path == P_LIFE_GATED;
+    if (gated) {
+        a.g = band_gate(s, gate);
+        a.seed = s->d.seed;
+        a.t = t;
+    }
     // the no-conversion kernel needs every value it may read, the Constant
     // edges included, to be 0 or 1
     const bool edges01 = (s->d.boundary[0] != CAH_BOUNDARY_CONSTANT || (a.cv1 == 0 || a.cv1 == 1)) &&
@@ -1109,12 +1126,21 @@ static int step_life(cah_state* s) {
     a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
     const int blocks = waves_to_blocks(a.nwaves);
     const int R = s->life_R;
-    int rc = checked(s->dry, "k_life_swar", [&](lchk::Chk& c) {
+    int rc = checked(s->dry, gated ? "k_life_gated" : "k_life_swar", [&](lchk::Chk& c) {
         if (vb == 1) lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, R, 2 * R + 1);
         else lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, R, 2 * R + 1);
+        if (gated && a.g.mask) {
+            if (vb == 1) lchk::band_mask<1>(c, a.L, a.g.mask, a.nbands, a.rows, 2 * R + 1);
+            else lchk::band_mask<2>(c, a.L, a.g.mask, a.nbands, a.rows, 2 * R + 1);
+        }
     });
-    if (!rc && !s->dry) launch_life(R, vb, s->bin_clean, life_niv(a), blocks, a, s->stream);
-    s->bin_clean = saved;
+    if (!rc && !s->dry) {
+        if (gated) launch_life_gated(R, vb, s->bin_clean, life_niv(a), blocks, a, s->stream);
+        else launch_life(R, vb, s->bin_clean, life_niv(a), blocks, a, s->stream);
+    }
+    // Life keeps a lattice of 0 and 1 so (the gate and mask keep cells as they
+    // are); Generations writes 2 and more
+    s->bin_clean = saved && s->d.rule == CAH_RULE_LIFE;
     return rc;
 }
 
@@ -1201,7 +1227,9 @@ static void launch_life_tb_batch(int shape, int32_t nblocks, const LifeTbArgs* i
     }
 }
 
-static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
+// One sweep of the forest fire (k_forest_vn1), or under sweep `gate` of a gated
+// scheme and the mask (k_forest_gated), at step t.
+static int step_forest(cah_state* s, int64_t t, const cah_params* p, int gate) {
     ForestArgs a{};
     a.cur = (const int8_t*)s->buf[s->cur];
     a.next = (int8_t*)s->buf[1 - s->cur];
@@ -1214,9 +1242,11 @@ static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
     a.f = p->ff_f;
     a.p_f32 = p->ff_p_is_f32;
     a.f_f32 = p->ff_f_is_f32;
+    const bool gated = s->path == P_FOREST_GATED;
+    if (gated) a.g = band_gate(s, gate);
     // 4 bytes a lane: 78 VGPRs, six waves a SIMD to hide the Philox chains' latency
     // (8 bytes a lane takes 125 VGPRs, four waves: 2-3% slower at 16384^2 on MI355X)
-    const int vb = s->tune_vb ? std::min(s->tune_vb, 2) : 1;
+    const int vb = s->tune_vb && !gated ? std::min(s->tune_vb, 2) : 1;
     const int64_t ow = 62 * 4 * vb;
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
     a.rows = band_rows(s, a.nbands, 1);
@@ -1227,13 +1257,19 @@ static int step_forest(cah_state* s, int64_t t, const cah_params* p) {
         const double v = s->d.boundary_value[k];
         return s->d.boundary[k] != CAH_BOUNDARY_CONSTANT || v == 0 || v == 1 || v == 2;
     };
-    if (int rc = checked(s->dry, "k_forest_vn1", [&](lchk::Chk& c) {
+    if (int rc = checked(s->dry, gated ? "k_forest_gated" : "k_forest_vn1", [&](lchk::Chk& c) {
             if (vb == 1) lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, 1, 3);
             else lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, 4LL * blocks, 1, 3);
+            if (gated && a.g.mask) lchk::band_mask<1>(c, a.L, a.g.mask, a.nbands, a.rows, 3);
         }))
         return rc;
-    if (!s->dry) launch_forest(vb, small, s->tri_clean && edge012(0) && edge012(1), blocks, a, s->stream);
-    s->tri_clean = true;  // the rule writes 0, 1 and 2 only
+    const bool tri = s->tri_clean && edge012(0) && edge012(1);
+    if (!s->dry) {
+        if (gated) launch_forest_gated(small, tri, blocks, a, s->stream);
+        else launch_forest(vb, small, tri, blocks, a, s->stream);
+    }
+    // the rule writes 0, 1 and 2 only; a gate or mask keeps other values it finds
+    if (!gated) s->tri_clean = true;
     return CAH_OK;
 }
 
@@ -1701,8 +1737,10 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
         for (int g = 0; g < ngates(s->d); ++g) {
             int rc = CAH_OK;
             switch (s->path) {
-            case P_LIFE_SWAR: rc = step_life(s); break;
-            case P_FOREST: rc = step_forest(s, t, p); break;
+            case P_LIFE_SWAR:
+            case P_LIFE_GATED: rc = step_life(s, t, g); break;
+            case P_FOREST:
+            case P_FOREST_GATED: rc = step_forest(s, t, p, g); break;
             case P_LENIA_SEQ:
             case P_LENIA_FFT: rc = launch_lenia_growth(s, t, g); break;
             case P_LENIA_F32COL: rc = launch_lenia_f32col(s, t); break;

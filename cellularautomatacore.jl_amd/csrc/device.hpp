@@ -65,6 +65,31 @@ CAH_DEV float uniform32(uint64_t seed, uint64_t cellid, int64_t t, uint32_t k) {
     return (float)(c0 >> 8) * (1.0f / 16777216.0f);
 }
 
+// The Philox4x32-10 draw of cell rowid + J (1-based) at step t with counter word
+// c3 (the draw's k): r0, r1 are the first two output words.  With fewer than
+// 2^32 cells (SMALL) the cell id is a 32-bit word without carries, so the first
+// round's product M0 * (rowid + J) is one multiply-add, M0 * rowid + M0 * J.
+template <bool SMALL, int J>
+CAH_DEV void cell_philox(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, uint32_t& r0, uint32_t& r1) {
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint32_t c0, c1, c2 = (uint32_t)(uint64_t)t;
+    if constexpr (SMALL) {
+        c0 = (uint32_t)rowid;
+        c1 = 0;
+        const uint64_t p0 = (uint64_t)PHILOX_M0 * c0 + (uint64_t)PHILOX_M0 * (uint32_t)J;
+        philox_round(p0, c0, c1, c2, c3, k0, k1);
+        philox_rounds<1>(c0, c1, c2, c3, k0, k1);
+    } else {
+        const uint64_t cellid = rowid + (uint64_t)J;
+        c0 = (uint32_t)cellid;
+        c1 = (uint32_t)(cellid >> 32);
+        philox10(c0, c1, c2, c3, k0, k1);
+    }
+    r0 = c0;
+    r1 = c1;
+}
+
+
 // The 53-bit draw of _uniform, x = (r0 << 32 | r1) >> 11, so u = x * 2^-53.
 CAH_DEV uint64_t draw53(uint32_t r0, uint32_t r1) { return (((uint64_t)r0 << 32) | r1) >> 11; }
 
@@ -162,7 +187,7 @@ CAH_DEV bool gate_open(int32_t scheme, int32_t gate, double sp, int32_t sp_f32, 
 
 // Bytes equal to `v` (v in 0..255 replicated) -> 0x01 in those bytes, 0 elsewhere.
 // Exact for every byte value (no borrow crosses a byte).
-CAH_DEV uint32_t bytes_eq(uint32_t x, uint32_t vrep) {
+CAH_HD uint32_t bytes_eq(uint32_t x, uint32_t vrep) {
     const uint32_t t = x ^ vrep;                                   // 0 where equal
     const uint32_t nz = (((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
     return (nz >> 7) ^ 0x01010101u;

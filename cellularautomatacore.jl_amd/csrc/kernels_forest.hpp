@@ -1,7 +1,12 @@
-// kernels_forest.hpp -- the forest fire with VonNeumann(1), Synchronous, no mask:
-// the band walk of kernels_life.hpp with one Philox4x32-10 draw per cell.
+// kernels_forest.hpp -- the forest fire with VonNeumann(1): the band walk of
+// kernels_life.hpp with one Philox4x32-10 draw per cell.
+//
+//   k_forest_vn1    Synchronous, no mask
+//   k_forest_gated  any gate (Synchronous, Stochastic, Checkerboard), with or
+//                   without a mask
 #pragma once
 #include "band.hpp"
+#include "gate.hpp"
 
 struct ForestArgs {
     const int8_t* cur;
@@ -13,6 +18,7 @@ struct ForestArgs {
     int64_t t;
     double p, f;
     int32_t p_f32, f_f32;
+    BandGate g;  // k_forest_gated only
 };
 
 // The forest fire (README.md:56-60) on four cells at once.  Each cell forms
@@ -21,38 +27,16 @@ struct ForestArgs {
 // FP64 test u < p with u = x * 2^-53 (x the 53-bit draw) is made exactly as the
 // integer test x < thr53(p), the Float32 one as x24 < thr24(p) (device.hpp).
 // PREC says which of p (bit 0) and f (bit 1) are Float32 and use the 24-bit draw.
-// The cell is rowid + J, and k (1 or 2) comes in as c3.  With fewer than 2^32
-// cells (SMALL) the cell id is a 32-bit word without carries, so the first
-// round's product M0 * (rowid + J) is one multiply-add, M0 * rowid + M0 * J.
-// With both p and f in FP64 the walk tests the draw's high word against hi53
-// (device.hpp) and repeats a lane's draws through forest_draw only on a tie, so
-// the last round forms one word and the test is one 32-bit compare.
-template <bool SMALL, int J>
-CAH_DEV void forest_philox(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, uint32_t& r0, uint32_t& r1) {
-    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    uint32_t c0, c1, c2 = (uint32_t)(uint64_t)t;
-    if constexpr (SMALL) {
-        c0 = (uint32_t)rowid;
-        c1 = 0;
-        const uint64_t p0 = (uint64_t)PHILOX_M0 * c0 + (uint64_t)PHILOX_M0 * (uint32_t)J;
-        philox_round(p0, c0, c1, c2, c3, k0, k1);
-        philox_rounds<1>(c0, c1, c2, c3, k0, k1);
-    } else {
-        const uint64_t cellid = rowid + (uint64_t)J;
-        c0 = (uint32_t)cellid;
-        c1 = (uint32_t)(cellid >> 32);
-        philox10(c0, c1, c2, c3, k0, k1);
-    }
-    r0 = c0;
-    r1 = c1;
-}
-
+// The cell is rowid + J, and k (1 or 2) comes in as c3 of cell_philox
+// (device.hpp).  With both p and f in FP64 the walk tests the draw's high word
+// against hi53 (device.hpp) and repeats a lane's draws through forest_draw only
+// on a tie, so the last round forms one word and the test is one 32-bit compare.
 template <bool SMALL, int PREC, int J>
 CAH_DEV bool forest_draw(uint32_t c3, uint64_t rowid, int64_t t, uint64_t seed, uint64_t tp53,
                          uint64_t tf53, uint32_t tp24, uint32_t tf24) {
     const bool empty = c3 == 1u;
     uint32_t c0, c1;
-    forest_philox<SMALL, J>(c3, rowid, t, seed, c0, c1);
+    cell_philox<SMALL, J>(c3, rowid, t, seed, c0, c1);
     if constexpr (PREC == 0) {
         return draw53(c0, c1) < (empty ? tp53 : tf53);
     } else if constexpr (PREC == 3) {
@@ -91,8 +75,10 @@ CAH_DEV uint32_t forest_empty4(uint32_t x) {
 
 // The walk of one wave down rows [ys, ye) of its band, as life_walk: rows in a
 // ring of 6 register slots (row r in slot (r - ys + 1) mod 6), each load writing
-// its final slot, 3 rows in flight.
-template <int VB, bool SMALL, int PREC, bool FAST, bool TRI>
+// its final slot, 3 rows in flight.  GATED (k_forest_gated): a cell the mask or
+// the gate keeps is written back unchanged; the mask of row y is loaded right
+// after row y + 1, into a second ring (row r in slot (r - ys) mod 6).
+template <int VB, bool SMALL, int PREC, bool FAST, bool TRI, bool GATED>
 CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer) {
     constexpr int S = 6;
     const int8_t cv1 = (int8_t)a.cv1, cv2 = (int8_t)a.cv2;
@@ -107,6 +93,20 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
     const uint64_t tp53 = thr53(a.p), tf53 = thr53(a.f);
     const uint32_t tp24 = thr24((float)a.p), tf24 = thr24((float)a.f);
     const uint32_t hp53 = hi53(tp53), hf53 = hi53(tf53);
+    const bool masked = GATED && a.g.mask != nullptr;
+    const __amdgpu_buffer_rsrc_t rmask = row_rsrc(reinterpret_cast<const int8_t*>(a.g.mask));
+    uint32_t mring[GATED ? S : 1][VB];
+    auto mload = [&](int32_t r, uint32_t (&X)[VB]) {
+        if constexpr (GATED) {
+            if (masked) mask_row<VB, FAST>(a.g.mask, rmask, a.L, lm.src, x, r, X);
+        }
+    };
+    uint64_t sp53 = 0;
+    uint32_t sp24 = 0;
+    if constexpr (GATED) {
+        sp53 = thr53(a.g.sp);
+        sp24 = thr24((float)a.g.sp);
+    }
 
     uint32_t raw[S][VB], fire[S][VB];
     load(ys - 1, raw[0]);
@@ -114,6 +114,7 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
     Unroll<2, S>::run([&](auto K) {
         constexpr int k = decltype(K)::value;
         load(min(ys + k - 1, last), raw[k]);
+        mload(min(ys + k - 2, ye - 1), mring[GATED ? k - 2 : 0]);
     });
 #pragma unroll
     for (int i = 0; i < VB; ++i) {
@@ -146,7 +147,7 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
                 bool hit;
                 if constexpr (PREC == 0) {  // the high word decides all but a tie
                     uint32_t r0, r1;
-                    forest_philox<SMALL, 4 * i + b>(k, rowid, a.t, a.seed, r0, r1);
+                    cell_philox<SMALL, 4 * i + b>(k, rowid, a.t, a.seed, r0, r1);
                     const uint32_t lim = k == 1u ? hp53 : hf53;
                     hit = r0 < lim;
                     tie |= r0 == lim;
@@ -178,12 +179,24 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
                                 fire[s_up][i] | fire[s_dn][i];
             out[i] = forest_bytes(e4[i], fire[s_c][i], nf, hit4[i]);
         });
+        if constexpr (GATED) {
+            if (band_gated(a.g)) {
+                uint32_t open[VB];
+                band_open<VB, SMALL>(a.g, mring[ph], rowid, y, a.seed, a.t, sp53, sp24, open);
+#pragma unroll
+                for (int i = 0; i < VB; ++i) out[i] = open_select(open[i], out[i], raw[s_c][i]);
+            }
+        }
         if (writer) {
             if constexpr (FAST) store_mapped<VB>(rnext, a.L, x, y, out);
             else store_row<VB>(a.next, a.L, x, y, out);
         }
-        // the slot of row y - 1 now takes row y + 5 (clamped past the chunk)
-        if (more) load(min(y + 5, last), raw[s_up]);
+        // the slot of row y - 1 now takes row y + 5 (clamped past the chunk), and
+        // the mask ring the mask of row y + 4
+        if (more) {
+            load(min(y + 5, last), raw[s_up]);
+            mload(min(y + 4, ye - 1), mring[GATED ? (ph + 4) % S : 0]);
+        }
         ++y;
     };
     while (y + S <= ye) Unroll<0, S>::run([&](auto PH) { body(PH, true); });
@@ -192,11 +205,11 @@ CAH_DEV void forest_walk(const ForestArgs& a, int32_t x, int32_t ys, int32_t ye,
     });
 }
 
-template <int VB, bool SMALL, int PREC, bool TRI>
+template <int VB, bool SMALL, int PREC, bool TRI, bool GATED = false>
 CAH_DEV void forest_band(const ForestArgs& a, int wid) {
     const BandPart b = band_part<VB>(a.nbands, a.rows, a.L, wid, threadIdx.x & 63);
-    if (mapped_ok<VB>(a.L, 3)) forest_walk<VB, SMALL, PREC, true, TRI>(a, b.x, b.ys, b.ye, b.writer);
-    else forest_walk<VB, SMALL, PREC, false, TRI>(a, b.x, b.ys, b.ye, b.writer);
+    if (mapped_ok<VB>(a.L, 3)) forest_walk<VB, SMALL, PREC, true, TRI, GATED>(a, b.x, b.ys, b.ye, b.writer);
+    else forest_walk<VB, SMALL, PREC, false, TRI, GATED>(a, b.x, b.ys, b.ye, b.writer);
 }
 
 template <int VB, bool SMALL, int PREC, bool TRI>
@@ -204,6 +217,14 @@ __global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
     const int wid = wave_id();
     if (wid >= a.nwaves) return;
     forest_band<VB, SMALL, PREC, TRI>(a, wid);
+}
+
+// The forest fire under the gate and mask of a.g, one sweep (4 cells a lane).
+template <bool SMALL, int PREC, bool TRI>
+__global__ __launch_bounds__(256) void k_forest_gated(ForestArgs a) {
+    const int wid = wave_id();
+    if (wid >= a.nwaves) return;
+    forest_band<1, SMALL, PREC, TRI, true>(a, wid);
 }
 
 // ---------------------------------------------------------------- batched ensembles

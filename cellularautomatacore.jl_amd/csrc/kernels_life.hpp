@@ -1,12 +1,17 @@
-// kernels_life.hpp -- Life with Moore(R <= 5), Synchronous, no mask.
+// kernels_life.hpp -- Life and Generations with Moore(R <= 5).
 //
 // Each lane owns 4*VB consecutive cells of a row and walks down a column band,
 // keeping the vertical (2R+1)-row running count in registers and forming the
 // horizontal (2R+1)-wide sum with byte-parallel (SWAR) adds and funnel shifts;
 // the rule is a union of count intervals tested four cells at a time.  A wave
 // covers 64 lanes and writes the middle 62 (the end lanes are guards).
+//
+//   k_life_swar   Life, Synchronous, no mask
+//   k_life_gated  Life or Generations under any gate (Synchronous, Stochastic,
+//                 Checkerboard), with or without a mask
 #pragma once
 #include "band.hpp"
+#include "gate.hpp"
 
 constexpr int LIFE_MAX_IV = 2;
 
@@ -20,7 +25,40 @@ struct LifeArgs {
     // interval [lo, hi] of counts: bytes n with n + (128 - lo) >= 128 and
     // n + (127 - hi) < 128, replicated over the four bytes
     uint32_t b_lo[LIFE_MAX_IV], b_hi[LIFE_MAX_IV], s_lo[LIFE_MAX_IV], s_hi[LIFE_MAX_IV];
+    // k_life_gated only: the rule (gen_C = 0 Life, else Generations with gen_C
+    // states: gen_k = max(0, 129 - C) and gen_dying = (C > 2 ? 2 : 0), each
+    // replicated over the four bytes), the gate and mask, the seed and the step
+    int32_t gen_C;
+    uint32_t gen_k, gen_dying;
+    BandGate g;
+    uint64_t seed;
+    int64_t t;
 };
+
+// Runs of set bits of a count mask -> [lo, hi] intervals, as the byte-parallel
+// constants of life_rule and gen_rule (host side).  Slots left over hold 0, which
+// matches no count (n & ~n = 0).  -1 when the mask has more than LIFE_MAX_IV runs.
+inline int life_intervals(uint64_t m, uint32_t* lo, uint32_t* hi) {
+    for (int k = 0; k < LIFE_MAX_IV; ++k) {
+        lo[k] = 0;
+        hi[k] = 0;
+    }
+    int k = 0;
+    for (int c = 0; c < 64;) {
+        if (!((m >> c) & 1u)) {
+            ++c;
+            continue;
+        }
+        int e = c;
+        while (e + 1 < 64 && ((m >> (e + 1)) & 1u)) ++e;
+        if (k == LIFE_MAX_IV) return -1;
+        lo[k] = (uint32_t)(128 - c) * 0x01010101u;
+        hi[k] = (uint32_t)(127 - e) * 0x01010101u;
+        ++k;
+        c = e + 1;
+    }
+    return k;
+}
 
 // Horizontal (2R+1)-wide byte sums of the dwords E (G guard dwords on each side
 // of the lane's VB dwords): W[i] = sum over d in [-R, R] of the bytes at offset d.
@@ -60,7 +98,7 @@ CAH_DEV void hsum(const uint32_t (&E)[VB + 2 * G], uint32_t (&W)[VB]) {
 // NIV intervals per mask are tested; unused ones are neutral (lo = 128 never
 // matches a count below 128).
 template <int VB, int NIV>
-CAH_DEV void life_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t (&c)[VB],
+CAH_HD void life_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t (&c)[VB],
                        uint32_t (&out)[VB]) {
 #pragma unroll
     for (int i = 0; i < VB; ++i) {
@@ -75,6 +113,35 @@ CAH_DEV void life_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_
     }
 }
 
+// local_rule(::Generations) (src/rules.jl:446-456) on four cells a dword: c the
+// cells as stored, n their live counts (neighbours equal to 1).  A dead cell (0)
+// is born by the birth intervals; a live one (1) survives by the survive
+// intervals or starts dying (2; 0 with two states); any other value s moves to
+// s + 1 while s + 1 < C and to 0 after.  s + 1 < C is s <= C - 2 as a signed
+// byte: every negative byte, and a byte s7 < 128 with s7 + (129 - C) < 128 (C is
+// at most 129 here: a larger C passes every byte).  Exact for every byte value.
+template <int VB, int NIV>
+CAH_HD void gen_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t (&c)[VB], uint32_t (&out)[VB]) {
+#pragma unroll
+    for (int i = 0; i < VB; ++i) {
+        uint32_t inb = 0, ins = 0;
+#pragma unroll
+        for (int k = 0; k < NIV; ++k) {
+            inb |= (n[i] + a.b_lo[k]) & ~(n[i] + a.b_hi[k]);
+            ins |= (n[i] + a.s_lo[k]) & ~(n[i] + a.s_hi[k]);
+        }
+        const uint32_t born = (inb & 0x80808080u) >> 7, keep = (ins & 0x80808080u) >> 7;
+        const uint32_t z = bytes_eq(c[i], 0u), o = bytes_eq(c[i], 0x01010101u);
+        const uint32_t zm = (z << 8) - z, om = (o << 8) - o, km = (keep << 8) - keep;
+        const uint32_t c7 = c[i] & 0x7F7F7F7Fu, sign = c[i] & 0x80808080u;
+        const uint32_t inc = (c7 + 0x01010101u) ^ sign;  // s + 1 in each byte, no carry between bytes
+        const uint32_t le = ((sign | ~(c7 + a.gen_k)) & 0x80808080u) >> 7;
+        const uint32_t older = inc & ((le << 8) - le);
+        const uint32_t live = keep | (~km & a.gen_dying);
+        out[i] = (zm & born) | (om & live) | (~(zm | om) & older);
+    }
+}
+
 // The walk of one wave down rows [ys, ye) of its band (ys, ye wave-uniform), or
 // up them when `up`: the walk runs over rows y' = ys + ye - 1 - y (the Moore sums
 // are symmetric in the row offset).  FAST: the lattice suits load_mapped, so every
@@ -83,7 +150,17 @@ CAH_DEV void life_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_
 // ring of 2P = 4R+2 register slots, row r in slot (r - ys + R) mod 2P, and each
 // load writes its final slot directly: the slot of the row that just left the
 // window.  P = 2R+1 rows are in flight per wave.
-template <int R, int VB, bool BIN, int NIV, bool FAST>
+//
+// GATED (k_life_gated): the ring keeps the cells as stored and the counts take
+// the bytes equal to 1 (for R <= 2 converted once, into a ring of their own
+// beside it; wider walks convert a row when it enters and when it leaves the
+// count, to stay within the registers; BIN: the cells are 0 or 1 and count as
+// they are, and a kept cell stays 0 or 1); the rule is Life or Generations; a cell
+// the mask or the gate keeps is written back unchanged.  The mask rows go
+// through a further ring (row r in slot (r - ys) mod 2P): the mask of row y is
+// loaded right after row y + R, the row whose wait precedes row y's output, so
+// it adds no wait of its own (loads complete in order).
+template <int R, int VB, bool BIN, int NIV, bool FAST, bool GATED>
 CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer, bool up) {
     constexpr int P = 2 * R + 1, S = 2 * P;
     constexpr int G = (R + 3) / 4;  // guard dwords needed on each side
@@ -100,18 +177,53 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         else load_row<VB>(a.cur, a.L, cv1, cv2, x, ya, X);
     };
     auto ones = [&](uint32_t (&X)[VB]) {
-        if constexpr (!BIN) {
+        if constexpr (!BIN && !GATED) {
 #pragma unroll
             for (int i = 0; i < VB; ++i) X[i] = bytes_eq(X[i], 0x01010101u);
         }
     };
-
+    // a cell's contribution to the counts (the ring holds it converted unless
+    // GATED); ONES: the gated walk's converted rows, in slots of their own
+    constexpr bool ONES = GATED && !BIN && R <= 2;
     uint32_t ring[S][VB];
-    // rows ys-R .. ys+R-1 counted now; rows ys+R .. ys+R+P in flight
+    uint32_t one[ONES ? S : 1][VB];
+    auto live = [&](uint32_t v) -> uint32_t {
+        if constexpr (GATED && !BIN) return bytes_eq(v, 0x01010101u);
+        else return v;
+    };
+    // the counts of ring slot k: converted now (into its ONES slot), or again
+    auto enter = [&](int k, int i) -> uint32_t {
+        if constexpr (ONES) return one[k][i] = live(ring[k][i]);
+        else return live(ring[k][i]);
+    };
+    auto leave = [&](int k, int i) -> uint32_t {
+        if constexpr (ONES) return one[k][i];
+        else return live(ring[k][i]);
+    };
+    // the mask (GATED): its rows [ys, ye) only, the clamp repeats the chunk's last
+    const bool masked = GATED && a.g.mask != nullptr;
+    const __amdgpu_buffer_rsrc_t rmask = row_rsrc(reinterpret_cast<const int8_t*>(a.g.mask));
+    uint32_t mring[GATED ? S : 1][VB];
+    auto mload = [&](int32_t r, uint32_t (&X)[VB]) {
+        if constexpr (GATED) {
+            if (masked) mask_row<VB, FAST>(a.g.mask, rmask, a.L, lm.src, x, up ? mirror - r : r, X);
+        }
+    };
+    uint64_t sp53 = 0;
+    uint32_t sp24 = 0;
+    if constexpr (GATED) {
+        sp53 = thr53(a.g.sp);
+        sp24 = thr24((float)a.g.sp);
+    }
+    const bool small = (uint64_t)a.L.n1 * (uint64_t)a.L.n2 < 0xFFFFFFFFull;
+
+    // rows ys-R .. ys+R-1 counted now; rows ys+R .. ys+R+P in flight, each with the
+    // mask of the row R above it
     Unroll<0, 2 * R>::run([&](auto K) { load(ys - R + decltype(K)::value, ring[decltype(K)::value]); });
     Unroll<2 * R, S>::run([&](auto K) {
         constexpr int k = decltype(K)::value;
         load(min(ys + k - R, last), ring[k]);
+        mload(min(ys + k - 2 * R, ye - 1), mring[GATED ? k - 2 * R : 0]);
     });
     uint32_t V[VB];
 #pragma unroll
@@ -120,7 +232,7 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         constexpr int k = decltype(K)::value;
         ones(ring[k]);
 #pragma unroll
-        for (int i = 0; i < VB; ++i) V[i] += ring[k][i];
+        for (int i = 0; i < VB; ++i) V[i] += enter(k, i);
     });
 
     int32_t y = ys;
@@ -131,7 +243,7 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         constexpr int s_old = ph % S;            // row y - R
         ones(ring[s_new]);
 #pragma unroll
-        for (int i = 0; i < VB; ++i) V[i] += ring[s_new][i];
+        for (int i = 0; i < VB; ++i) V[i] += enter(s_new, i);
         uint32_t E[VB + 2 * G];
 #pragma unroll
         for (int j = 0; j < G; ++j) {
@@ -140,21 +252,41 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         }
 #pragma unroll
         for (int i = 0; i < VB; ++i) E[G + i] = V[i];
-        uint32_t Wd[VB], n[VB], out[VB];
+        uint32_t Wd[VB], n[VB], c1[VB], out[VB];
         hsum<R, VB, G>(E, Wd);
 #pragma unroll
-        for (int i = 0; i < VB; ++i) n[i] = Wd[i] - ring[s_c][i];  // Moore leaves the centre out
-        life_rule<VB, NIV>(a, n, ring[s_c], out);
+        for (int i = 0; i < VB; ++i) {
+            c1[i] = leave(s_c, i);
+            n[i] = Wd[i] - c1[i];  // Moore leaves the centre out
+        }
+        const int32_t yr = up ? mirror - y : y;  // the row written
+        if constexpr (GATED) {
+            if (a.gen_C) gen_rule<VB, NIV>(a, n, ring[s_c], out);
+            else life_rule<VB, NIV>(a, n, c1, out);
+            if (band_gated(a.g)) {
+                uint32_t open[VB];
+                const uint64_t rowid = (uint64_t)yr * (uint64_t)a.L.n1 + (uint64_t)(int64_t)x + 1;
+                if (small) band_open<VB, true>(a.g, mring[ph % S], rowid, yr, a.seed, a.t, sp53, sp24, open);
+                else band_open<VB, false>(a.g, mring[ph % S], rowid, yr, a.seed, a.t, sp53, sp24, open);
+#pragma unroll
+                for (int i = 0; i < VB; ++i) out[i] = open_select(open[i], out[i], ring[s_c][i]);
+            }
+        } else {
+            life_rule<VB, NIV>(a, n, c1, out);
+        }
         if (writer) {
-            if constexpr (FAST) store_mapped<VB>(rnext, a.L, x, (up ? mirror - y : y), out);
-            else store_row<VB>(a.next, a.L, x, (up ? mirror - y : y), out);
+            if constexpr (FAST) store_mapped<VB>(rnext, a.L, x, yr, out);
+            else store_row<VB>(a.next, a.L, x, yr, out);
         }
 #pragma unroll
-        for (int i = 0; i < VB; ++i) V[i] -= ring[s_old][i];
+        for (int i = 0; i < VB; ++i) V[i] -= leave(s_old, i);
         // the slot of row y - R now takes row y - R + 2P = y + R + P + 1
         // (clamped: rows past the chunk are never read, the clamp repeats the
-        // last row)
-        if (more) load(min(y + R + P + 1, last), ring[s_old]);
+        // last row), and the mask ring the mask of row y + P + 1
+        if (more) {
+            load(min(y + R + P + 1, last), ring[s_old]);
+            mload(min(y + P + 1, ye - 1), mring[GATED ? (ph + P + 1) % S : 0]);
+        }
         ++y;
     };
     // whole blocks of 2P rows, then the remaining (< 2P) rows
@@ -168,11 +300,11 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 // wave-uniform, so the row bookkeeping stays in scalar registers).  Odd chunks
 // walk up: the two chunks beside a chunk boundary then read the rows around it
 // at the same time, so the second read finds them in L2.
-template <int R, int VB, bool BIN, int NIV>
+template <int R, int VB, bool BIN, int NIV, bool GATED = false>
 CAH_DEV void life_band(const LifeArgs& a, int wid) {
     const BandPart b = band_part<VB>(a.nbands, a.rows, a.L, wid, threadIdx.x & 63);
-    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true>(a, b.x, b.ys, b.ye, b.writer, b.up);
-    else life_walk<R, VB, BIN, NIV, false>(a, b.x, b.ys, b.ye, b.writer, b.up);
+    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true, GATED>(a, b.x, b.ys, b.ye, b.writer, b.up);
+    else life_walk<R, VB, BIN, NIV, false, GATED>(a, b.x, b.ys, b.ye, b.writer, b.up);
 }
 
 template <int R, int VB, bool BIN, int NIV>
@@ -180,6 +312,15 @@ __global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
     const int wid = wave_id_xcd();
     if (wid >= a.nwaves) return;  // whole waves only
     life_band<R, VB, BIN, NIV>(a, wid);
+}
+
+// Life or Generations (a.gen_C) under the gate and mask of a.g, one sweep;
+// NIV intervals per count mask; BIN: every cell read is 0 or 1.
+template <int R, int VB, int NIV, bool BIN>
+__global__ __launch_bounds__(256) void k_life_gated(LifeArgs a) {
+    const int wid = wave_id_xcd();
+    if (wid >= a.nwaves) return;
+    life_band<R, VB, BIN, NIV, true>(a, wid);
 }
 
 // ---------------------------------------------------------------- batched ensembles

@@ -1,6 +1,7 @@
 // launch.hpp -- host-side launchers of the band-walk kernels.  The kernels of
 // each family (and each Life radius) are instantiated in their own translation
-// unit (life_r1.hip .. life_r5.hip, forest.hip) so that they compile in parallel.
+// unit (life_r1.hip .. life_r5.hip, forest.hip, forest_batch.hip,
+// forest_gated.hip) so that they compile in parallel.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,6 +13,10 @@
 // Life, Moore(R): vb in {1, 2} (R = 5 uses 2); niv = intervals per mask (1 or 2).
 template <int R>
 void launch_life_r(int vb, bool bin, int niv, int blocks, const LifeArgs& a, hipStream_t st);
+// Life or Generations under a gate and/or a mask (k_life_gated): vb in {1, 2};
+// bin = every cell read is 0 or 1; niv = intervals per mask (1 or 2).
+template <int R>
+void launch_life_gated_r(int vb, bool bin, int niv, int blocks, const LifeArgs& a, hipStream_t st);
 template <int R>
 void launch_life_batch_r(int vb, bool bin, int niv, int blocks, const LifeArgs* items,
                          const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,
@@ -20,6 +25,8 @@ void launch_life_batch_r(int vb, bool bin, int niv, int blocks, const LifeArgs* 
 // forest fire, VonNeumann(1): vb in {1, 2}; small = fewer than 2^32 cells; tri =
 // the lattice and its Constant edges hold only 0, 1 and 2.
 void launch_forest(int vb, bool small, bool tri, int blocks, const ForestArgs& a, hipStream_t st);
+// The same under a gate and/or a mask (k_forest_gated, 4 cells a lane).
+void launch_forest_gated(bool small, bool tri, int blocks, const ForestArgs& a, hipStream_t st);
 // The batch items carry p, f and their time base; prec = p_f32 | f_f32 << 1;
 // vb in {1, 2}; tri: every item's lattice and Constant edges hold only 0, 1 and 2.
 void launch_forest_batch(int vb, bool small, bool tri, int prec, int blocks, const ForestArgs* items,

@@ -252,6 +252,48 @@ inline void band_walk(Chk& c, const Lat& L, const int8_t* cur, const int8_t* nex
     if (wcol_hi > INT64_MIN) c.range(next, 0, (int64_t)(L.n2 - 1) * L.pitch + wcol_hi, "band walk writes");
 }
 
+// The mask reads of a gated band walk (mask_row, gate.hpp): the mask is n1 x n2
+// bytes without pitch, and each wave reads the rows of its chunk, [ys, ye), at
+// each lane's 4*VB bytes: the lane map's source group in a mapped walk (32-bit
+// buffer offsets), else the bytes of [x, x + 4*VB) that lie in [0, n1).
+template <int VB>
+inline void band_mask(Chk& c, const Lat& L, const uint8_t* mask, int32_t nbands, int32_t rows, int halo_rows) {
+    constexpr int W = 4 * VB;
+    const bool fast = mapped_ok<VB>(L, halo_rows);
+    int64_t col_lo = INT64_MAX, col_hi = INT64_MIN;
+    for (int32_t band = 0; band < nbands && c.ok(); ++band) {
+        for (int lane = 0; lane < 64; ++lane) {
+            const BandPart b = band_part<VB>(nbands, rows, L, band, lane);
+            int64_t lo, hi;
+            if (fast) {
+                const LaneMap m = lane_map<VB>(L, b.x);
+                c.need(m.src >= 0 && m.src + W <= L.n1 && m.src % W == 0, "mask: lane map of column %d gives %d",
+                       b.x, m.src);
+                if (b.writer) c.need(m.src == b.x, "mask: writer lane at column %d reads group %d", b.x, m.src);
+                lo = m.src;
+                hi = m.src + W;
+            } else {
+                lo = b.x < 0 ? 0 : b.x;
+                hi = b.x + W > L.n1 ? L.n1 : b.x + W;
+                if (lo >= hi) continue;
+            }
+            col_lo = lo < col_lo ? lo : col_lo;
+            col_hi = hi > col_hi ? hi : col_hi;
+        }
+    }
+    const int64_t chunks = (L.n2 + rows - 1) / rows;
+    int64_t row_lo = INT64_MAX, row_hi = INT64_MIN;
+    for (int64_t ch = 0; ch < chunks && c.ok(); ++ch) {
+        const BandPart b = band_part<VB>(nbands, rows, L, (int)(ch * nbands), 0);
+        c.need(b.ys >= 0 && b.ys < b.ye && b.ye <= L.n2, "mask: chunk %lld rows [%d, %d)", (long long)ch, b.ys, b.ye);
+        row_lo = b.ys < row_lo ? b.ys : row_lo;
+        row_hi = b.ye - 1 > row_hi ? b.ye - 1 : row_hi;
+    }
+    if (!c.ok() || col_lo > col_hi || row_lo > row_hi) return;
+    if (fast) c.need((int64_t)L.n1 * L.n2 < ((int64_t)1 << 31), "mask: 32-bit buffer offsets overflow");
+    c.range(mask, row_lo * L.n1 + col_lo, row_hi * L.n1 + col_hi, "band walk mask reads");
+}
+
 // The wave (or block) tables of a batched launch: entry w of `map` names item
 // map[w], of which it is the (w - base[item])-th; items are listed in order and
 // each gets exactly `count(item)` consecutive entries.
