@@ -499,10 +499,12 @@ def bench_lenia(be, ev, steps=500):
         algo_tflops = 2 * taps * n * n / sec / 1e12
         out[field]["hbm_gbs_algorithmic"] = round(8 * n * n / sec / 1e9, 1)
         if field.startswith("mfma"):
-            # one 64 x 64 block: (2H+1) row offsets x 2 K-chunks x 4 row tiles x 4
-            # waves of 16x16x32 MFMAs (16384 flop each), three per product when split
-            H = 12
-            per_block = (2 * H + 1) * 2 * 4 * 4 * 2 * 16 * 16 * 32 * (3 if field == "mfma_f16x2" else 1)
+            # one 64 x 64 block: 4 waves x (row-offset pairs over the rows holding
+            # weights) x 12 16x16x32 MFMAs (6 input chunks, 4 strips, 3 chunk offsets
+            # each; 16384 flop an MFMA), three per product when split
+            dys = np.array([o[1] for o, w in zip(*prob.taps) if w != 0])
+            npairs = (int(dys.max() - dys.min()) + 2) // 2
+            per_block = 4 * npairs * 12 * 2 * 16 * 16 * 32 * (3 if field == "mfma_f16x2" else 1)
             exec_tflops = per_block * (n // 64) ** 2 / sec / 1e12
             out[field]["roofline"] = {
                 "bound": "mfma", "kernel": "k_lenia_mfma", "achieved": round(algo_tflops, 1),
@@ -655,6 +657,70 @@ def latency_pair(probs, arr, budget=200e9):
                       "p99_ms": round(float(np.percentile(lp, 99)), 2),
                       "sims_per_s": round(n / fp.max(), 1)},
             "identical_results": bool(all(np.array_equal(a, b) for a, b in zip(rs, rp)))}
+
+
+def bench_gated(be, ev, steps=50):
+    """The sweep beyond the two headline kernels, each lattice timed alone (a
+    window of `steps` steps after 100 ms of untimed ones): Generations, the
+    Stochastic and Checkerboard gates and a mask on the gated band walks
+    (k_life_gated, k_forest_gated), and a neighbourhood no band walk takes
+    (VonNeumann(2)) on the general LDS-tile kernel (k_generic_i8).  Roofline:
+    algorithmic bytes (one Int8 read and write a cell, and a mask byte read
+    where there is a mask) against 8 TB/s."""
+    from cahip import model as M
+    from cahip.backend import check, make_params
+    from cahip.problem import CAProblem
+    n1, n2 = 8192, 16384
+    torus = M.Torus(n1, n1)
+    refl = M.Grid(n2, n2, boundary=M.Reflecting())
+    ff = {"p": 0.03, "f": 1e-4}
+    i16 = np.arange(n2, dtype=np.int16)
+    mask = ((i16[:, None] + i16[None, :]) % 7 != 0)  # 6 in 7 cells open, in diagonal stripes
+    cases = [
+        ("brians_brain_8192", "Generations brians_brain (B2/S/3) Moore(1) Torus 8192^2, Bernoulli(0.3) seed 1, "
+         "Synchronous", CAProblem(M.Generations("brians_brain"), np.zeros((n1, n1), np.int8), 1, space=torus),
+         M.Synchronous(), (1, 0.3), 0),
+        ("life_stochastic_8192", "Life B3/S23 Moore(1) Torus 8192^2, Bernoulli(0.5) seed 1, Stochastic(0.5) seed 3",
+         CAProblem(M.Life(), np.zeros((n1, n1), np.int8), 1, space=torus, seed=3), M.Stochastic(0.5), (1, 0.5), 0),
+        ("ltl_checkerboard_8192", "Larger-than-Life Moore(5) B34..45/S34..58 Torus 8192^2, Bernoulli(0.5) seed 1, "
+         "Checkerboard (two half-sweeps a step)",
+         CAProblem(M.Life(range(34, 46), range(34, 59), neighbourhood=M.Moore(5)), np.zeros((n1, n1), np.int8), 1,
+                   space=torus), M.Checkerboard(), (1, 0.5), 0),
+        ("forest_stochastic_16384", "forest fire VN(1) Reflecting 16384^2 all empty p=0.03 f=1e-4 seed 7, "
+         "Stochastic(0.5)", CAProblem(M.ForestFire(), np.zeros((n2, n2), np.int8), 1, ff, space=refl, seed=7),
+         M.Stochastic(0.5), None, 0),
+        ("forest_masked_16384", "forest fire VN(1) Reflecting 16384^2 all empty p=0.03 f=1e-4 seed 7, "
+         "Synchronous, mask (i + j) % 7 != 0",
+         CAProblem(M.ForestFire(), np.zeros((n2, n2), np.int8), 1, ff, space=refl, seed=7, mask=mask),
+         M.Synchronous(), None, 1),
+        ("life_vn2_8192", "Life B3/S23 VonNeumann(2) Torus 8192^2, Bernoulli(0.5) seed 1, Synchronous "
+         "(the general kernel)", CAProblem(M.Life("B3/S23", neighbourhood=M.VonNeumann(2)),
+                                            np.zeros((n1, n1), np.int8), 1, space=torus),
+         M.Synchronous(), (1, 0.5), 0),
+    ]
+    out = {}
+    for name, what, prob, scheme, fill, mask_bytes in cases:
+        st = be.allocate_state(prob, scheme)
+        if fill:
+            check(be.lib.cah_state_fill_bernoulli(st.handle, *fill))
+        kernels = sorted(be.check_launches(prob, scheme))
+        cp = make_params(prob, prob.p)
+        stream, t = be.stream(st), 0
+        spent = 0.0
+        while spent < 100.0:
+            spent += timed(ev, stream, lambda: check(be.lib.cah_advance(st.handle, t, 10, ctypes.byref(cp))))
+            t += 10
+        ms = timed(ev, stream, lambda: check(be.lib.cah_advance(st.handle, t, steps, ctypes.byref(cp)))) / steps
+        be.free_state(st)
+        cells = prob.u0.size
+        gbs = cells * (2 + mask_bytes) / (ms * 1e-3) / 1e9
+        out[name] = {"workload": what, "kernels": kernels, "ms_per_step": round(ms, 5),
+                     "gcups": round(cells / (ms * 1e-3) / 1e9, 2),
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                  "bytes_per_cell": 2 + mask_bytes}}
+    del mask
+    return out
 
 
 def bench_serving():
@@ -911,6 +977,7 @@ def main():
                    "field_epidemic_2048": lambda: bench_field_epidemic(be, ev),
                    "counts_16384": lambda: bench_counts(be, ev),
                    "sparse_16384": lambda: bench_sparse(be, ev),
+                   "gated_sweeps": lambda: bench_gated(be, ev),
                    "serving": bench_serving,
                    "serving_ensembles": bench_serving_ensembles}
         names = list(configs) if args.configs is None else args.configs.split(",")

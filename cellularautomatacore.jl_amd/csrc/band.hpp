@@ -112,15 +112,10 @@ CAH_DEV int wave_id() {
     return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
 }
 
-// The wave index in an XCD-aware block order.  Workgroups are dealt to the eight
-// XCDs round robin (block b to XCD b % 8), and each XCD has its own L2.
-// Numbering the blocks so that XCD x runs one contiguous range of them puts
-// neighbouring row chunks, which read each other's edge rows, on one L2.
+// The wave index in the XCD-aware block order of block_id_xcd: neighbouring row
+// chunks, which read each other's edge rows, share one L2.
 CAH_DEV int wave_id_xcd() {
-    const int n = gridDim.x, b = blockIdx.x;
-    const int q = n >> 3, r = n & 7, xcd = b & 7;
-    const int lb = xcd * q + min(xcd, r) + (b >> 3);
-    return __builtin_amdgcn_readfirstlane(lb * (blockDim.x / 64) + threadIdx.x / 64);
+    return __builtin_amdgcn_readfirstlane(block_id_xcd() * (blockDim.x / 64) + threadIdx.x / 64);
 }
 
 

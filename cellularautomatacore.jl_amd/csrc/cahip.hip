@@ -321,6 +321,7 @@ struct cah_state {
     LeniaFFT* fft = nullptr;
     int32_t halo = 1;
     int32_t lenia_H = 0;
+    int32_t lenia_dy0 = 0, lenia_np = 0;  // MFMA field: first row offset, row-offset pairs
     Path path = P_GENERIC;
     int32_t life_R = 0;
     bool bin_clean = false;  // every cell is 0 or 1 (Life only)
@@ -625,8 +626,9 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
             for (int k = 0; k < d->n_taps; ++k)
                 wd[(size_t)(s->taps[2 * k + 1] + ht) * Kd + (s->taps[2 * k] + ht)] += s->weights[k];
             s->lenia_H = ht;
+            lenia_mfma_pairs(ht, wd.data(), s->lenia_dy0, s->lenia_np);
             std::vector<char> frag;
-            lenia_mfma_fragments(s->lenia_mode, ht, wd.data(), frag);
+            lenia_mfma_fragments(s->lenia_mode, ht, wd.data(), s->lenia_dy0, s->lenia_np, frag);
             s->frag_bytes = frag.size() / 2;  // one plane
             return upload(s, (char**)&s->frag_d, frag.data(), frag.size());
         }
@@ -1665,6 +1667,8 @@ static int launch_lenia_mfma_step(cah_state* s, int64_t t) {
     a.afrag = s->frag_d;
     a.L = lat_of(s);
     a.H = s->lenia_H;
+    a.dy0 = s->lenia_dy0;
+    a.npairs = s->lenia_np;
     a.cv1 = (float)s->d.boundary_value[0];
     a.cv2 = (float)s->d.boundary_value[1];
     a.prec32 = s->d.lenia_precision == 32;
@@ -1680,7 +1684,7 @@ static int launch_lenia_mfma_step(cah_state* s, int64_t t) {
     a.nbx = (int32_t)nbx;
     const int32_t ntiles = (int32_t)(nbx * nby);
     if (int rc = checked(s->dry, "k_lenia_mfma", [&](lchk::Chk& c) {
-            lchk::lenia_mfma(c, a, s->lenia_mode, nbx, nby, LM_KCHUNKS, s->frag_bytes);
+            lchk::lenia_mfma(c, a, s->lenia_mode, nbx, nby, s->frag_bytes);
         }))
         return rc;
     if (!s->dry) launch_lenia_mfma(s->lenia_mode, ntiles, a, s->stream);
@@ -1816,7 +1820,15 @@ struct cah_batch {
     void* items_d = nullptr;
     int32_t* wave_item_d = nullptr;
     int32_t* wave_base_d = nullptr;
+    // the steps of one cah_batch_advance, captured as a graph, and what it was
+    // captured for: step count, buffer parity before and after, the short byte
+    // tests; the forest fire's starting time is read from t0_d, so one graph
+    // serves every t0
     hipGraphExec_t exec = nullptr;
+    int64_t g_nsteps = -1;
+    int g_parity0 = -1, g_parity1 = 0;
+    bool g_clean = false;
+    int64_t* t0_d = nullptr;
     std::vector<ForestArgs> fitems;  // host copy of the forest items (p and f change)
     bool fp_set = false;
     cah_params fp{};
@@ -1844,7 +1856,7 @@ static void free_batch(cah_batch* b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
     for (void* p : {b->items_d, (void*)b->wave_item_d, (void*)b->wave_base_d, (void*)b->counts_d,
-                    (void*)b->tb_items_d, (void*)b->tb_item_d, (void*)b->tb_base_d})
+                    (void*)b->tb_items_d, (void*)b->tb_item_d, (void*)b->tb_base_d, (void*)b->t0_d})
         dfree(p, b->dry);
     for (auto* m : b->members) free_state(m);
     if (b->stream) hipStreamDestroy(b->stream);
@@ -2140,6 +2152,7 @@ static int check_batch(cah_batch* b, int par) {
                 const int R = b->kind == B_FOREST ? 1 : b->life_R;
                 if (b->kind == B_FOREST) {
                     const ForestArgs a = swap(b->fitems[k]);
+                    if (k == 0) c.range(b->t0_d, 0, 8, "batch time base");
                     if (b->vb == 1) lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, 1, 3);
                     else lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, 1, 3);
                 } else {
@@ -2156,7 +2169,13 @@ static int check_batch(cah_batch* b, int par) {
     return CAH_OK;
 }
 
-static void launch_batch_step(cah_batch* b, int64_t t, bool clean) {
+// One 8-byte value into device memory in stream order.
+__global__ void k_store_i64(int64_t* p, int64_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+// Step k of a cah_batch_advance (the forest fire's time: *t0_d + k).
+static void launch_batch_step(cah_batch* b, int64_t k, bool clean) {
     const int blocks = waves_to_blocks(b->nwaves);
     if (b->dry) {
         b->parity ^= 1;
@@ -2164,7 +2183,7 @@ static void launch_batch_step(cah_batch* b, int64_t t, bool clean) {
     }
     if (b->kind == B_FOREST)
         launch_forest_batch(b->vb, b->small, clean, forest_prec(b->fp), blocks, (const ForestArgs*)b->items_d,
-                            b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, t, b->stream);
+                            b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, k, b->t0_d, b->stream);
     else
         launch_life_batch(b->life_R, b->vb, clean, b->niv, blocks, (const LifeArgs*)b->items_d,
                           b->wave_item_d, b->wave_base_d, b->nwaves, b->parity, b->stream);
@@ -2183,6 +2202,10 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
     }
     if (b->kind == B_FOREST && !p)
         return fail(CAH_ERR_ARGUMENT, "the forest fire needs its parameters (p, f)");
+    if (b->kind == B_FOREST && !b->t0_d) {
+        if (b->dry) b->t0_d = (int64_t*)lchk::dry_alloc(8, "batch time base (dry)");
+        else HIPCHK(dmalloc((void**)&b->t0_d, 8, "batch time base"));
+    }
     // members may have been set through their own streams
     if (!b->dry)
         for (auto* s : b->members) HIPCHK(hipStreamSynchronize(s->stream));
@@ -2229,20 +2252,31 @@ extern "C" int cah_batch_advance(cah_batch* b, int64_t t0, int64_t nsteps, const
                                          b->parity, TB_K - n, b->stream);
                 k += n;
             }
-        for (; k < nsteps; ++k) launch_batch_step(b, t0 + k, bin);
+        for (; k < nsteps; ++k) launch_batch_step(b, k, bin);
     };
+    // the starting time, read by the forest kernels on the device (a kernel
+    // argument, so the value is taken when the launch is queued)
+    if (b->kind == B_FOREST && !b->dry) k_store_i64<<<1, 64, 0, b->stream>>>(b->t0_d, t0);
     if (use_graph && nsteps > 1 && !b->dry) {
-        if (b->exec) {
-            HIPCHK(hipStreamSynchronize(b->stream));
-            HIPCHK(hipGraphExecDestroy(b->exec));
-            b->exec = nullptr;
+        if (b->exec && b->g_nsteps == nsteps && b->g_parity0 == b->parity && b->g_clean == bin) {
+            b->parity = b->g_parity1;  // the captured launches, replayed
+        } else {
+            if (b->exec) {
+                HIPCHK(hipStreamSynchronize(b->stream));
+                HIPCHK(hipGraphExecDestroy(b->exec));
+                b->exec = nullptr;
+            }
+            b->g_nsteps = nsteps;
+            b->g_parity0 = b->parity;
+            b->g_clean = bin;
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            queue();
+            HIPCHK(hipStreamEndCapture(b->stream, &graph));
+            HIPCHK(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(graph));
+            b->g_parity1 = b->parity;
         }
-        hipGraph_t graph = nullptr;
-        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        queue();
-        HIPCHK(hipStreamEndCapture(b->stream, &graph));
-        HIPCHK(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(graph));
         HIPCHK(hipGraphLaunch(b->exec, b->stream));
     } else {
         queue();
@@ -2264,6 +2298,7 @@ static int free_layout(cah_batch* b) {
     if (!b->dry) HIPCHK(hipStreamSynchronize(b->stream));
     if (b->exec) HIPCHK(hipGraphExecDestroy(b->exec));
     b->exec = nullptr;
+    b->g_nsteps = -1;
     for (void** q : {&b->items_d, (void**)&b->wave_item_d, (void**)&b->wave_base_d}) {
         dfree(*q, b->dry);
         *q = nullptr;
@@ -2962,6 +2997,8 @@ static int build_tick(cah_server* s) {
                 a.afrag = st->frag_d;
                 a.L = lat_of(st);
                 a.H = st->lenia_H;
+                a.dy0 = st->lenia_dy0;
+                a.npairs = st->lenia_np;
                 a.cv1 = (float)st->d.boundary_value[0];
                 a.cv2 = (float)st->d.boundary_value[1];
                 a.prec32 = st->d.lenia_precision == 32;
@@ -3030,7 +3067,7 @@ static int build_tick(cah_server* s) {
                             LeniaMfmaArgs a;
                             std::memcpy(&a, item + k * sizeof a, sizeof a);
                             a = swap(a);
-                            lchk::lenia_mfma(c, a, g.key, a.nbx, (a.L.n2 + LM_BY - 1) / LM_BY, LM_KCHUNKS,
+                            lchk::lenia_mfma(c, a, g.key, a.nbx, (a.L.n2 + LM_BY - 1) / LM_BY,
                                              g.reqs[k]->st->frag_bytes);
                         }
                     }
@@ -3069,7 +3106,7 @@ static int build_tick(cah_server* s) {
                 // the tick's second step on, every item has taken a step of the rule
                 const bool tri = g.tri_edges && (g.tri_cells || k > 0);
                 launch_forest_batch(1, g.key % 2, tri, g.key / 2, waves_to_blocks(nw), (const ForestArgs*)items,
-                                    map, base, nw, k & 1, k, s->st[SC_FOREST]);
+                                    map, base, nw, k & 1, k, nullptr, s->st[SC_FOREST]);
             } else {
                 launch_lenia_mfma_batch(g.key, nw, (const LeniaMfmaArgs*)items, map, base, k & 1, k,
                                         s->st[SC_LENIA]);

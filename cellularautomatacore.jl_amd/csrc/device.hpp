@@ -156,6 +156,17 @@ struct Lat {
     int32_t two_d;
 };
 
+// The block's index in an XCD-aware order.  Workgroups are dealt to the eight
+// XCDs round robin (block b to XCD b % 8), and each XCD has its own L2.
+// Numbering the blocks so that XCD x runs one contiguous range of them puts
+// neighbouring tiles, which read each other's edges, on one L2.  A bijection of
+// 0 .. gridDim.x - 1 for any grid size.
+CAH_DEV int block_id_xcd() {
+    const int n = gridDim.x, b = blockIdx.x;
+    const int q = n >> 3, r = n & 7, xcd = b & 7;
+    return xcd * q + min(xcd, r) + (b >> 3);
+}
+
 // _boundary_read (src/backends.jl:177-200): the value at 0-based (g1, g2), which may
 // lie off the lattice.  When both axes are off a Constant edge, axis 2 wins.
 template <typename T>

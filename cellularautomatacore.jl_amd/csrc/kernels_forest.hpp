@@ -233,8 +233,10 @@ __global__ __launch_bounds__(256) void k_forest_gated(ForestArgs a) {
 // trajectories form one flat list: wave w belongs to item wave_item[w] and is
 // that item's wave (w - wave_base[item]).  Each item is the trajectory's own
 // argument block (buffers for parity 0, lattice, seed, band layout, p and f, and
-// its time base); the launch adds the step offset dt to the item's t and swaps
-// the buffers when parity is set.  A launch over the first nwaves waves advances
+// its time base); the launch adds the step offset dt and, when tbase is not
+// null, the time base *tbase read on the device (so that a captured sequence of
+// launches serves any starting time) to the item's t, and swaps the buffers when
+// parity is set.  A launch over the first nwaves waves advances
 // only the items those waves belong to, so items ordered by steps left can stop
 // at different steps of one sequence of launches.
 
@@ -242,7 +244,8 @@ template <int VB, bool SMALL, int PREC, bool TRI>
 __global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restrict__ items,
                                                       const int32_t* __restrict__ wave_item,
                                                       const int32_t* __restrict__ wave_base,
-                                                      int32_t nwaves, int32_t parity, int64_t dt) {
+                                                      int32_t nwaves, int32_t parity, int64_t dt,
+                                                      const int64_t* __restrict__ tbase) {
     const int wid = wave_id();
     if (wid >= nwaves) return;
     const int k = __builtin_amdgcn_readfirstlane(wave_item[wid]);
@@ -252,6 +255,6 @@ __global__ __launch_bounds__(256) void k_forest_batch(const ForestArgs* __restri
         a.cur = a.next;
         a.next = const_cast<int8_t*>(c);
     }
-    a.t += dt;
+    a.t += dt + (tbase ? *tbase : 0);
     forest_band<VB, SMALL, PREC, TRI>(a, wid - wave_base[k]);
 }

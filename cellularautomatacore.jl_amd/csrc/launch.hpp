@@ -28,7 +28,9 @@ void launch_forest(int vb, bool small, bool tri, int blocks, const ForestArgs& a
 // The same under a gate and/or a mask (k_forest_gated, 4 cells a lane).
 void launch_forest_gated(bool small, bool tri, int blocks, const ForestArgs& a, hipStream_t st);
 // The batch items carry p, f and their time base; prec = p_f32 | f_f32 << 1;
-// vb in {1, 2}; tri: every item's lattice and Constant edges hold only 0, 1 and 2.
+// vb in {1, 2}; tri: every item's lattice and Constant edges hold only 0, 1 and 2;
+// each item's step time is its t + dt (+ *tbase, read on the device, when tbase
+// is not null).
 void launch_forest_batch(int vb, bool small, bool tri, int prec, int blocks, const ForestArgs* items,
                          const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,
-                         int32_t parity, int64_t dt, hipStream_t st);
+                         int32_t parity, int64_t dt, const int64_t* tbase, hipStream_t st);

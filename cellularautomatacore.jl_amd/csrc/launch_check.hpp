@@ -25,6 +25,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 namespace lchk {
 
@@ -398,42 +399,61 @@ inline void tile_kernel(Chk& c, const Lat& L, const void* cur, int64_t es, int T
 
 // ---------------------------------------------------------------- k_lenia_mfma
 
-// One lattice of k_lenia_mfma<mode> over a gx x gy grid of 64 x 64 blocks: the
-// input tile's float4 chunks (or cells, when the lattice is too small for the
-// chunked fill), the A fragments, the LDS rows and columns the MFMAs read, and
-// the outputs.
-// gx x gy: the lattice's 64 x 64 tiles; kchunks: the K-chunks the launched kernel
-// runs with; plane: the bytes of one plane of the A fragments the state holds.
-inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int64_t gy, int kchunks,
-                       size_t plane) {
+// One lattice of k_lenia_mfma<mode> over a gx x gy grid of 64 x 64 blocks:
+// the row-offset pairs and the tile rows and columns the MFMAs read, the fill's
+// cover of the tile (every element the MFMAs read is written first; the growth's
+// cells are the fill's first four loads), its float4 chunks (or cells, when the
+// lattice is too small for the chunked fill), the A fragments and the outputs.
+// gx x gy: the lattice's 64 x 64 tiles; plane: the bytes of one plane of the A
+// fragments the state holds.
+inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int64_t gy, size_t plane) {
     const Lat& L = a.L;
     const int H = a.H;
     c.need(H >= 1 && H <= LM_HMAX, "H = %d", H);
     c.need(gx * LM_BX >= L.n1 && gy * LM_BY >= L.n2, "grid %lld x %lld does not cover %d x %d", (long long)gx,
            (long long)gy, L.n1, L.n2);
     c.need(L.pitch % 4 == 0, "pitch %lld is not a whole number of float4s", (long long)L.pitch);
-    if (!c.ok()) return;
-    const int rows = LM_BY + 2 * H;
-    c.need(rows <= LM_ROWS, "%d tile rows of %d", rows, LM_ROWS);
-    // the MFMAs read tile rows up to 63 + 2H and columns up to 16 w + 32 c + 8 g + 7 <= 111
-    const int nch = kchunks;
-    c.need(nch == 2, "%d K-chunks", nch);
     c.need(a.nbx == gx, "nbx %d for %lld tiles a row", a.nbx, (long long)gx);
-    const int cols = 112;
-    const int stride = mode == LM_F16X2 ? lm_stride<LM_F16X2>() : lm_stride<LM_F16>();
-    c.need(63 + 2 * H < rows && cols <= LM_COLS && LM_COLS <= stride, "MFMA reads beyond the tile");
+    if (!c.ok()) return;
+    const int rows = LM_BY + 2 * H + 1;
+    c.need(rows <= LM_ROWS, "%d tile rows of %d", rows, LM_ROWS);
+    // lane rows 16 w + n + H + dy0 + 2 p + (g >> 1) <= 63 + H + dy0 + 2 npairs - 1
+    c.need(a.npairs >= 1 && a.dy0 >= -H && a.dy0 + 2 * a.npairs - 1 <= H + 1,
+           "row-offset pairs from %d, %d of them, for H = %d", a.dy0, a.npairs, H);
+    c.need(63 + H + a.dy0 + 2 * a.npairs - 1 < rows, "MFMA rows beyond the tile");
+    c.need(16 * (LM_NC - 1) + 15 < LM_STRIDE, "MFMA columns beyond the LDS row");
     const int np = mode == LM_F16X2 ? 2 : 1;
-    c.need(plane == (size_t)(2 * H + 1) * nch * 64 * 16, "fragment plane of %zu bytes for H = %d", plane, H);
+    lds(c, sizeof(LmTile<LM_F16X2>) / 2 * np);
+    c.need(LM_BY * LM_UST * 4 <= (int)(sizeof(LmTile<LM_F16>)), "potentials past the tile buffer");
+    c.need(plane == lenia_mfma_fragment_bytes(a.npairs), "fragment plane of %zu bytes for %d pairs", plane,
+           a.npairs);
     c.range(a.afrag, 0, (int64_t)np * (int64_t)plane, "A fragments");
-    constexpr int NCH = LM_COLS / 4 + 1;
-    const bool chunked = L.n1 % 4 == 0 && L.n1 >= 4 * NCH && L.n2 >= rows;
+    if (!c.ok()) return;
+    const bool chunked = lenia_chunked(L, H);
+    if (chunked) {
+        // the fill's loads cover the tile once, and loads 0..3 are the growth's cells
+        std::vector<uint8_t> hit((size_t)rows * LM_NCH, 0);
+        for (int tid = 0; tid < LM_NT && c.ok(); ++tid)
+            for (int q = 0; q < LM_FILL_Q; ++q) {
+                int ly, j;
+                if (!lenia_fill_at(H, tid, q, ly, j)) continue;
+                c.need(ly >= 0 && ly < rows && j >= 0 && j < LM_NCH, "fill load (%d, %d) off the tile", ly, j);
+                if (!c.ok()) break;
+                c.need(hit[(size_t)ly * LM_NCH + j]++ == 0, "tile chunk (%d, %d) loaded twice", ly, j);
+                if (q < LM_CELLS_Q) {
+                    const int i = tid + LM_NT * q;
+                    c.need(ly == H + i / (LM_BX / 4) && 4 * j == 16 * LM_HC + 4 * (i % (LM_BX / 4)),
+                           "fill load %d of thread %d is not its growth's cells", q, tid);
+                }
+            }
+        for (size_t k = 0; k < hit.size() && c.ok(); ++k)
+            c.need(hit[k] == 1, "tile chunk (%zu, %zu) never loaded", k / LM_NCH, k % LM_NCH);
+    }
     int64_t clo = INT64_MAX, chi = INT64_MIN;
     for (int64_t bx = 0; bx < gx && c.ok(); ++bx) {
-        const int x0 = (int)bx * LM_BX, xs = x0 - H, sh = xs & 3, xa = xs - sh;
+        const int xa = (int)bx * LM_BX - 16 * LM_HC;
         if (chunked) {
-            const int nload = (LM_BX + 2 * H + sh + 3) / 4;
-            c.need(nload <= NCH, "%d chunks of %d", nload, NCH);
-            for (int j = 0; j < nload; ++j) {
+            for (int j = 0; j < LM_NCH; ++j) {
                 const int x = xa + 4 * j;
                 int xx;
                 if (x >= 0 && x + 4 <= L.n1) xx = x;
@@ -446,7 +466,7 @@ inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int
             }
         } else {
             int32_t p, q;
-            c.need(axis_span(L.b1, xs, xs + LM_COLS - 1, L.n1, p, q), "tile columns from %d map off", xs);
+            c.need(axis_span(L.b1, xa, xa + LM_COLS - 1, L.n1, p, q), "tile columns from %d map off", xa);
             if (p <= q) clo = p < clo ? p : clo, chi = q + 1 > chi ? q + 1 : chi;
         }
     }

@@ -23,44 +23,65 @@ void launch_lenia_mfma_batch(int mode, int nblocks, const LeniaMfmaArgs* items,
         k_lenia_mfma_batch<LM_F16><<<nblocks, LM_NT, 0, st>>>(items, block_item, block_base, parity, dt);
 }
 
-// Fragment f = (dyi * 2 + c) * 64 + lane holds A_dy[m][k] for m = lane % 16 and
-// k = 32 c + 8 (lane / 16) + i, i = 0..7 (the operand order of
-// v_mfma_f32_16x16x32_*; B uses the same k order, so the sum is the same under
-// any fixed k permutation).  A_dy[m][k] = W(k - H - m, dy), zero off the band.
-template <typename E>
-static void fill_fragments(int H, const double* wd, std::vector<E>& hi, std::vector<E>& lo) {
+// The row offsets the MFMA loop visits: from the first row holding a nonzero
+// weight, in pairs, up to the last such row (the last pair may take one row
+// more, at most H + 1, whose weights are zero).
+void lenia_mfma_pairs(int H, const double* wd, int32_t& dy0, int32_t& npairs) {
     const int K = 2 * H + 1;
-    const size_t nfrag = (size_t)K * LM_KCHUNKS * 64;
+    int lo = K, hi = -1;
+    for (int r = 0; r < K; ++r)
+        for (int c = 0; c < K; ++c)
+            if (wd[(size_t)r * K + c] != 0.0) {
+                lo = r < lo ? r : lo;
+                hi = r;
+            }
+    if (hi < 0) {  // no weight: one pair of zero fragments
+        dy0 = 0;
+        npairs = 1;
+        return;
+    }
+    dy0 = lo - H;
+    npairs = (hi - lo + 2) / 2;
+}
+
+// Fragment (p, r) lane l holds A[m][k] for m = l % 16 and k = 8 g + i, g = l / 16,
+// i = 0..7 (the operand order of v_mfma_f32_16x16x32_*; the B operand takes the
+// same k order, so the sum is the same under any fixed k permutation):
+// A[m][k] = W(16 (r - HC) + 8 (g & 1) + i - m, dy0 + 2 p + (g >> 1)), zero off
+// the (2H + 1)^2 grid.
+template <typename E>
+static void fill_fragments(int H, const double* wd, int32_t dy0, int32_t npairs, std::vector<E>& hi,
+                           std::vector<E>& lo) {
+    const int K = 2 * H + 1, HC = LM_HC, NR = LM_NR;
+    const size_t nfrag = (size_t)npairs * NR * 64;
     hi.assign(nfrag * 8, (E)0.0f);
     lo.assign(nfrag * 8, (E)0.0f);
-    for (int dyi = 0; dyi < K; ++dyi)
-        for (int c = 0; c < LM_KCHUNKS; ++c)
+    for (int p = 0; p < npairs; ++p)
+        for (int r = 0; r < NR; ++r)
             for (int lane = 0; lane < 64; ++lane)
                 for (int i = 0; i < 8; ++i) {
-                    const int m = lane % 16, k = 32 * c + 8 * (lane / 16) + i;
-                    const int dx = k - H - m;
+                    const int m = lane % 16, g = lane / 16;
+                    const int dx = 16 * (r - HC) + 8 * (g & 1) + i - m, dy = dy0 + 2 * p + (g >> 1);
                     double w = 0.0;
-                    if (dx >= -H && dx <= H) w = wd[(size_t)dyi * K + (dx + H)];
-                    const size_t at = (((size_t)dyi * LM_KCHUNKS + c) * 64 + lane) * 8 + i;
+                    if (dx >= -H && dx <= H && dy >= -H && dy <= H) w = wd[(size_t)(dy + H) * K + (dx + H)];
+                    const size_t at = (((size_t)p * NR + r) * 64 + lane) * 8 + i;
                     const E h = (E)(float)w;
                     hi[at] = h;
                     lo[at] = (E)(float)(w - (double)(float)h);
                 }
 }
 
-size_t lenia_mfma_fragment_bytes(int H) { return (size_t)(2 * H + 1) * LM_KCHUNKS * 64 * 8 * 2; }
-
-void lenia_mfma_fragments(int mode, int H, const double* wd, std::vector<char>& buf) {
-    const size_t plane = lenia_mfma_fragment_bytes(H);
+void lenia_mfma_fragments(int mode, int H, const double* wd, int32_t dy0, int32_t npairs, std::vector<char>& buf) {
+    const size_t plane = lenia_mfma_fragment_bytes(npairs);
     buf.assign(plane * 2, 0);
     if (mode == LM_BF16) {
         std::vector<__bf16> hi, lo;
-        fill_fragments(H, wd, hi, lo);
+        fill_fragments(H, wd, dy0, npairs, hi, lo);
         std::memcpy(buf.data(), hi.data(), plane);
         std::memcpy(buf.data() + plane, lo.data(), plane);
     } else {
         std::vector<_Float16> hi, lo;
-        fill_fragments(H, wd, hi, lo);
+        fill_fragments(H, wd, dy0, npairs, hi, lo);
         std::memcpy(buf.data(), hi.data(), plane);
         std::memcpy(buf.data() + plane, lo.data(), plane);
     }

@@ -16,12 +16,12 @@ void launch_lenia_mfma(int mode, int32_t ntiles, const LeniaMfmaArgs& a, hipStre
 void launch_lenia_mfma_batch(int mode, int nblocks, const LeniaMfmaArgs* items,
                              const int32_t* block_item, const int32_t* block_base,
                              int32_t parity, int64_t dt, hipStream_t st);
-// The A fragments of the implicit GEMM for a dense (2H+1)^2 FP64 weight grid
-// wd[(dy + H) * (2H + 1) + (dx + H)], as the bytes of the device buffer (two
-// planes: the values, then their residuals for split FP16).
-void lenia_mfma_fragments(int mode, int H, const double* wd, std::vector<char>& out);
-// Bytes of one plane of those fragments.
-size_t lenia_mfma_fragment_bytes(int H);
+// The first row offset and the row-offset pairs the MFMA loop visits for a dense
+// (2H+1)^2 FP64 weight grid wd[(dy + H) * (2H + 1) + (dx + H)].
+void lenia_mfma_pairs(int H, const double* wd, int32_t& dy0, int32_t& npairs);
+// The A fragments of the implicit GEMM for that grid, as the bytes of the device
+// buffer (two planes: the values, then their residuals for split FP16).
+void lenia_mfma_fragments(int mode, int H, const double* wd, int32_t dy0, int32_t npairs, std::vector<char>& out);
 
 // FP32 FFT field on a periodic lattice (the FFTW extension's cross-correlation).
 struct LeniaFFT;

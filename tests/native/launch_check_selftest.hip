@@ -109,37 +109,48 @@ int main() {
         lchk::tile_kernel(e, L, cur, 1, 64, 16, 8, 16, 19, 4096);
         expect("tile kernel, right", true, e);
     }
-    // k_lenia_mfma: 2048^2 FP32, H = 12: fragments of the right size, then one short
+    // k_lenia_mfma: 2048^2 FP32, H = 12, rows -11 .. 11 (12 pairs): fragments of
+    // the right size, then planted mistakes
     {
         const Lat F = lat(2048, 2048, 2048, P, P);
         const float* fc = (const float*)lchk::dry_alloc((size_t)2048 * 2048 * 4, "lenia cur");
         float* fn = (float*)lchk::dry_alloc((size_t)2048 * 2048 * 4, "lenia next");
-        const size_t fb = (size_t)25 * 2 * 64 * 16;
+        const size_t fb = lenia_mfma_fragment_bytes(12);
         LeniaMfmaArgs a{};
         a.cur = fc;
         a.next = fn;
         a.L = F;
         a.H = 12;
+        a.dy0 = -11;
+        a.npairs = 12;
         a.nbx = 32;
         a.afrag = lchk::dry_alloc(fb, "fragments");
         lchk::Chk c("k_lenia_mfma, right");
-        lchk::lenia_mfma(c, a, LM_F16, 32, 32, 2, fb);
+        lchk::lenia_mfma(c, a, LM_F16, 32, 32, fb);
         expect("k_lenia_mfma, right", true, c);
         lchk::Chk d("k_lenia_mfma, split fragments missing their low half");
-        lchk::lenia_mfma(d, a, LM_F16X2, 32, 32, 2, fb);
+        lchk::lenia_mfma(d, a, LM_F16X2, 32, 32, fb);
         expect(d.kernel.c_str(), false, d);
         lchk::Chk f("k_lenia_mfma, a tile count a row that is not the lattice's");
         a.nbx = 31;
-        lchk::lenia_mfma(f, a, LM_F16, 32, 32, 2, fb);
+        lchk::lenia_mfma(f, a, LM_F16, 32, 32, fb);
         expect(f.kernel.c_str(), false, f);
         a.nbx = 32;
-        lchk::Chk g("k_lenia_mfma, fragments for another H (H = 8)");
+        lchk::Chk h("k_lenia_mfma, fragments for fewer pairs");
+        lchk::lenia_mfma(h, a, LM_F16, 32, 32, lenia_mfma_fragment_bytes(11));
+        expect(h.kernel.c_str(), false, h);
+        lchk::Chk p("k_lenia_mfma, one row-offset pair too many (rows past the tile)");
+        a.npairs = 13;
+        lchk::lenia_mfma(p, a, LM_F16, 32, 32, lenia_mfma_fragment_bytes(13));
+        expect(p.kernel.c_str(), false, p);
+        a.npairs = 12;
+        lchk::Chk g("k_lenia_mfma, pairs past the rows of a smaller H (H = 8)");
         a.H = 8;
-        lchk::lenia_mfma(g, a, LM_F16, 32, 32, 2, fb);
+        lchk::lenia_mfma(g, a, LM_F16, 32, 32, fb);
         expect(g.kernel.c_str(), false, g);
-        a.H = 25;
+        a.H = 17;
         lchk::Chk e("k_lenia_mfma, H past the tile");
-        lchk::lenia_mfma(e, a, LM_F16, 32, 32, 2, fb);
+        lchk::lenia_mfma(e, a, LM_F16, 32, 32, fb);
         expect(e.kernel.c_str(), false, e);
     }
     // a batch table whose entries skip an item

@@ -95,7 +95,7 @@ def test_fft_needs_periodic_edges():
 @pytest.mark.parametrize("edges", [(M.Reflecting(), M.Constant(0.25)), (M.Constant(0.5), M.Reflecting()),
                                    (M.Periodic(), M.Constant(0.75))])
 def test_mfma_chunked_edges(oracle, edges):
-    # wide enough (first side >= 116, a multiple of 4) for the tile fill that maps
+    # wide enough (first side >= 96, a multiple of 4) for the tile fill that maps
     # whole float4 chunks at the edges; the state reaches every edge
     dims = (160, 136)
     u = np.asfortranarray(np.random.default_rng(11).random(dims).astype(np.float32))
@@ -118,3 +118,24 @@ def test_mfma_with_edges_and_gates(oracle):
         for t in range(3):
             ref = oracle.advance_plain(d, sol.u[t], t, 1)
             assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("R", [2, 5, 12, 13, 17])
+@pytest.mark.parametrize("dims", [(192, 176), (100, 72)])
+def test_mfma_kernel_radii(oracle, R, dims):
+    """Kernel radii up to the largest the MFMA forms take (halo H = R - 1 <= 16),
+    odd and even numbers of nonzero kernel rows (R = 12: 23, R = 13: 25), and
+    both tile fills (chunked: first side a multiple of 4 and at least the tile's
+    96 columns, 65 + 2H rows; cell by cell otherwise), against the FP64 oracle
+    per step."""
+    u = patch_state(dims, (20, 12), (60, 52), 13)
+    space = M.Lattice(dims, (M.Periodic(), M.Reflecting()))
+    prob = CAProblem(M.Lenia(R=R), u, 2, space=space, seed=1)
+    for field, tol in (("mfma_f16x2", 1e-5), ("mfma_f16", 1e-3)):
+        assert "k_lenia_mfma" in HIPBackend(lenia_field=field).check_launches(prob, M.Synchronous())
+        sol = solve(prob, M.Synchronous(), backend=HIPBackend(lenia_field=field), saveat=1)
+        d, keep = desc_for(prob, M.Synchronous(), field_method=0)
+        for t in range(2):
+            ref = oracle.advance_plain(d, sol.u[t], t, 1)
+            err = np.abs(sol.u[t + 1].astype(np.float64) - ref).max()
+            assert err < tol, (field, t, err)

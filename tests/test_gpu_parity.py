@@ -468,3 +468,21 @@ def test_batch_equals_single_runs(oracle, backend):
         # trajectory i + 1 runs with seed prob.seed + i + 1 (seed_trajectory)
         assert np.array_equal(a, oracle_final(oracle, base.remake(seed=i + 1)))
     assert len({a.tobytes() for a in batched}) == 5
+
+
+@pytest.mark.parametrize("saveat", [10, [0, 5, 15, 20, 30, 35, 45, 50]])
+def test_batch_graph_replays_at_every_start(oracle, saveat):
+    """A batch advanced between saved times replays one captured graph per step
+    count (the starting time is read on the device): every saved state of every
+    trajectory equals the oracle's state at that time, with equal step counts
+    (saveat = 10: five replays of one graph) and with step counts and buffer
+    parities that alternate between calls."""
+    probs = [forest_problem((96 + 8 * i, 70), p=0.05, f=0.002, seed=40 + i, u0=np.zeros((96 + 8 * i, 70), np.int8),
+                            tspan=50) for i in range(3)]
+    series = HIPBackend().solve_batch(probs, M.Synchronous(), saveat=saveat)
+    times = list(range(0, 51, 10)) if saveat == 10 else saveat
+    for pr, states in zip(probs, series):
+        assert len(states) == len(times)
+        for t, got in zip(times, states):
+            want = oracle_final(oracle, pr.remake(tspan=(0, t))) if t > 0 else pr.u0
+            assert np.array_equal(got, want), (pr.seed, t)
