@@ -770,7 +770,7 @@ def bench_serving_ensembles():
 
 
 PROFILE_WORKLOADS = ("ltl", "forest", "lenia_mfma_f16", "lenia_mfma_f16x2", "lenia_mfma_bf16", "lenia_fft_f32",
-                     "brians_brain", "forest_stochastic", "copy")
+                     "brians_brain", "forest_stochastic", "copy", "serving_capacity", "plain_capacity")
 
 
 def profile_workload(name, steps):
@@ -779,10 +779,19 @@ def profile_workload(name, steps):
     (forest), Lenia R=13 2048^2 with a field method (lenia_*), Brian's Brain
     Moore(1) Torus 8192^2 (brians_brain), a Stochastic(0.5) forest fire
     16384^2 (forest_stochastic), or the calibration copy (copy); prints the mean
-    ms per step from HIP events."""
+    ms per step from HIP events.  serving_capacity / plain_capacity: the serving
+    configuration's 400 requests, all present at t = 0, through the server or the
+    plain per-request sweep; prints sims/s."""
     from cahip import model as M
     from cahip.backend import HIPBackend, check, make_params
     from cahip.problem import CAProblem
+    if name in ("serving_capacity", "plain_capacity"):
+        # the serving configuration's 400 requests, all present at t = 0
+        probs = serving_problems(400, seed=5, lo=128, hi=4096, tmax=500, ltl=False)
+        fin, _ = (serve if name == "serving_capacity" else plain)(probs, np.zeros(len(probs)))
+        print(json.dumps({"workload": name, "requests": len(probs), "sims_per_s": round(len(probs) / fin.max(), 1)}),
+              flush=True)
+        return
     ev = HipEvents()
     if name in ("ltl", "forest", "copy"):
         be = HIPBackend(0)

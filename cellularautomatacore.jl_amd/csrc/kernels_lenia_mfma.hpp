@@ -95,17 +95,24 @@ using LmTile = typename LmT<MODE>::E[MODE == LM_F16X2 ? 2 : 1][LM_ROWS * LM_STRI
 //
 // Tile row ly, column lx hold the state at (x0 - 16 HC + lx, y0 - H + ly) in the
 // MFMA input type (and, for split FP16, the residual in the second plane), rows
-// 0 .. 64 + 2H.  The tile is read as aligned float4 chunks (rows are 16-float
-// aligned and x0 - 16 is a multiple of 16).  With a first side that is a
-// multiple of 4 and a lattice at least as large as the tile, every chunk lies
-// wholly inside the lattice or wholly beyond one edge, where it maps onto an
-// aligned chunk inside (wrapped for Periodic, mirrored and read back to front for
-// Reflecting) or onto the Constant value, as _map_axis does cell by cell; rows
-// map the same way.  Smaller lattices go through the boundary map cell by cell
-// (lenia_fill_direct).
+// 0 .. 64 + 2H.  The tile is read as float4 chunks (rows are 16-float aligned
+// and x0 - 16 is a multiple of 16), each loaded whole where it lies inside the
+// lattice.  A chunk beyond an edge maps, as _map_axis does cell by cell:
+//  - when the first side is a multiple of 4 and at least the tile's width
+//    (lenia_cols_whole), onto an aligned chunk inside, wrapped for Periodic,
+//    mirrored and read back to front for Reflecting, or onto the Constant value;
+//  - otherwise cell by cell (four loads; a chunk may also straddle the edge).
+// Rows map by one formula for every edge kind when the second side is at least
+// the tile's height (lenia_rows_near: no row lies more than one period beyond
+// an edge), else through map_axis0.
 
-CAH_HD bool lenia_chunked(const Lat& L, int H) {
-    return L.n1 % 4 == 0 && L.n1 >= LM_COLS && L.n2 >= LM_BY + 2 * H + 1;
+CAH_HD bool lenia_cols_whole(const Lat& L) { return L.n1 % 4 == 0 && L.n1 >= LM_COLS; }
+
+CAH_HD bool lenia_rows_near(const Lat& L, int H) { return L.n2 >= LM_BY + 2 * H + 1; }
+
+CAH_HD bool lenia_tile_wide(const Lat& L, int H, int x0) {
+    const int xa = x0 - 16 * LM_HC;
+    return lenia_rows_near(L, H) && (lenia_cols_whole(L) || (xa >= 0 && xa + LM_COLS <= L.n1));
 }
 
 // Load q of a thread: q < 4 are the 64 x 64 centre (row H + i / 16, chunk
@@ -150,29 +157,84 @@ CAH_HD LmMap lm_axis(int32_t kind, int32_t n, int32_t w) {
     return {0, 0, 0, 0};
 }
 
-// The chunks this thread stages, loaded into registers: every load is issued
-// before any is used, from an address mapped without branches (a chunk lies at
-// most one period beyond an edge), with what the chunk still needs in fix, two
-// bits a load: 1 read back to front (a mirrored column chunk), 2 the first
-// axis's Constant value, 3 the second's (axis 2 wins, as in _boundary_read).
-CAH_DEV void lenia_fill_load(const LeniaMfmaArgs& a, int x0, int y0, float4 (&v)[LM_FILL_Q], uint32_t& fix) {
-    const int xa = x0 - 16 * LM_HC, n1 = a.L.n1, n2 = a.L.n2;
-    fix = 0;
-    const LmMap my_ = lm_axis(a.L.b2, n2, 1), mx_ = lm_axis(a.L.b1, n1, 4);
-    const bool c2 = a.L.b2 == CAH_BOUNDARY_CONSTANT, c1 = a.L.b1 == CAH_BOUNDARY_CONSTANT;
-    const uint32_t fx = c1 ? 2 : (a.L.b1 == CAH_BOUNDARY_REFLECTING ? 1 : 0);  // a column chunk off the lattice
+// The row of tile row ly (lattice row y0 - H + ly) as an element offset, and
+// whether it lies beyond a Constant edge.
+CAH_DEV int64_t lenia_fill_row(const LeniaMfmaArgs& a, int y, bool near, const LmMap& my_, bool& oy) {
+    int my;
+    if (near) {
+        const bool ylo = y < 0, yhi = y >= a.L.n2;
+        my = ylo ? my_.alo + my_.slo * y : (yhi ? my_.ahi + my_.shi * y : y);
+        oy = a.L.b2 == CAH_BOUNDARY_CONSTANT && (ylo || yhi);
+    } else {
+        my = map_axis0(a.L.b2, y, a.L.n2, oy);
+    }
+    return oy ? 0 : (int64_t)my * a.L.pitch;
+}
+
+// The chunks this thread stages, loaded into registers with one float4 load
+// each, all issued before any is used, from addresses mapped without branches.
+// What a chunk still needs goes in fix, two bits a load: 1 read back to front
+// (a whole chunk mirrored), 2 the first axis's Constant value, 3 the second's
+// (axis 2 wins, as in _boundary_read); and in by_cell, one bit a load, a chunk
+// off the lattice that is not whole (lenia_fill_cells maps it cell by cell).
+// WIDE: rows near (lenia_rows_near) and every column chunk of the tile either
+// inside the lattice or whole (lenia_cols_whole): the tiles of every lattice of
+// 96 x (65 + 2H) cells or more whose first side is a multiple of 4, and the
+// tiles clear of the first axis's edges on any lattice (the test is made once
+// per block, lenia_tile_wide).
+template <bool WIDE>
+CAH_DEV void lenia_fill_load(const LeniaMfmaArgs& a, int x0, int y0, float4 (&v)[LM_FILL_Q], uint32_t& fix,
+                             uint32_t& by_cell) {
+    const int xa = x0 - 16 * LM_HC, n1 = a.L.n1;
+    const bool whole = WIDE || lenia_cols_whole(a.L), near = WIDE || lenia_rows_near(a.L, a.H);
+    const LmMap my_ = lm_axis(a.L.b2, a.L.n2, 1), mx_ = lm_axis(a.L.b1, n1, 4);
+    const bool c1 = a.L.b1 == CAH_BOUNDARY_CONSTANT;
+    const uint32_t fx = c1 ? 2 : (a.L.b1 == CAH_BOUNDARY_REFLECTING ? 1 : 0);  // a whole chunk off the lattice
+    fix = by_cell = 0;
 #pragma unroll
     for (int q = 0; q < LM_FILL_Q; ++q) {
         int ly, j;
         const bool in = lenia_fill_at(a.H, (int)threadIdx.x, q, ly, j);
-        const int y = y0 - a.H + ly, x = xa + 4 * j;
-        const bool ylo = y < 0, yhi = y >= n2, xlo = x < 0, xhi = x >= n1;
-        const int my = ylo ? my_.alo + my_.slo * y : (yhi ? my_.ahi + my_.shi * y : y);
-        const int mx = xlo ? mx_.alo + mx_.slo * x : (xhi ? mx_.ahi + mx_.shi * x : x);
-        const uint32_t f = (c2 && (ylo || yhi)) ? 3u : ((xlo || xhi) ? fx : 0u);
+        bool oy;
+        const int64_t row = lenia_fill_row(a, y0 - a.H + ly, near, my_, oy);
+        const int x = xa + 4 * j;
+        const bool inr = x >= 0 && x + 4 <= n1;
+        int xm = inr ? x : 0;
+        uint32_t f = oy ? 3u : 0u;
+        if (whole) {
+            if (!inr && !c1) xm = x < 0 ? mx_.alo + mx_.slo * x : mx_.ahi + mx_.shi * x;
+            if (!inr && !oy) f = fx;
+        } else if (in && !inr && !oy) {
+            by_cell |= 1u << q;
+        }
+        v[q] = *reinterpret_cast<const float4*>(a.cur + (in && !oy ? row + xm : 0));
         fix |= f << (2 * q);
-        if (in) v[q] = *reinterpret_cast<const float4*>(a.cur + (int64_t)my * a.L.pitch + mx);
-        else v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// The chunks of by_cell (lenia_fill_load) through the boundary map cell by
+// cell: a lattice whose first side is not a multiple of 4 or is narrower than
+// the tile.
+CAH_DEV void lenia_fill_cells(const LeniaMfmaArgs& a, int x0, int y0, float4 (&v)[LM_FILL_Q], uint32_t by_cell) {
+    if (!by_cell) return;
+    const bool near = lenia_rows_near(a.L, a.H);
+    const LmMap my_ = lm_axis(a.L.b2, a.L.n2, 1);
+#pragma unroll
+    for (int q = 0; q < LM_FILL_Q; ++q) {
+        if (!((by_cell >> q) & 1)) continue;
+        int ly, j;
+        lenia_fill_at(a.H, (int)threadIdx.x, q, ly, j);
+        bool oy;
+        const float* row = a.cur + lenia_fill_row(a, y0 - a.H + ly, near, my_, oy);
+        const int x = x0 - 16 * LM_HC + 4 * j;
+        float e[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bool o1;
+            const int m = map_axis0(a.L.b1, x + k, a.L.n1, o1);
+            e[k] = o1 ? a.cv1 : row[m];
+        }
+        v[q] = make_float4(e[0], e[1], e[2], e[3]);
     }
 }
 
@@ -210,28 +272,13 @@ CAH_DEV void lenia_fill_store(const LeniaMfmaArgs& a, const float4 (&v)[LM_FILL_
     }
 }
 
-// The tile cell by cell through the boundary map (lattices too small for chunks).
-template <int MODE>
-CAH_DEV void lenia_fill_direct(const LeniaMfmaArgs& a, int x0, int y0, LmTile<MODE>& tile) {
-    using E = typename LmT<MODE>::E;
-    const int rows = LM_BY + 2 * a.H + 1;
-    for (int i = threadIdx.x; i < rows * LM_COLS; i += LM_NT) {
-        const int lx = i % LM_COLS, ly = i / LM_COLS;
-        const float v = read_mapped<float>(a.cur, a.L, a.cv1, a.cv2, x0 - 16 * LM_HC + lx, y0 - a.H + ly);
-        const E hi = (E)v;
-        tile[0][ly * LM_STRIDE + lx] = hi;
-        if constexpr (MODE == LM_F16X2) tile[1][ly * LM_STRIDE + lx] = (E)(v - (float)hi);
-    }
-}
-
 // ---------------------------------------------------------------- the growth
 
 // The growth of the 64 x 64 block at (x0, y0) from its potentials U (row stride
-// LM_UST floats, in LDS), in FP32 (lenia_fast32): mu, -1 / (2 sigma^2), dt.  A
-// block wholly inside the lattice takes its cells from c (float4 q of the
-// thread: row q' / 16, columns 4 (q' % 16) .. + 3 for q' = tid + 256 q), loaded
-// with the tile; others read them here.  The result is written in whole
-// 256-byte rows.
+// LM_UST floats, in LDS), in FP32 (lenia_fast32): mu, -1 / (2 sigma^2), dt.
+// The cells come from the registers the fill loaded them into (the centre is
+// its first four loads), so the state is read once a step.  A block wholly
+// inside the lattice writes whole 256-byte rows.
 constexpr int LM_UST = LM_BX + 4;
 constexpr int LM_CELLS_Q = LM_BX * LM_BY / 4 / LM_NT;  // float4s of state a thread
 
@@ -279,34 +326,38 @@ CAH_DEV void lenia_growth_full(const LeniaMfmaArgs& a, int x0, int y0, const flo
     }
 }
 
+// c: the block's cells from the fill (float4 k of the thread: row q / 16,
+// columns 4 (q % 16) .. + 3 for q = tid + 256 k), right wherever they lie on
+// the lattice; a block past an edge writes only those.
 CAH_DEV void lenia_growth_block(const LeniaMfmaArgs& a, int x0, int y0, const float* U,
-                                const float4 (&c)[LM_CELLS_Q], bool have_cells) {
+                                const float4 (&c)[LM_CELLS_Q]) {
     if (lenia_full_block(a.L, x0, y0)) {
-        if (have_cells) {
-            lenia_growth_full(a, x0, y0, U, c);
-        } else {
-            float4 cl[LM_CELLS_Q];
-#pragma unroll
-            for (int k = 0; k < LM_CELLS_Q; ++k) {
-                const int q = threadIdx.x + LM_NT * k;
-                cl[k] = *reinterpret_cast<const float4*>(a.cur + (int64_t)(y0 + q / (LM_BX / 4)) * a.L.pitch + x0 +
-                                                         4 * (q % (LM_BX / 4)));
-            }
-            lenia_growth_full(a, x0, y0, U, cl);
-        }
+        lenia_growth_full(a, x0, y0, U, c);
         return;
     }
     const float mu = (float)a.mu, nk = (float)(-1.0 / (2.0 * a.sigma * a.sigma)), dt = (float)a.dt;
-    for (int q = threadIdx.x; q < LM_BY * LM_BX / 4; q += LM_NT) {
+    const bool every = a.mask == nullptr && a.scheme == CAH_SCHEME_SYNCHRONOUS;
+#pragma unroll
+    for (int k = 0; k < LM_CELLS_Q; ++k) {
+        const int q = threadIdx.x + LM_NT * k;
         const int ly = q / (LM_BX / 4), lx = 4 * (q % (LM_BX / 4));
         const int y = y0 + ly, x = x0 + lx;
         if (y >= a.L.n2) continue;
         const float4 u4 = *reinterpret_cast<const float4*>(&U[ly * LM_UST + lx]);
-        const float uu[4] = {u4.x, u4.y, u4.z, u4.w};
+        const float uu[4] = {u4.x, u4.y, u4.z, u4.w}, cc[4] = {c[k].x, c[k].y, c[k].z, c[k].w};
         const int64_t at = (int64_t)y * a.L.pitch + x;
+        float r[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-            if (x + e < a.L.n1) a.next[at + e] = lenia_grow(a, x + e, y, a.cur[at + e], uu[e], mu, nk, dt);
+            r[e] = x + e >= a.L.n1 ? cc[e]  // off the lattice: not written (and no mask byte read)
+                   : every ? lenia_fast32(cc[e], uu[e], mu, nk, dt) : lenia_grow(a, x + e, y, cc[e], uu[e], mu, nk, dt);
+        if (x + 4 <= a.L.n1) {
+            *reinterpret_cast<float4*>(a.next + at) = make_float4(r[0], r[1], r[2], r[3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (x + e < a.L.n1) a.next[at + e] = r[e];
+        }
     }
 }
 
@@ -378,17 +429,20 @@ CAH_DEV void lenia_mfma_tile(const LeniaMfmaArgs& a, int k) {
     __shared__ __attribute__((aligned(16))) LmTile<MODE> tile;
     const int x0 = (k % a.nbx) * LM_BX, y0 = (k / a.nbx) * LM_BY;
     float4 cells[LM_CELLS_Q];
-    const bool chunked = lenia_chunked(a.L, a.H);
-    if (chunked) {
+    {
         float4 v[LM_FILL_Q];
-        uint32_t fix;
-        lenia_fill_load(a, x0, y0, v, fix);
-        lenia_fill_fix(a, v, fix);
+        uint32_t fix, by_cell;
+        if (lenia_tile_wide(a.L, a.H, x0)) {
+            lenia_fill_load<true>(a, x0, y0, v, fix, by_cell);
+            lenia_fill_fix(a, v, fix);
+        } else {
+            lenia_fill_load<false>(a, x0, y0, v, fix, by_cell);
+            lenia_fill_fix(a, v, fix);
+            lenia_fill_cells(a, x0, y0, v, by_cell);
+        }
 #pragma unroll
         for (int q = 0; q < LM_CELLS_Q; ++q) cells[q] = v[q];
         lenia_fill_store<MODE>(a, v, tile);
-    } else {
-        lenia_fill_direct<MODE>(a, x0, y0, tile);
     }
     __syncthreads();
     lm_f4 acc[4];
@@ -397,7 +451,7 @@ CAH_DEV void lenia_mfma_tile(const LeniaMfmaArgs& a, int k) {
     float* U = reinterpret_cast<float*>(&tile[0][0]);
     lenia_store_potentials(acc, U);
     __syncthreads();
-    lenia_growth_block(a, x0, y0, U, cells, chunked);
+    lenia_growth_block(a, x0, y0, U, cells);
 }
 
 // One lattice: block b computes tile block_id_xcd(b), so that each XCD's L2

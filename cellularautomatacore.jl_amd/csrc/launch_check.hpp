@@ -402,8 +402,9 @@ inline void tile_kernel(Chk& c, const Lat& L, const void* cur, int64_t es, int T
 // One lattice of k_lenia_mfma<mode> over a gx x gy grid of 64 x 64 blocks:
 // the row-offset pairs and the tile rows and columns the MFMAs read, the fill's
 // cover of the tile (every element the MFMAs read is written first; the growth's
-// cells are the fill's first four loads), its float4 chunks (or cells, when the
-// lattice is too small for the chunked fill), the A fragments and the outputs.
+// cells are the fill's first four loads), its float4 chunks (whole, whole and
+// mapped, or mapped cell by cell at an edge) and rows, the A fragments and the
+// outputs.
 // gx x gy: the lattice's 64 x 64 tiles; plane: the bytes of one plane of the A
 // fragments the state holds.
 inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int64_t gy, size_t plane) {
@@ -429,58 +430,78 @@ inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int
            a.npairs);
     c.range(a.afrag, 0, (int64_t)np * (int64_t)plane, "A fragments");
     if (!c.ok()) return;
-    const bool chunked = lenia_chunked(L, H);
-    if (chunked) {
-        // the fill's loads cover the tile once, and loads 0..3 are the growth's cells
-        std::vector<uint8_t> hit((size_t)rows * LM_NCH, 0);
-        for (int tid = 0; tid < LM_NT && c.ok(); ++tid)
-            for (int q = 0; q < LM_FILL_Q; ++q) {
-                int ly, j;
-                if (!lenia_fill_at(H, tid, q, ly, j)) continue;
-                c.need(ly >= 0 && ly < rows && j >= 0 && j < LM_NCH, "fill load (%d, %d) off the tile", ly, j);
-                if (!c.ok()) break;
-                c.need(hit[(size_t)ly * LM_NCH + j]++ == 0, "tile chunk (%d, %d) loaded twice", ly, j);
-                if (q < LM_CELLS_Q) {
-                    const int i = tid + LM_NT * q;
-                    c.need(ly == H + i / (LM_BX / 4) && 4 * j == 16 * LM_HC + 4 * (i % (LM_BX / 4)),
-                           "fill load %d of thread %d is not its growth's cells", q, tid);
-                }
+    // the fill's loads cover the tile once, and loads 0..3 are the growth's cells
+    std::vector<uint8_t> hit((size_t)rows * LM_NCH, 0);
+    for (int tid = 0; tid < LM_NT && c.ok(); ++tid)
+        for (int q = 0; q < LM_FILL_Q; ++q) {
+            int ly, j;
+            if (!lenia_fill_at(H, tid, q, ly, j)) continue;
+            c.need(ly >= 0 && ly < rows && j >= 0 && j < LM_NCH, "fill load (%d, %d) off the tile", ly, j);
+            if (!c.ok()) break;
+            c.need(hit[(size_t)ly * LM_NCH + j]++ == 0, "tile chunk (%d, %d) loaded twice", ly, j);
+            if (q < LM_CELLS_Q) {
+                const int i = tid + LM_NT * q;
+                c.need(ly == H + i / (LM_BX / 4) && 4 * j == 16 * LM_HC + 4 * (i % (LM_BX / 4)),
+                       "fill load %d of thread %d is not its growth's cells", q, tid);
             }
-        for (size_t k = 0; k < hit.size() && c.ok(); ++k)
-            c.need(hit[k] == 1, "tile chunk (%zu, %zu) never loaded", k / LM_NCH, k % LM_NCH);
-    }
+        }
+    for (size_t k = 0; k < hit.size() && c.ok(); ++k)
+        c.need(hit[k] == 1, "tile chunk (%zu, %zu) never loaded", k / LM_NCH, k % LM_NCH);
+    // columns: whole chunks inside, whole chunks mapped (lenia_cols_whole), or cells
+    const bool whole = lenia_cols_whole(L), near = lenia_rows_near(L, H);
+    const LmMap mx = lm_axis(L.b1, L.n1, 4), my = lm_axis(L.b2, L.n2, 1);
     int64_t clo = INT64_MAX, chi = INT64_MIN;
+    auto col_span = [&](int64_t lo, int64_t hi) {  // [lo, hi)
+        clo = lo < clo ? lo : clo;
+        chi = hi > chi ? hi : chi;
+    };
     for (int64_t bx = 0; bx < gx && c.ok(); ++bx) {
         const int xa = (int)bx * LM_BX - 16 * LM_HC;
-        if (chunked) {
-            for (int j = 0; j < LM_NCH; ++j) {
-                const int x = xa + 4 * j;
-                int xx;
-                if (x >= 0 && x + 4 <= L.n1) xx = x;
-                else if (L.b1 == CAH_BOUNDARY_PERIODIC) xx = x < 0 ? x + L.n1 : x - L.n1;
-                else if (L.b1 == CAH_BOUNDARY_REFLECTING) xx = x < 0 ? -x - 4 : 2 * L.n1 - x - 4;
-                else continue;
+        for (int j = 0; j < LM_NCH && c.ok(); ++j) {
+            const int x = xa + 4 * j;
+            if (x >= 0 && x + 4 <= L.n1) {
+                c.need(x % 4 == 0, "chunk at column %d not float4-aligned", x);
+                col_span(x, x + 4);
+            } else if (whole) {
+                if (L.b1 == CAH_BOUNDARY_CONSTANT) continue;
+                const int xx = x < 0 ? mx.alo + mx.slo * x : mx.ahi + mx.shi * x;
                 c.need(xx >= 0 && xx + 4 <= L.n1 && xx % 4 == 0, "chunk at column %d read at %d", x, xx);
-                clo = xx < clo ? xx : clo;
-                chi = xx + 4 > chi ? xx + 4 : chi;
+                col_span(xx, xx + 4);
+            } else {
+                col_span(0, 4);  // the chunk's float4 load at the row's start, replaced cell by cell
+                for (int k = 0; k < 4; ++k) {
+                    bool o1;
+                    const int m = map_axis0(L.b1, x + k, L.n1, o1);
+                    if (o1) continue;
+                    c.need(m >= 0 && m < L.n1, "cell at column %d read at %d", x + k, m);
+                    col_span(m, m + 1);
+                }
             }
-        } else {
-            int32_t p, q;
-            c.need(axis_span(L.b1, xa, xa + LM_COLS - 1, L.n1, p, q), "tile columns from %d map off", xa);
-            if (p <= q) clo = p < clo ? p : clo, chi = q + 1 > chi ? q + 1 : chi;
         }
     }
     int32_t rlo = INT32_MAX, rhi = INT32_MIN;
-    for (int64_t by = 0; by < gy && c.ok(); ++by) {
-        int32_t p, q;
-        const int32_t y0 = (int32_t)by * LM_BY - H;
-        c.need(axis_span(L.b2, y0, y0 + rows - 1, L.n2, p, q), "tile rows from %d map off", y0);
-        if (p <= q) rlo = p < rlo ? p : rlo, rhi = q > rhi ? q : rhi;
-    }
+    for (int64_t by = 0; by < gy && c.ok(); ++by)
+        for (int ly = 0; ly < rows; ++ly) {
+            const int y = (int)by * LM_BY - H + ly;
+            int r;
+            bool oy;
+            if (near) {
+                const bool ylo = y < 0, yhi = y >= L.n2;
+                r = ylo ? my.alo + my.slo * y : (yhi ? my.ahi + my.shi * y : y);
+                oy = L.b2 == CAH_BOUNDARY_CONSTANT && (ylo || yhi);
+            } else {
+                r = map_axis0(L.b2, y, L.n2, oy);
+            }
+            if (oy) continue;
+            c.need(r >= 0 && r < L.n2, "tile row %d read at %d", y, r);
+            rlo = r < rlo ? r : rlo;
+            rhi = r > rhi ? r : rhi;
+        }
     if (c.ok() && clo <= chi && rlo <= rhi)
         c.range(a.cur, ((int64_t)rlo * L.pitch + clo) * 4, ((int64_t)rhi * L.pitch + chi) * 4, "MFMA tile reads");
-    // the growth reads and writes whole float4s of full blocks, cells of partial ones
-    lattice(c, a.cur, L, 4, "state read by the growth");
+    c.need(L.pitch >= 4, "pitch %lld below a float4", (long long)L.pitch);
+    c.range(a.cur, 0, 16, "the fill's loads past the tile or off a Constant edge (the lattice's first float4)");
+    // the growth writes whole float4s of full blocks, cells of partial ones
     lattice(c, a.next, L, 4, "state written");
     if (a.mask) c.range(a.mask, 0, (int64_t)L.n1 * L.n2, "mask");
 }

@@ -120,16 +120,24 @@ def test_mfma_with_edges_and_gates(oracle):
             assert np.abs(sol.u[t + 1].astype(np.float64) - ref).max() < 1e-5
 
 
+EDGES = {"PR": (M.Periodic(), M.Reflecting()), "RC": (M.Reflecting(), M.Constant(0.25)),
+         "CP": (M.Constant(0.5), M.Periodic())}
+
+
 @pytest.mark.parametrize("R", [2, 5, 12, 13, 17])
-@pytest.mark.parametrize("dims", [(192, 176), (100, 72)])
-def test_mfma_kernel_radii(oracle, R, dims):
+@pytest.mark.parametrize("dims", [(192, 176), (100, 72), (101, 75), (50, 40)])
+@pytest.mark.parametrize("edges", list(EDGES))
+def test_mfma_kernel_radii(oracle, R, dims, edges):
     """Kernel radii up to the largest the MFMA forms take (halo H = R - 1 <= 16),
     odd and even numbers of nonzero kernel rows (R = 12: 23, R = 13: 25), and
-    both tile fills (chunked: first side a multiple of 4 and at least the tile's
-    96 columns, 65 + 2H rows; cell by cell otherwise), against the FP64 oracle
-    per step."""
-    u = patch_state(dims, (20, 12), (60, 52), 13)
-    space = M.Lattice(dims, (M.Periodic(), M.Reflecting()))
+    every way the tile fill maps its edges: whole float4 chunks (first side a
+    multiple of 4 and at least the tile's 96 columns) or cell by cell (101, 50:
+    chunks straddle the edge; 50: narrower than the tile), rows by the one-period
+    formula (second side at least 65 + 2H) or through map_axis0 (75, 40 and the
+    larger radii at 72), partial blocks on every lattice but the first, against
+    the FP64 oracle per step."""
+    u = patch_state(dims, (dims[0] // 5, dims[1] // 6), (dims[0] // 2, dims[1] // 2), 13)
+    space = M.Lattice(dims, EDGES[edges])
     prob = CAProblem(M.Lenia(R=R), u, 2, space=space, seed=1)
     for field, tol in (("mfma_f16x2", 1e-5), ("mfma_f16", 1e-3)):
         assert "k_lenia_mfma" in HIPBackend(lenia_field=field).check_launches(prob, M.Synchronous())
