@@ -542,18 +542,22 @@ def serving_problems(n, seed=5, lo=64, hi=1024, tmax=300, ltl=True):
         side = int(round(lo * (hi / lo) ** rng.random()))
         steps = int(rng.integers(50, tmax + 1))
         kind = kinds[i % len(kinds)]
+        # initial states in the column-major order the backend takes, as a
+        # Julia caller's arrays are (no transposing copy when submitted), and
+        # written, not lazily mapped zero pages (np.full, not np.zeros), as a
+        # request's data arriving over a connection is
         if kind == 0:
-            u = rng.integers(0, 2, (side, side), dtype=np.int8)
+            u = np.asfortranarray(rng.integers(0, 2, (side, side), dtype=np.int8))
             p = CAProblem(M.Life("conway"), u, steps, seed=i)
         elif kind == 1:
-            p = CAProblem(M.ForestFire(), np.zeros((side, side), np.int8), steps,
+            p = CAProblem(M.ForestFire(), np.full((side, side), 0, np.int8, order="F"), steps,
                           {"p": 0.03, "f": 1e-4}, seed=1000 + i,
                           space=M.Grid(side, side, boundary=M.Reflecting()))
         elif kind == 2:
-            u = rng.integers(0, 2, (side, side), dtype=np.int8)
+            u = np.asfortranarray(rng.integers(0, 2, (side, side), dtype=np.int8))
             p = CAProblem(rule_ltl, u, steps, seed=i)
         else:
-            u = np.zeros((side, side), np.float32)
+            u = np.full((side, side), 0, np.float32, order="F")
             k = max(side // 8, 1)
             a = (side - k) // 2
             u[a:a + k, a:a + k] = rng.random((k, k), dtype=np.float32)
@@ -568,16 +572,24 @@ def poisson_arrivals(n, rate, seed=6):
 
 def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inline_bytes=4 << 20):
     """The server (cah_server_*) fed in arrival order; per request the time its
-    result reached host memory, and the results.  The serving thread submits,
-    ticks and polls; results of `inline_bytes` and more are copied out by
-    `clients` threads, as a server's connection threads would, while the serving
-    thread goes on (a smaller one it fetches itself: its copy takes less than
-    handing it to another Python thread)."""
+    result reached host memory, and the results.  As a server's connection
+    threads would, a submitting thread hands each request over at its arrival
+    time (cah_server_submit copies its initial state into the server's pinned
+    memory without holding the server's lock) and `clients` threads copy out
+    results of `inline_bytes` and more, while the serving thread ticks and polls
+    (a smaller result it fetches itself: its copy takes less than handing it to
+    another Python thread)."""
     import concurrent.futures as cf
+    import threading
     from cahip.server import Server
     srv = Server(budget_bytes=budget, chunk=chunk, lenia_field="mfma_f16")
     if tick_us:
         srv.tune(chunk, tick_us)
+    # sized for the stream at start, as a server sized for its traffic is: the
+    # lattices (two buffers each, rows padded) and the pinned staging of initial
+    # states and results
+    state_bytes = sum(p.u0.nbytes for p in probs)
+    srv.reserve(min(budget, 2.5 * state_bytes), 2.2 * state_bytes)
     for p in probs[:4]:  # warm-up: one request of each kind
         rid = srv.submit(p)
         while not srv.done(rid):
@@ -585,29 +597,39 @@ def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inli
         srv.fetch(rid)
     srv.poll()
     n = len(probs)
-    k, fin, res, index = 0, [0.0] * n, [None] * n, {}
+    fin, res, index = [0.0] * n, [None] * n, {}
     left = n
     pool = cf.ThreadPoolExecutor(max_workers=clients) if clients else None
-    futures = []
+    futures, unknown = [], []
 
     def take(rid, j):
         res[j] = srv.fetch(rid)
         fin[j] = time.perf_counter() - start
 
+    def submitter():
+        for j in range(n):
+            wait = arrivals[j] - (time.perf_counter() - start)
+            if wait > 0:
+                time.sleep(wait)
+            index[srv.submit(probs[j])] = j
+
     start = time.perf_counter()
+    sub = threading.Thread(target=submitter)
+    sub.start()
     while left:
-        now = time.perf_counter() - start
-        while k < n and arrivals[k] <= now:
-            index[srv.submit(probs[k])] = k
-            k += 1
         srv.tick()
-        for rid in srv.poll():
-            j = index[rid]
+        done, unknown = unknown + srv.poll(), []
+        for rid in done:
+            j = index.get(rid)
+            if j is None:  # finished before its submitter recorded it
+                unknown.append(rid)
+                continue
             if pool and probs[j].u0.nbytes >= inline_bytes:
                 futures.append(pool.submit(take, rid, j))
             else:
                 take(rid, j)
             left -= 1
+    sub.join()
     for f in futures:
         f.result()
     if pool:

@@ -122,6 +122,7 @@ SIGNATURES = {
     "cah_server_create": (_I, [_P, _SZ, _I32, ctypes.POINTER(_P)]),
     "cah_server_submit": (_I, [_P, _PD, _P, _SZ, _I64, _I64, _PP, ctypes.POINTER(_U64)]),
     "cah_server_tune": (_I, [_P, _I32, _D]),
+    "cah_server_reserve": (_I, [_P, ctypes.c_size_t, ctypes.c_size_t]),
     "cah_server_tick": (_I, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "cah_server_poll": (_I, [_P, ctypes.POINTER(_U64), _I32, ctypes.POINTER(_I32)]),
     "cah_server_status": (_I, [_P, _U64, ctypes.POINTER(_I32)]),

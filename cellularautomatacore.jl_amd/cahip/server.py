@@ -46,6 +46,11 @@ class Server:
         self.meta[rid.value] = (prob.space.dims, u0.dtype)
         return rid.value
 
+    def reserve(self, device_bytes, pinned_bytes):
+        """Size the server's device and pinned host memory up front
+        (cah_server_reserve), as a server sized for its traffic does at start."""
+        check(self.lib.cah_server_reserve(self.h, int(device_bytes), int(pinned_bytes)))
+
     def tune(self, chunk, tick_us):
         """Steps per request per tick at most, and the estimated device time of a
         tick in microseconds."""

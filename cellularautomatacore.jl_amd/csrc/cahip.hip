@@ -413,6 +413,23 @@ struct Arena {
         return nullptr;
     }
 
+    // Slabs allocated up front until the arena holds `bytes` in all (one more
+    // slab of the shortfall); false when the allocation fails.
+    bool reserve(size_t bytes) {
+        size_t have = 0;
+        for (const auto& kv : free_) have += kv.second.size;
+        for (const auto& kv : used) have += kv.second.size;
+        if (have >= bytes) return true;
+        const size_t sz = (bytes - have + 255) / 256 * 256;
+        char* slab = nullptr;
+        const hipError_t e = pinned ? hipHostMalloc((void**)&slab, sz, hipHostMallocDefault)
+                                    : hipMalloc((void**)&slab, sz);
+        if (e != hipSuccess) return false;
+        slabs.push_back(slab);
+        free_[slab] = Blk{sz, (int)slabs.size() - 1};
+        return true;
+    }
+
     void release(void* vp) {
         char* p = (char*)vp;
         auto u = used.find(p);
@@ -2522,6 +2539,7 @@ struct SrvReq {
     std::vector<char> u0;                // initial state, when it is not in pinned memory
     char* u0_pinned = nullptr;           // initial state in the server's pinned arena
     size_t u0_bytes = 0;
+    bool u0_binary = false;              //   ... every byte of it 0 or 1 (found while copying it)
     int64_t t0 = 0, t1 = 0, queued = 0;  // queued: the time its queued steps reach
     cah_params p{};
     cah_state* st = nullptr;
@@ -2551,6 +2569,8 @@ struct SrvSlot {
 // Pinned memory that the initial states of waiting requests may hold; beyond it
 // they wait in pageable memory and are staged when admitted.
 constexpr size_t SRV_PENDING_PINNED = (size_t)4 << 30;
+// Initial-state bytes one tick call admits at most (beyond its first request).
+constexpr size_t SRV_ADMIT_BYTES = (size_t)256 << 20;
 
 struct cah_server {
     // every entry point holds it, but a fetch drops it while its result is copied
@@ -2696,9 +2716,11 @@ extern "C" int cah_server_submit(cah_server* s, const cah_desc* d, const void* u
     }
     // the copy runs without the lock; the request is not visible until it ends
     lk.unlock();
-    if (r->u0_pinned)
-        hostcopy::copy(r->u0_pinned, u0, nbytes);
-    else if (u0)
+    if (r->u0_pinned) {
+        bool above = true;
+        hostcopy::copy(r->u0_pinned, u0, nbytes, d->dtype == CAH_DTYPE_I8 ? &above : nullptr);
+        r->u0_binary = !above;
+    } else if (u0)
         r->u0.assign((const char*)u0, (const char*)u0 + nbytes);
     lk.lock();
     r->t0 = r->queued = t0;
@@ -2742,7 +2764,8 @@ static int admit(cah_server* s, SrvReq* r) {
     if (r->u0_pinned || !r->u0.empty()) {
         void* h;
         size_t n;
-        if (r->u0_pinned) {
+        const bool pinned = r->u0_pinned != nullptr;
+        if (pinned) {
             // released with the state's other staging once the upload has completed
             h = r->u0_pinned;
             n = r->u0_bytes;
@@ -2760,7 +2783,7 @@ static int admit(cah_server* s, SrvReq* r) {
             if (int rc = copy_rows(dense, (int64_t)row, st->buf[st->cur], (int64_t)pitchb, (int64_t)row,
                                    st->n2, s->up))
                 return rc;
-        if (st->es == 1) st->bin_clean = scan_binary((const int8_t*)h, n);
+        if (st->es == 1) st->bin_clean = pinned ? r->u0_binary : scan_binary((const int8_t*)h, n);
         if (st->es == 1) st->tri_clean = st->bin_clean;
         std::vector<char>().swap(r->u0);
     }
@@ -3148,18 +3171,34 @@ extern "C" int cah_server_tick(cah_server* s, int32_t* active, int32_t* pending)
     std::lock_guard<std::mutex> lk(s->mu);
     HIPCHK(hipSetDevice(s->ctx->device));
     if (int rc = retire(s)) return rc;
-    // admit in arrival order while the budget allows
-    while (!s->pending.empty() && s->used + s->pending.front()->bytes <= s->budget) {
+    // admit in arrival order while the budget allows, at most SRV_ADMIT_BYTES of
+    // initial states a call (at least one request), so that the device starts on
+    // the first requests while later ones are still being set up
+    size_t admitted = 0;
+    while (!s->pending.empty() && s->used + s->pending.front()->bytes <= s->budget &&
+           (admitted == 0 || admitted < SRV_ADMIT_BYTES)) {
         SrvReq* r = s->pending.front();
         if (int rc = admit(s, r)) return rc;
         s->pending.pop_front();
         s->used += r->bytes;
+        admitted += std::max<size_t>(r->bytes / 2, 1);
         s->active.push_back(r);
     }
     if (s->nbusy < SRV_SLOTS)
         if (int rc = build_tick(s)) return rc;
     if (active) *active = (int32_t)s->active.size();
     if (pending) *pending = (int32_t)s->pending.size();
+    return CAH_OK;
+}
+
+extern "C" int cah_server_reserve(cah_server* s, size_t device_bytes, size_t pinned_bytes) {
+    if (!s) return fail(CAH_ERR_ARGUMENT, "server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    HIPCHK(hipSetDevice(s->ctx->device));
+    if (!s->arena.reserve(std::min(device_bytes, s->budget)))
+        return fail(CAH_ERR_OUT_OF_MEMORY, "cannot reserve %zu bytes of device memory", device_bytes);
+    if (!s->pinned.reserve(pinned_bytes))
+        return fail(CAH_ERR_OUT_OF_MEMORY, "cannot reserve %zu bytes of pinned host memory", pinned_bytes);
     return CAH_OK;
 }
 

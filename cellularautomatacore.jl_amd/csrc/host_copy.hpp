@@ -13,6 +13,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstddef>
+#include <cstdint>
 #include <cstring>
 #include <deque>
 #include <mutex>
@@ -29,16 +30,32 @@ struct Batch {
     const char* src;
     size_t n, part;
     int parts;
+    bool scan = false;  // also find whether a byte copied is above 1
     std::atomic<int> next{0};
+    std::atomic<bool> above{false};
     int users = 0;     // helpers copying parts of it (under the pool's lock)
     int finished = 0;  // parts copied (under the pool's lock)
 };
 
+// Whether a byte of the n at p is above 1 (read as unsigned), 64 bits a step.
+inline bool any_above_one(const char* p, size_t n) {
+    uint64_t hi = 0;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        hi |= w;
+    }
+    for (; i < n; ++i) hi |= (uint8_t)p[i];
+    return (hi & 0xFEFEFEFEFEFEFEFEull) != 0;
+}
+
 inline int copy_parts(Batch* b) {
     int k = 0;
     for (int i; (i = b->next.fetch_add(1, std::memory_order_relaxed)) < b->parts; ++k) {
-        const size_t o = (size_t)i * b->part;
-        std::memcpy(b->dst + o, b->src + o, std::min(b->part, b->n - o));
+        const size_t o = (size_t)i * b->part, len = std::min(b->part, b->n - o);
+        std::memcpy(b->dst + o, b->src + o, len);
+        if (b->scan && any_above_one(b->dst + o, len)) b->above.store(true, std::memory_order_relaxed);
     }
     return k;
 }
@@ -79,20 +96,20 @@ inline Pool& pool() {
     return *p;
 }
 
-inline void copy(void* dst, const void* src, size_t n) {
-    if (n < PAR_COPY_MIN) {
-        std::memcpy(dst, src, n);
-        return;
-    }
+// Copies n bytes; with above, also stores whether a byte copied is above 1
+// (read as unsigned), found while each part is still in cache.
+inline void copy(void* dst, const void* src, size_t n, bool* above = nullptr) {
     Pool& P = pool();
-    if (P.helpers == 0) {
+    if (n < PAR_COPY_MIN || P.helpers == 0) {
         std::memcpy(dst, src, n);
+        if (above) *above = any_above_one((const char*)dst, n);
         return;
     }
     Batch b;
     b.dst = (char*)dst;
     b.src = (const char*)src;
     b.n = n;
+    b.scan = above != nullptr;
     // about four parts per thread, whole cache lines each
     b.part = std::max(PART_MIN, (n / (4 * (size_t)(P.helpers + 1)) + 63) / 64 * 64);
     b.parts = (int)((n + b.part - 1) / b.part);
@@ -108,6 +125,7 @@ inline void copy(void* dst, const void* src, size_t n) {
     // every part is claimed now: drop the requests no helper has taken
     P.queue.erase(std::remove(P.queue.begin(), P.queue.end(), &b), P.queue.end());
     P.done.wait(lk, [&] { return b.users == 0 && b.finished == b.parts; });
+    if (above) *above = b.above.load();
 }
 
 }  // namespace hostcopy

@@ -357,6 +357,13 @@ int cah_server_create(cah_ctx* ctx, size_t budget_bytes, int32_t chunk, cah_serv
  * resident request advances `chunk` steps a tick; a limit gives the steps to the
  * oldest requests first). */
 int cah_server_tune(cah_server* s, int32_t chunk, double tick_us);
+/* Allocate the server's device arena (lattices, tick tables; at most the
+ * budget) and pinned host arena (initial states waiting for admission, uploads,
+ * results) up front until they hold device_bytes and pinned_bytes, so that a
+ * server sized for its traffic pays no allocation while it serves (each arena
+ * otherwise grows by slabs as requests need them).  No counterpart in the
+ * reference: a serving loop's start-up sizing. */
+int cah_server_reserve(cah_server* s, size_t device_bytes, size_t pinned_bytes);
 /* Queue a request: the problem (copied), its initial state (host, copied; NULL =
  * all zero), steps t0 .. t1-1 and parameters.  Returns its id. */
 int cah_server_submit(cah_server* s, const cah_desc* d, const void* u0, size_t nbytes,

@@ -1,5 +1,7 @@
 // Concurrent callers of hostcopy::copy at sizes around its thresholds, each
-// copy checked byte for byte; prints "ok" and exits 0 when all match.
+// copy checked byte for byte, and its scan (whether a byte copied is above 1)
+// against a direct count on contents of 0s and 1s with at most one byte 2..255
+// planted at a random place; prints "ok" and exits 0 when all match.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -27,6 +29,14 @@ int main(int argc, char** argv) {
                     hostcopy::copy(dst.data() + 3, src.data() + off, n);
                     for (size_t i = 0; i < n; ++i)
                         if (dst[3 + i] != src[off + i]) { ++bad[c]; break; }
+                    // the scan: 0/1 contents, then one byte above 1 planted (or none)
+                    for (size_t i = 0; i < src.size(); ++i) src[i] &= 1;
+                    if (n && (r & 1)) src[off + rng() % n] = (uint8_t)(2 + rng() % 254);
+                    bool want = false;
+                    for (size_t i = 0; i < n; ++i) want |= src[off + i] > 1;
+                    bool above = !want;
+                    hostcopy::copy(dst.data() + 3, src.data() + off, n, &above);
+                    if (above != want) ++bad[c];
                     // bytes outside the destination range stay untouched
                     if (dst[0] != 0xA5 || dst[1] != 0xA5 || dst[2] != 0xA5 || dst[3 + n] != 0xA5) ++bad[c];
                 }
