@@ -570,15 +570,15 @@ def poisson_arrivals(n, rate, seed=6):
     return np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
 
 
-def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inline_bytes=4 << 20):
+def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inline_bytes=4 << 20, submitters=4):
     """The server (cah_server_*) fed in arrival order; per request the time its
     result reached host memory, and the results.  As a server's connection
-    threads would, a submitting thread hands each request over at its arrival
-    time (cah_server_submit copies its initial state into the server's pinned
-    memory without holding the server's lock) and `clients` threads copy out
-    results of `inline_bytes` and more, while the serving thread ticks and polls
-    (a smaller result it fetches itself: its copy takes less than handing it to
-    another Python thread)."""
+    threads would, `submitters` threads hand the requests over, each taking the
+    next one in arrival order at its arrival time (cah_server_submit copies its
+    initial state into the server's pinned memory without holding the server's
+    lock), and `clients` threads copy out results of `inline_bytes` and more,
+    while the serving thread ticks and polls (a smaller result it fetches itself:
+    its copy takes less than handing it to another Python thread)."""
     import concurrent.futures as cf
     import threading
     from cahip.server import Server
@@ -606,16 +606,24 @@ def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inli
         res[j] = srv.fetch(rid)
         fin[j] = time.perf_counter() - start
 
+    nxt, lock = [0], threading.Lock()
+
     def submitter():
-        for j in range(n):
+        while True:
+            with lock:
+                j = nxt[0]
+                nxt[0] += 1
+            if j >= n:
+                return
             wait = arrivals[j] - (time.perf_counter() - start)
             if wait > 0:
                 time.sleep(wait)
             index[srv.submit(probs[j])] = j
 
     start = time.perf_counter()
-    sub = threading.Thread(target=submitter)
-    sub.start()
+    subs = [threading.Thread(target=submitter) for _ in range(submitters)]
+    for t in subs:
+        t.start()
     while left:
         srv.tick()
         done, unknown = unknown + srv.poll(), []
@@ -629,7 +637,8 @@ def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inli
             else:
                 take(rid, j)
             left -= 1
-    sub.join()
+    for t in subs:
+        t.join()
     for f in futures:
         f.result()
     if pool:
