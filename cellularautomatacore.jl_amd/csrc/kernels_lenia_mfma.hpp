@@ -43,15 +43,12 @@ constexpr int LM_HMAX = 16;  // kernel radius the MFMA forms take
 constexpr int LM_NT = 256;   // threads a block
 
 // The tile geometry: 16-column chunks of halo a side (HC), of a row (NC) and
-// columns a row, the LDS row stride in elements (a 16-byte-aligned row, padded
-// by 16 bytes so that the 16 rows one MFMA operand reads start in different
-// banks), the most rows (65 + 2H), float4 chunks a row, and the float4 loads a
-// thread makes for the largest tile.
+// columns a row, the most rows (65 + 2H), float4 chunks a row, and the float4
+// loads a thread makes for the largest tile.
 constexpr int LM_HC = 1;
 constexpr int LM_NC = 4 + 2 * LM_HC;
 constexpr int LM_NR = 2 * LM_HC + 1;  // chunk offsets a strip needs
 constexpr int LM_COLS = 16 * LM_NC;
-constexpr int LM_STRIDE = LM_COLS + 8;
 constexpr int LM_ROWS = LM_BY + 2 * LM_HMAX + 1;
 constexpr int LM_NCH = LM_COLS / 4;
 constexpr int LM_FILL_Q = (LM_ROWS * LM_NCH + LM_NT - 1) / LM_NT;
@@ -88,8 +85,17 @@ struct LmT {
     }
 };
 
+// The LDS row stride in elements, a whole number of 16-byte chunks.  Measured
+// at 2048^2, R = 13 (SQ_LDS_BANK_CONFLICT): 224-byte rows take the B reads of the
+// MFMA loop with 0.11 M conflict cycles a launch, 208-, 240- and 272-byte rows
+// with 1.39 M (the time is the same: the loop waits on the MFMAs).  Split FP16
+// keeps 208-byte rows, with which four blocks of its two planes fit a CU's
+// 160 KB of LDS (with 224 three do: 26.0 against 24.5 us).
 template <int MODE>
-using LmTile = typename LmT<MODE>::E[MODE == LM_F16X2 ? 2 : 1][LM_ROWS * LM_STRIDE];
+constexpr int lm_stride() { return MODE == LM_F16X2 ? LM_COLS + 8 : LM_COLS + 16; }
+
+template <int MODE>
+using LmTile = typename LmT<MODE>::E[MODE == LM_F16X2 ? 2 : 1][LM_ROWS * lm_stride<MODE>()];
 
 // ---------------------------------------------------------------- the input tile
 //
@@ -254,7 +260,7 @@ template <int MODE>
 CAH_DEV void lenia_fill_store(const LeniaMfmaArgs& a, const float4 (&v)[LM_FILL_Q], LmTile<MODE>& tile) {
     using E = typename LmT<MODE>::E;
     typedef E e4 __attribute__((ext_vector_type(4)));
-    constexpr int S = LM_STRIDE;
+    constexpr int S = lm_stride<MODE>();
 #pragma unroll
     for (int q = 0; q < LM_FILL_Q; ++q) {
         int ly, j;
@@ -376,7 +382,7 @@ CAH_DEV void lenia_mfma_loop(const LeniaMfmaArgs& a, const LmTile<MODE>& tile, l
     const V8* A = static_cast<const V8*>(a.afrag) + lane;
     const int nfrag = a.npairs * NR * 64;
     // this lane's row of the first pair, and its half of a chunk
-    const int base = (16 * w + n + a.H + a.dy0 + (g >> 1)) * LM_STRIDE + 8 * (g & 1);
+    const int base = (16 * w + n + a.H + a.dy0 + (g >> 1)) * lm_stride<MODE>() + 8 * (g & 1);
     const E* b0 = &tile[0][base];
     const E* b1 = &tile[MODE == LM_F16X2 ? 1 : 0][base];
 #pragma unroll
@@ -388,7 +394,7 @@ CAH_DEV void lenia_mfma_loop(const LeniaMfmaArgs& a, const LmTile<MODE>& tile, l
             ahi[r] = A[(p * NR + r) * 64];
             if constexpr (MODE == LM_F16X2) alo[r] = A[nfrag + (p * NR + r) * 64];
         }
-        const int po = 2 * p * LM_STRIDE;
+        const int po = 2 * p * lm_stride<MODE>();
 #pragma unroll
         for (int c = 0; c < LM_NC; ++c) {
             const V8 b = *reinterpret_cast<const V8*>(b0 + po + 16 * c);

@@ -422,9 +422,10 @@ inline void lenia_mfma(Chk& c, const LeniaMfmaArgs& a, int mode, int64_t gx, int
     c.need(a.npairs >= 1 && a.dy0 >= -H && a.dy0 + 2 * a.npairs - 1 <= H + 1,
            "row-offset pairs from %d, %d of them, for H = %d", a.dy0, a.npairs, H);
     c.need(63 + H + a.dy0 + 2 * a.npairs - 1 < rows, "MFMA rows beyond the tile");
-    c.need(16 * (LM_NC - 1) + 15 < LM_STRIDE, "MFMA columns beyond the LDS row");
+    c.need(16 * (LM_NC - 1) + 15 < (mode == LM_F16X2 ? lm_stride<LM_F16X2>() : lm_stride<LM_F16>()),
+           "MFMA columns beyond the LDS row");
     const int np = mode == LM_F16X2 ? 2 : 1;
-    lds(c, sizeof(LmTile<LM_F16X2>) / 2 * np);
+    lds(c, mode == LM_F16X2 ? sizeof(LmTile<LM_F16X2>) : sizeof(LmTile<LM_F16>));
     c.need(LM_BY * LM_UST * 4 <= (int)(sizeof(LmTile<LM_F16>)), "potentials past the tile buffer");
     c.need(plane == lenia_mfma_fragment_bytes(a.npairs), "fragment plane of %zu bytes for %d pairs", plane,
            a.npairs);
