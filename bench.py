@@ -38,6 +38,7 @@ VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12
 # cells (4 draws, the rule, the store; the branch taken on a tie of the draw's high word left out)
 FOREST_VALU_PER_DRAW = 181 / 4
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_ltl8192_forest16384.json")
+LENIA_TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_lenia_mfma_2048.json")
 COPY_N = 16384 * 16384  # bytes of the calibration copy (one 16384^2 Int8 lattice)
 KERNELS = {"ltl": "k_life_swar<5, 2, true, 1>", "forest": "k_forest_vn1<1, true, 0, true>",
            "copy": "k_copy_rows<16>"}
@@ -506,14 +507,20 @@ def bench_lenia(be, ev, steps=500):
             npairs = (int(dys.max() - dys.min()) + 2) // 2
             per_block = 4 * npairs * 12 * 2 * 16 * 16 * 32 * (3 if field == "mfma_f16x2" else 1)
             exec_tflops = per_block * (n // 64) ** 2 / sec / 1e12
+            traffic = None
+            if os.path.exists(LENIA_TRAFFIC_JSON):
+                with open(LENIA_TRAFFIC_JSON) as f:
+                    traffic = json.load(f).get(field, {}).get("bytes_per_launch")
             out[field]["roofline"] = {
                 "bound": "mfma", "kernel": "k_lenia_mfma", "achieved": round(algo_tflops, 1),
                 "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(algo_tflops / FP16_PEAK_TFLOPS, 4),
                 "algorithmic_flop_per_cell": 2 * taps,
                 "executed_tflops": round(exec_tflops, 1),
                 "executed_frac": round(exec_tflops / FP16_PEAK_TFLOPS, 4),
+                "traffic": traffic,
                 "note": "achieved counts the direct sum's 2 x 516 flop a cell; executed counts every MFMA "
-                        "the implicit GEMM issues, the Toeplitz band's zeros included"}
+                        "the implicit GEMM issues, the Toeplitz band's zeros included; traffic: HBM bytes a "
+                        "launch from FETCH_SIZE / WRITE_SIZE (profiles/hbm_traffic_lenia_mfma_2048.json)"}
         else:
             out[field]["tflops_algorithmic"] = round(algo_tflops, 2)
     ref_be.free_state(ref)
