@@ -212,9 +212,11 @@ CAH_DEV void forest_band(const ForestArgs& a, int wid) {
     else forest_walk<VB, SMALL, PREC, false, TRI, GATED>(a, b.x, b.ys, b.ye, b.writer);
 }
 
+// Waves in the XCD-aware block order (wave_id_xcd): a wave's neighbouring bands,
+// which share the cache lines at the band edges, run on the same XCD.
 template <int VB, bool SMALL, int PREC, bool TRI>
 __global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
-    const int wid = wave_id();
+    const int wid = wave_id_xcd();
     if (wid >= a.nwaves) return;
     forest_band<VB, SMALL, PREC, TRI>(a, wid);
 }
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(256) void k_forest_vn1(ForestArgs a) {
 // The forest fire under the gate and mask of a.g, one sweep (4 cells a lane).
 template <bool SMALL, int PREC, bool TRI>
 __global__ __launch_bounds__(256) void k_forest_gated(ForestArgs a) {
-    const int wid = wave_id();
+    const int wid = wave_id_xcd();
     if (wid >= a.nwaves) return;
     forest_band<1, SMALL, PREC, TRI, true>(a, wid);
 }
