@@ -58,6 +58,16 @@ inline void launch_life_gated(int R, int vb, bool bin, int niv, int blocks, cons
     }
 }
 
+inline int life_gated_blocks_per_cu(int R, int vb, bool bin, int niv) {
+    switch (R) {
+    case 1: return life_gated_blocks_per_cu_r<1>(vb, bin, niv);
+    case 2: return life_gated_blocks_per_cu_r<2>(vb, bin, niv);
+    case 3: return life_gated_blocks_per_cu_r<3>(vb, bin, niv);
+    case 4: return life_gated_blocks_per_cu_r<4>(vb, bin, niv);
+    default: return life_gated_blocks_per_cu_r<5>(vb, bin, niv);
+    }
+}
+
 inline void launch_life_batch(int R, int vb, bool bin, int niv, int blocks, const LifeArgs* it,
                               const int32_t* wi, const int32_t* wb, int32_t nw, int32_t parity,
                               hipStream_t st) {
@@ -328,6 +338,7 @@ struct cah_state {
     bool tri_clean = false;  // every cell is 0, 1 or 2 (the forest fire's k_forest_vn1 variant)
     LifeArgs life{};
     int32_t tune_vb = 0, tune_rows = 0;
+    int gated_occ[2] = {-1, -1};  // k_life_gated blocks a CU holds (by bin_clean), -1: not asked yet
     struct Arena* arena = nullptr;        // device memory source (nullptr: hipMalloc)
     struct Arena* staging = nullptr;      // pinned host memory of a server's uploads
     bool own_stream = true;               // false: the stream belongs to a server
@@ -1080,14 +1091,15 @@ static int device_cus(int device) {
 // blocks: 24.8-25.5 us; 45 rows, a few CUs with four: 28.5 us).  A small
 // lattice gets short chunks (4 rows at least): its waves are few, and more of
 // them finish sooner despite the 2R-row warm-up each.
-static int32_t band_rows(const cah_state* s, int64_t nbands, int R) {
+// waves_cu: the walk's waves a CU holds at once (16: four blocks of four waves).
+static int32_t band_rows(const cah_state* s, int64_t nbands, int R, int waves_cu = 16) {
     if (s->tune_rows > 0) return s->tune_rows;
     int64_t rows;
     if (R == 1) {
         rows = (s->n2 * nbands + 4095) / 4096;
         rows = std::min<int64_t>(std::max<int64_t>(rows, 4), 32);
     } else {
-        const int64_t chunks = std::max<int64_t>(1, (int64_t)16 * device_cus(s->device) / nbands);
+        const int64_t chunks = std::max<int64_t>(1, (int64_t)waves_cu * device_cus(s->device) / nbands);
         rows = (s->n2 + chunks - 1) / chunks;
         rows = std::min<int64_t>(std::max<int64_t>(rows, 4), 256);
     }
@@ -1141,7 +1153,15 @@ static int step_life(cah_state* s, int64_t t, int gate) {
     const int vb = life_vb(s);
     const int64_t ow = 62 * 4 * vb;
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
-    a.rows = band_rows(s, a.nbands, s->life_R);
+    // the gated walk of a wide radius holds fewer waves a CU (registers): its
+    // chunks are sized so that every wave is resident at once all the same
+    int waves_cu = 16;
+    if (gated && !s->dry) {
+        int& occ = s->gated_occ[s->bin_clean ? 1 : 0];
+        if (occ < 0) occ = life_gated_blocks_per_cu(s->life_R, vb, s->bin_clean, life_niv(a));
+        if (occ > 0) waves_cu = std::min(16, 4 * occ);
+    }
+    a.rows = band_rows(s, a.nbands, s->life_R, waves_cu);
     a.nwaves = (int32_t)(a.nbands * ((s->n2 + a.rows - 1) / a.rows));
     const int blocks = waves_to_blocks(a.nwaves);
     const int R = s->life_R;
@@ -1348,10 +1368,10 @@ static void check_generic_args(lchk::Chk& c, const cah_state* s, const GenArgs& 
 static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p) {
     const GenArgs a = generic_args(s, t, gate, p);
     dim3 grid((unsigned)((s->n1 + GEN_TX - 1) / GEN_TX), (unsigned)((s->n2 + GEN_TY - 1) / GEN_TY));
-    const size_t lds = (size_t)(GEN_TX + 2 * a.h) * (GEN_TY + 2 * a.h);
+    const size_t lds = (size_t)gen_w(a.h) * gen_rows(a.h);
     if (int rc = checked(s->dry, "k_generic_i8", [&](lchk::Chk& c) {
             lchk::grid_y(c, grid.y);
-            lchk::tile_kernel(c, a.L, a.cur, 1, GEN_TX, GEN_TY, a.h, grid.x, grid.y, lds);
+            lchk::generic_tile(c, a.L, a.cur, a.h, grid.x, grid.y, lds);
             check_generic_args(c, s, a);
         }))
         return rc;

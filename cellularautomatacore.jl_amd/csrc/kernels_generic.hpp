@@ -2,7 +2,8 @@
 //
 //   k_generic_i8  any neighbourhood (halo <= 32), any Int8 rule, every gated
 //                 scheme, masks.  An LDS tile with the boundary conditions applied
-//                 while it is loaded (no halo buffer, no separate halo launch).
+//                 while it is loaded (no halo buffer, no separate halo launch);
+//                 four cells a thread, counted a dword at a time.
 #pragma once
 #include <type_traits>
 
@@ -76,21 +77,15 @@ CAH_DEV int epidemic_rule(const GenArgs& a, int c, int64_t lin, uint64_t cellid)
 
 constexpr int GEN_TX = 64, GEN_TY = 16;
 
-// One cell of the general sweep: cp points at the cell in an LDS tile of width W
-// that holds its neighbourhood; (x, y) are its 0-based coordinates.
-CAH_DEV int generic_cell(const GenArgs& a, const int8_t* cp, int W, int x, int y) {
+// The rule of one cell of the general sweep given its neighbour counts n1 (cells
+// equal to 1) and n2 (equal to 2): the gate and mask first (_update_cell!,
+// src/backends.jl:444-455); (x, y) are its 0-based coordinates.
+CAH_DEV int generic_rule(const GenArgs& a, int c, int n1, int n2, int x, int y) {
     const int64_t lin = (int64_t)y * a.L.n1 + x;  // unpadded, 0-based
     const uint64_t cellid = (uint64_t)lin + 1;
-    const int c = *cp;
     const bool open = (a.mask == nullptr || a.mask[lin] != 0) &&
                       gate_open(a.scheme, a.gate, a.sp, a.sp_f32, a.seed, cellid, a.t, x, y, a.L.two_d);
     if (!open) return c;
-    int n1 = 0, n2 = 0;  // neighbours equal to 1, and to 2
-    for (int k = 0; k < a.nofs; ++k) {
-        const int v = cp[a.ofs[2 * k] + a.ofs[2 * k + 1] * W];
-        n1 += (v == 1);
-        n2 += (v == 2);
-    }
     if (a.rule == CAH_RULE_LIFE) {
         // local_rule(::Life) (src/rules.jl:385-389)
         const uint64_t m = c == 1 ? a.survive : a.birth;
@@ -110,22 +105,89 @@ CAH_DEV int generic_cell(const GenArgs& a, const int8_t* cp, int W, int x, int y
     return bernoulli(a.seed, cellid, a.t, 2, a.fff, a.fff_f32) ? 2 : 1;
 }
 
+// One cell of the general sweep: cp points at the cell in an LDS tile of width W
+// that holds its neighbourhood (the sparse sweep's form, a cell at a time).
+CAH_DEV int generic_cell(const GenArgs& a, const int8_t* cp, int W, int x, int y) {
+    int n1 = 0, n2 = 0;
+    for (int k = 0; k < a.nofs; ++k) {
+        const int v = cp[a.ofs[2 * k] + a.ofs[2 * k + 1] * W];
+        n1 += (v == 1);
+        n2 += (v == 2);
+    }
+    return generic_rule(a, *cp, n1, n2, x, y);
+}
+
+// The general sweep's LDS tile for halo h: columns x0 - hp .. (hp = h rounded up
+// to 4, so that each thread's four cells start a dword), W bytes a row (the 64
+// cells, hp on each side and 8 more, so that the dword after the last one a
+// count reads is in the row), 16 + 2h rows.
+CAH_HD int gen_hp(int h) { return (h + 3) & ~3; }
+CAH_HD int gen_w(int h) { return GEN_TX + 2 * gen_hp(h) + 8; }
+CAH_HD int gen_rows(int h) { return GEN_TY + 2 * h; }
+
+// The general sweep, four cells a thread.  Block (bx, by) computes the 64 x 16
+// cells at (64 bx, 16 by): thread tid the dword of cells 4 (tid % 16) .. + 3 of
+// row tid / 16.  The tile is filled a dword at a time where it lies inside the
+// lattice, else cell by cell through the boundary map.  A neighbour offset
+// (o1, o2) is the same for all threads, so each thread reads the four
+// neighbours of its four cells as one dword (two aligned dwords and a byte
+// shift by o1 mod 4), tests all four for == 1 and == 2 at once (bytes_eq) and
+// adds the results into byte counters (flushed into 32-bit counts every 255
+// offsets); the rule then runs cell by cell.
 __global__ __launch_bounds__(256) void k_generic_i8(GenArgs a) {
-    extern __shared__ int8_t tile[];
-    const int W = GEN_TX + 2 * a.h, H = GEN_TY + 2 * a.h;
+    extern __shared__ uint32_t tile32[];
+    int8_t* tile = reinterpret_cast<int8_t*>(tile32);
+    const int h = a.h, hp = gen_hp(h), W = gen_w(h), rows = gen_rows(h), WD = W / 4;
     const int x0 = blockIdx.x * GEN_TX, y0 = blockIdx.y * GEN_TY;
-    for (int i = threadIdx.x; i < W * H; i += blockDim.x) {
-        const int lx = i % W, ly = i / W;
-        tile[i] = read_mapped<int8_t>(a.cur, a.L, (int8_t)a.cv1, (int8_t)a.cv2, x0 - a.h + lx,
-                                      y0 - a.h + ly);
+    const int xs = x0 - hp, ys = y0 - h;
+    if (xs >= 0 && xs + W <= a.L.n1 && ys >= 0 && ys + rows <= a.L.n2) {
+        const int8_t* src = a.cur + (int64_t)ys * a.L.pitch + xs;
+        for (int i = threadIdx.x; i < rows * WD; i += blockDim.x) {
+            const int ly = i / WD, d = i - ly * WD;
+            tile32[i] = *reinterpret_cast<const uint32_t*>(src + (int64_t)ly * a.L.pitch + 4 * d);
+        }
+    } else {
+        for (int i = threadIdx.x; i < W * rows; i += blockDim.x) {
+            const int lx = i % W, ly = i / W;
+            tile[i] = read_mapped<int8_t>(a.cur, a.L, (int8_t)a.cv1, (int8_t)a.cv2, xs + lx, ys + ly);
+        }
     }
     __syncthreads();
-    const int tx = threadIdx.x % GEN_TX;
-    for (int ty = threadIdx.x / GEN_TX; ty < GEN_TY; ty += blockDim.x / GEN_TX) {
-        const int x = x0 + tx, y = y0 + ty;
-        if (x >= a.L.n1 || y >= a.L.n2) continue;
-        const int out = generic_cell(a, tile + (tx + a.h) + (ty + a.h) * W, W, x, y);
-        a.next[(int64_t)y * a.L.pitch + x] = (int8_t)out;
+    const int tx = 4 * (threadIdx.x % 16), ty = threadIdx.x / 16;
+    const int x = x0 + tx, y = y0 + ty;
+    if (y >= a.L.n2 || x >= a.L.n1) return;
+    const int at = (ty + h) * W + hp + tx;  // byte of the first cell, a multiple of 4
+    uint32_t n1[4] = {0, 0, 0, 0}, n2[4] = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < a.nofs; k0 += 255) {
+        const int k1 = min(a.nofs, k0 + 255);
+        uint32_t b1 = 0, b2 = 0;  // byte counters, at most 255 each
+        for (int k = k0; k < k1; ++k) {
+            const int o1 = a.ofs[2 * k], o2 = a.ofs[2 * k + 1];
+            const int b = at + o2 * W + o1, sh = b & 3, dw = b >> 2;
+            const uint32_t lo = tile32[dw];
+            const uint32_t v = sh ? __builtin_amdgcn_alignbyte(tile32[dw + 1], lo, sh) : lo;
+            b1 += bytes_eq(v, 0x01010101u);
+            b2 += bytes_eq(v, 0x02020202u);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            n1[e] += (b1 >> (8 * e)) & 0xFFu;
+            n2[e] += (b2 >> (8 * e)) & 0xFFu;
+        }
+    }
+    const uint32_t cw = tile32[at >> 2];
+    uint32_t out = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int c = (int)(int8_t)(cw >> (8 * e));
+        const int o = x + e < a.L.n1 ? generic_rule(a, c, (int)n1[e], (int)n2[e], x + e, y) : 0;
+        out |= (uint32_t)(uint8_t)o << (8 * e);
+    }
+    int8_t* dst = a.next + (int64_t)y * a.L.pitch + x;
+    if (x + 4 <= a.L.n1) {
+        *reinterpret_cast<uint32_t*>(dst) = out;
+    } else {
+        for (int e = 0; x + e < a.L.n1; ++e) dst[e] = (int8_t)(out >> (8 * e));
     }
 }
 

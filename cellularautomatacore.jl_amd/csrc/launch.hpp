@@ -17,6 +17,9 @@ void launch_life_r(int vb, bool bin, int niv, int blocks, const LifeArgs& a, hip
 // bin = every cell read is 0 or 1; niv = intervals per mask (1 or 2).
 template <int R>
 void launch_life_gated_r(int vb, bool bin, int niv, int blocks, const LifeArgs& a, hipStream_t st);
+// Blocks of that kernel a CU holds at once (0: unknown).
+template <int R>
+int life_gated_blocks_per_cu_r(int vb, bool bin, int niv);
 template <int R>
 void launch_life_batch_r(int vb, bool bin, int niv, int blocks, const LifeArgs* items,
                          const int32_t* wave_item, const int32_t* wave_base, int32_t nwaves,

@@ -397,6 +397,42 @@ inline void tile_kernel(Chk& c, const Lat& L, const void* cur, int64_t es, int T
         c.range(cur, ((int64_t)rlo * L.pitch + clo) * es, ((int64_t)rhi * L.pitch + chi + 1) * es, "tile reads");
 }
 
+// k_generic_i8 over a gx x gy grid of 64 x 16 tiles, halo h: each block's tile
+// (columns from 64 bx - gen_hp(h), gen_w(h) of them; rows from 16 by - h,
+// gen_rows(h) of them) read a dword at a time where it lies inside the lattice,
+// else cell by cell through the boundary map; the count's LDS reads (the dword
+// after a thread's last neighbour dword included) within the tile.
+inline void generic_tile(Chk& c, const Lat& L, const void* cur, int h, int64_t gx, int64_t gy, size_t lds_bytes) {
+    const int hp = gen_hp(h), W = gen_w(h), rows = gen_rows(h);
+    c.need(h >= 0 && h <= 32, "halo %d", h);
+    c.need(gx * GEN_TX >= L.n1 && gy * GEN_TY >= L.n2, "grid %lld x %lld does not cover %d x %d", (long long)gx,
+           (long long)gy, L.n1, L.n2);
+    c.need(lds_bytes >= (size_t)W * rows && W % 4 == 0, "LDS of %zu bytes for a %d x %d tile", lds_bytes, W, rows);
+    lds(c, lds_bytes);
+    // the last dword a count reads: cell column hp + 60 + h + 3 within the dword
+    // after it, all in the row
+    c.need(((hp + 60 + h) / 4 + 2) * 4 <= W, "count reads past the tile row (h = %d)", h);
+    c.need(L.pitch % 4 == 0, "pitch %lld", (long long)L.pitch);
+    if (!c.ok()) return;
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int64_t by = 0; by < gy && c.ok(); ++by)
+        for (int64_t bx = 0; bx < gx && c.ok(); ++bx) {
+            const int32_t xs = (int32_t)(bx * GEN_TX) - hp, ys = (int32_t)(by * GEN_TY) - h;
+            int32_t c0, c1, r0, r1;
+            if (xs >= 0 && xs + W <= L.n1 && ys >= 0 && ys + rows <= L.n2) {
+                c0 = xs, c1 = xs + W - 1, r0 = ys, r1 = ys + rows - 1;
+            } else {
+                c.need(axis_span(L.b1, xs, xs + W - 1, L.n1, c0, c1), "tile columns from %d map off", xs);
+                c.need(axis_span(L.b2, ys, ys + rows - 1, L.n2, r0, r1), "tile rows from %d map off", ys);
+                if (c0 > c1 || r0 > r1) continue;
+            }
+            const int64_t a = (int64_t)r0 * L.pitch + c0, z = (int64_t)r1 * L.pitch + c1 + 1;
+            lo = a < lo ? a : lo;
+            hi = z > hi ? z : hi;
+        }
+    if (c.ok() && lo <= hi) c.range(cur, lo, hi, "tile reads");
+}
+
 // ---------------------------------------------------------------- k_lenia_mfma
 
 // One lattice of k_lenia_mfma<mode> over a gx x gy grid of 64 x 64 blocks:

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <string>
 
+#include "kernels_generic.hpp"
 #include "kernels_life_tb.hpp"
 #include "launch.hpp"
 #include "lenia_launch.hpp"
@@ -108,6 +109,20 @@ int main() {
         lchk::Chk e("tile kernel, right");
         lchk::tile_kernel(e, L, cur, 1, 64, 16, 8, 16, 19, 4096);
         expect("tile kernel, right", true, e);
+        // the general kernel's tile: halo 5 (columns from 64 bx - 8, 88 a row, 26 rows)
+        const size_t lds = (size_t)gen_w(5) * gen_rows(5);
+        lchk::Chk f("general tile, right");
+        lchk::generic_tile(f, L, cur, 5, 16, 19, lds);
+        expect("general tile, right", true, f);
+        lchk::Chk g("general tile, LDS for a smaller halo");
+        lchk::generic_tile(g, L, cur, 5, 16, 19, (size_t)gen_w(4) * gen_rows(4));
+        expect(g.kernel.c_str(), false, g);
+        lchk::Chk h("general tile, grid short");
+        lchk::generic_tile(h, L, cur, 5, 16, 18, lds);
+        expect(h.kernel.c_str(), false, h);
+        lchk::Chk i("general tile, halo past 32");
+        lchk::generic_tile(i, L, cur, 33, 16, 19, (size_t)gen_w(33) * gen_rows(33));
+        expect(i.kernel.c_str(), false, i);
     }
     // k_lenia_mfma: 2048^2 FP32, H = 12, rows -11 .. 11 (12 pairs): fragments of
     // the right size, then planted mistakes
