@@ -577,15 +577,17 @@ def poisson_arrivals(n, rate, seed=6):
     return np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
 
 
-def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=2, inline_bytes=4 << 20, submitters=4):
+def serve(probs, arrivals, chunk=32, budget=200e9, tick_us=None, clients=4, inline_bytes=0, submitters=4):
     """The server (cah_server_*) fed in arrival order; per request the time its
     result reached host memory, and the results.  As a server's connection
     threads would, `submitters` threads hand the requests over, each taking the
     next one in arrival order at its arrival time (cah_server_submit copies its
     initial state into the server's pinned memory without holding the server's
-    lock), and `clients` threads copy out results of `inline_bytes` and more,
-    while the serving thread ticks and polls (a smaller result it fetches itself:
-    its copy takes less than handing it to another Python thread)."""
+    lock), and `clients` threads copy out results of `inline_bytes` and more
+    (by default every result), while the serving thread ticks and polls.  A
+    result the serving thread fetched itself would hold back the next tick: with
+    the results below 4 MB fetched inline the device sat idle for about half of
+    the stream's last 80 ms (1,020 against 1,086 sims/s on the 400 requests)."""
     import concurrent.futures as cf
     import threading
     from cahip.server import Server
