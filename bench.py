@@ -704,7 +704,8 @@ def bench_gated(be, ev, steps=50):
     window of `steps` steps after 100 ms of untimed ones): Generations, the
     Stochastic and Checkerboard gates and a mask on the gated band walks
     (k_life_gated, k_forest_gated), and a neighbourhood no band walk takes
-    (VonNeumann(2)) on the general LDS-tile kernel (k_generic_i8).  Roofline:
+    (VonNeumann(2), and Moore(10) with 440 neighbours) on the window-sum kernel
+    (k_box_i8).  Roofline:
     algorithmic bytes (one Int8 read and write a cell, and a mask byte read
     where there is a mask) against 8 TB/s."""
     from cahip import model as M
@@ -734,9 +735,13 @@ def bench_gated(be, ev, steps=50):
          CAProblem(M.ForestFire(), np.zeros((n2, n2), np.int8), 1, ff, space=refl, seed=7, mask=mask),
          M.Synchronous(), None, 1),
         ("life_vn2_8192", "Life B3/S23 VonNeumann(2) Torus 8192^2, Bernoulli(0.5) seed 1, Synchronous "
-         "(the general kernel)", CAProblem(M.Life("B3/S23", neighbourhood=M.VonNeumann(2)),
-                                            np.zeros((n1, n1), np.int8), 1, space=torus),
+         "(the window-sum kernel)", CAProblem(M.Life("B3/S23", neighbourhood=M.VonNeumann(2)),
+                                               np.zeros((n1, n1), np.int8), 1, space=torus),
          M.Synchronous(), (1, 0.5), 0),
+        ("ltl_moore10_8192", "Larger-than-Life Moore(10) B34..45/S34..58 Torus 8192^2, Bernoulli(0.05) seed 1, "
+         "Synchronous (the window-sum kernel)",
+         CAProblem(M.Life(range(34, 46), range(34, 59), neighbourhood=M.Moore(10)), np.zeros((n1, n1), np.int8), 1,
+                   space=torus), M.Synchronous(), (1, 0.05), 0),
     ]
     out = {}
     for name, what, prob, scheme, fill, mask_bytes in cases:

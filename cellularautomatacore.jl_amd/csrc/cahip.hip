@@ -29,6 +29,7 @@
 #include "kernels_counts.hpp"
 #include "kernels_epidemic.hpp"
 #include "kernels_life_tb.hpp"
+#include "kernels_box.hpp"
 #include "kernels_generic.hpp"
 #include "launch.hpp"
 #include "lenia_launch.hpp"
@@ -285,9 +286,11 @@ extern "C" int cah_check_problem(const cah_desc* d) {
 
 // P_LIFE_GATED and P_FOREST_GATED: the band walks of Life / Generations and of the
 // forest fire under a Stochastic or Checkerboard gate or a mask (Generations
-// under any scheme), one sweep a launch
+// under any scheme), one sweep a launch.  P_BOX: Life / Generations with the
+// neighbourhoods the band walks do not take, Moore(R <= 32) and VonNeumann(r <= 8)
+// (k_box_i8, any scheme and mask)
 enum Path { P_GENERIC, P_LIFE_SWAR, P_FOREST, P_LENIA_SEQ, P_LENIA_F32COL, P_LENIA_MFMA,
-            P_LENIA_FFT, P_LIFE_GATED, P_FOREST_GATED };
+            P_LENIA_FFT, P_LIFE_GATED, P_FOREST_GATED, P_BOX };
 
 struct cah_state {
     int device = 0;
@@ -334,6 +337,7 @@ struct cah_state {
     int32_t lenia_dy0 = 0, lenia_np = 0;  // MFMA field: first row offset, row-offset pairs
     Path path = P_GENERIC;
     int32_t life_R = 0;
+    bool box_vn = false;     // P_BOX: von Neumann (else Moore) of radius halo
     bool bin_clean = false;  // every cell is 0 or 1 (Life only)
     bool tri_clean = false;  // every cell is 0, 1 or 2 (the forest fire's k_forest_vn1 variant)
     LifeArgs life{};
@@ -363,6 +367,19 @@ static int moore_radius(const std::vector<int32_t>& o) {
     int r = 0;
     for (int k = 0; k < 2 * m; ++k) r = std::max(r, std::abs(o[k]));
     if (r < 1 || m != (2 * r + 1) * (2 * r + 1) - 1) return 0;
+    std::set<std::pair<int, int>> s;
+    for (int k = 0; k < m; ++k) s.insert({o[2 * k], o[2 * k + 1]});
+    if ((int)s.size() != m || s.count({0, 0})) return 0;
+    return r;
+}
+
+// r when the offsets are exactly VonNeumann(r) (every nonzero offset of L1 norm
+// at most r, in any order), else 0.
+static int vn_radius(const std::vector<int32_t>& o) {
+    const int m = (int)o.size() / 2;
+    int r = 0;
+    for (int k = 0; k < m; ++k) r = std::max(r, std::abs(o[2 * k]) + std::abs(o[2 * k + 1]));
+    if (r < 1 || m != 2 * r * (r + 1)) return 0;
     std::set<std::pair<int, int>> s;
     for (int k = 0; k < m; ++k) s.insert({o[2 * k], o[2 * k + 1]});
     if ((int)s.size() != m || s.count({0, 0})) return 0;
@@ -706,6 +723,16 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
             }
             s->life_R = R;
             s->path = plain && d->rule == CAH_RULE_LIFE ? P_LIFE_SWAR : P_LIFE_GATED;
+        }
+        if (s->path == P_GENERIC) {
+            const int rv = vn_radius(s->offsets);
+            if (R >= 1 && R <= BOX_MOORE_MAX) {
+                s->path = P_BOX;
+                s->box_vn = false;
+            } else if (rv >= 1 && rv <= BOX_VN_MAX) {
+                s->path = P_BOX;
+                s->box_vn = true;
+            }
         }
     } else if (d->ndims == 2 && d->rule == CAH_RULE_FOREST_FIRE && is_vn1(s->offsets)) {
         s->path = plain ? P_FOREST : P_FOREST_GATED;
@@ -1379,6 +1406,29 @@ static int launch_generic(cah_state* s, int64_t t, int gate, const cah_params* p
     return CAH_OK;
 }
 
+static int launch_box(cah_state* s, int64_t t, int gate, const cah_params* p) {
+    const GenArgs a = generic_args(s, t, gate, p);
+    dim3 grid((unsigned)((s->n1 + BOX_T - 1) / BOX_T), (unsigned)((s->n2 + BOX_T - 1) / BOX_T));
+    const size_t lds = box_lds(s->box_vn, a.h);
+    if (int rc = checked(s->dry, "k_box_i8", [&](lchk::Chk& c) {
+            lchk::grid_y(c, grid.y);
+            lchk::box_tile(c, a.L, a.cur, a.next, s->box_vn, a.h, grid.x, grid.y, lds);
+            check_generic_args(c, s, a);
+        }))
+        return rc;
+    if (!s->dry) {
+        const bool gated = a.mask || a.scheme != CAH_SCHEME_SYNCHRONOUS;
+        if (s->box_vn) {
+            if (gated) k_box_i8<true, true><<<grid, 256, lds, s->stream>>>(a);
+            else k_box_i8<true, false><<<grid, 256, lds, s->stream>>>(a);
+        } else {
+            if (gated) k_box_i8<false, true><<<grid, 256, lds, s->stream>>>(a);
+            else k_box_i8<false, false><<<grid, 256, lds, s->stream>>>(a);
+        }
+    }
+    return CAH_OK;
+}
+
 // ---------------------------------------------------------------- sparse sweep
 
 static SparseArgs sparse_args(cah_state* s) {
@@ -1786,6 +1836,7 @@ extern "C" int cah_advance(cah_state* s, int64_t t0, int64_t nsteps, const cah_p
             case P_LENIA_FFT: rc = launch_lenia_growth(s, t, g); break;
             case P_LENIA_F32COL: rc = launch_lenia_f32col(s, t); break;
             case P_LENIA_MFMA: rc = launch_lenia_mfma_step(s, t); break;
+            case P_BOX: rc = launch_box(s, t, g, p); break;
             default: rc = launch_generic(s, t, g, p); break;
             }
             if (rc) return rc;

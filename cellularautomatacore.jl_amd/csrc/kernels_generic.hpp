@@ -121,6 +121,49 @@ CAH_DEV int generic_cell(const GenArgs& a, const int8_t* cp, int W, int x, int y
 // to 4, so that each thread's four cells start a dword), W bytes a row (the 64
 // cells, hp on each side and 8 more, so that the dword after the last one a
 // count reads is in the row), 16 + 2h rows.
+// The rows x W bytes (W a multiple of 4) of the state at (xs, ys) into tile32,
+// sixteen loads of a thread in flight at a time: a dword at a time where the
+// whole tile lies inside the lattice, else cell by cell through the boundary map.
+constexpr int TILE_FILL_Q = 16;
+CAH_DEV void fill_tile_i8(const GenArgs& a, uint32_t* tile32, int xs, int ys, int W, int rows) {
+    constexpr int Q = TILE_FILL_Q;
+    const int nt = blockDim.x;
+    if (xs >= 0 && xs + W <= a.L.n1 && ys >= 0 && ys + rows <= a.L.n2) {
+        const int8_t* src = a.cur + (int64_t)ys * a.L.pitch + xs;
+        const int WD = W / 4, n = rows * WD;
+        for (int base = 0; base < n; base += Q * nt) {
+            uint32_t v[Q];
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const int i = base + (int)threadIdx.x + nt * k, ly = i / WD, d = i - ly * WD;
+                v[k] = i < n ? *reinterpret_cast<const uint32_t*>(src + (int64_t)ly * a.L.pitch + 4 * d) : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const int i = base + (int)threadIdx.x + nt * k;
+                if (i < n) tile32[i] = v[k];
+            }
+        }
+    } else {
+        int8_t* tile = reinterpret_cast<int8_t*>(tile32);
+        const int n = W * rows;
+        for (int base = 0; base < n; base += Q * nt) {
+            int8_t v[Q];
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const int i = base + (int)threadIdx.x + nt * k, lx = i % W, ly = i / W;
+                v[k] = i < n ? read_mapped<int8_t>(a.cur, a.L, (int8_t)a.cv1, (int8_t)a.cv2, xs + lx, ys + ly)
+                             : (int8_t)0;
+            }
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const int i = base + (int)threadIdx.x + nt * k;
+                if (i < n) tile[i] = v[k];
+            }
+        }
+    }
+}
+
 CAH_HD int gen_hp(int h) { return (h + 3) & ~3; }
 CAH_HD int gen_w(int h) { return GEN_TX + 2 * gen_hp(h) + 8; }
 CAH_HD int gen_rows(int h) { return GEN_TY + 2 * h; }
@@ -136,22 +179,10 @@ CAH_HD int gen_rows(int h) { return GEN_TY + 2 * h; }
 // offsets); the rule then runs cell by cell.
 __global__ __launch_bounds__(256) void k_generic_i8(GenArgs a) {
     extern __shared__ uint32_t tile32[];
-    int8_t* tile = reinterpret_cast<int8_t*>(tile32);
-    const int h = a.h, hp = gen_hp(h), W = gen_w(h), rows = gen_rows(h), WD = W / 4;
+    const int h = a.h, hp = gen_hp(h), W = gen_w(h), rows = gen_rows(h);
     const int x0 = blockIdx.x * GEN_TX, y0 = blockIdx.y * GEN_TY;
     const int xs = x0 - hp, ys = y0 - h;
-    if (xs >= 0 && xs + W <= a.L.n1 && ys >= 0 && ys + rows <= a.L.n2) {
-        const int8_t* src = a.cur + (int64_t)ys * a.L.pitch + xs;
-        for (int i = threadIdx.x; i < rows * WD; i += blockDim.x) {
-            const int ly = i / WD, d = i - ly * WD;
-            tile32[i] = *reinterpret_cast<const uint32_t*>(src + (int64_t)ly * a.L.pitch + 4 * d);
-        }
-    } else {
-        for (int i = threadIdx.x; i < W * rows; i += blockDim.x) {
-            const int lx = i % W, ly = i / W;
-            tile[i] = read_mapped<int8_t>(a.cur, a.L, (int8_t)a.cv1, (int8_t)a.cv2, xs + lx, ys + ly);
-        }
-    }
+    fill_tile_i8(a, tile32, xs, ys, W, rows);
     __syncthreads();
     const int tx = 4 * (threadIdx.x % 16), ty = threadIdx.x / 16;
     const int x = x0 + tx, y = y0 + ty;

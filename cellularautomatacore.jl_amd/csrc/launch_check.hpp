@@ -433,6 +433,54 @@ inline void generic_tile(Chk& c, const Lat& L, const void* cur, int h, int64_t g
     if (c.ok() && lo <= hi) c.range(cur, lo, hi, "tile reads");
 }
 
+// One lattice of k_box_i8 (kernels_box.hpp) over a gx x gy grid of 64 x 64
+// blocks: the radius within the kernel's limits, the grid covering the lattice,
+// the LDS holding the tile, the prefix rows and the sums; the window reads of
+// the prefix rows (aligned dwords up to the one after the last byte used) inside
+// the LDS with every byte used inside its row; the sum and tile rows the last
+// pass reads; the tile's global reads (a dword at a time inside the lattice,
+// else through the boundary map) and the writes.
+inline void box_tile(Chk& c, const Lat& L, const void* cur, const void* next, bool vn, int r, int64_t gx,
+                     int64_t gy, size_t lds_bytes) {
+    c.need(r >= 1 && r <= (vn ? BOX_VN_MAX : BOX_MOORE_MAX), "radius %d", r);
+    if (!c.ok()) return;
+    const int hp = box_hp(r), W = box_w(r), PW = box_pw(r), rows = box_rows(r), planes = box_planes(vn, r);
+    c.need(gx * BOX_T >= L.n1 && gy * BOX_T >= L.n2, "grid %lld x %lld does not cover %d x %d", (long long)gx,
+           (long long)gy, L.n1, L.n2);
+    c.need(lds_bytes >= box_lds(vn, r) && W % 4 == 0 && PW == W + 4, "LDS of %zu bytes for radius %d", lds_bytes, r);
+    lds(c, lds_bytes);
+    c.need(L.pitch % 4 == 0, "pitch %lld", (long long)L.pitch);
+    const int64_t pre0 = (int64_t)rows * W, sum0 = pre0 + (int64_t)rows * PW;
+    for (int w = vn ? 0 : r; w <= r && c.ok(); ++w) {
+        // outputs x = 0..63 read prefix bytes 3 + hp + x - w and 4 + hp + x + w, four at a time (x = 4k)
+        c.need(3 + hp - w >= 0, "prefix read before its row (r = %d, w = %d)", r, w);
+        c.need(4 + hp + 60 + w + 3 <= PW - 1, "window past its prefix row (r = %d, w = %d)", r, w);
+        const int64_t end = pre0 + (int64_t)(rows - 1) * PW + ((4 + hp + 60 + w) / 4 + 2) * 4;
+        c.need(end <= sum0 + (int64_t)planes * rows * BOX_T && end <= (int64_t)lds_bytes,
+               "prefix read past the LDS (r = %d, w = %d)", r, w);
+    }
+    c.need(63 + 2 * r <= rows - 1 && 63 + r <= rows - 1 && hp + 63 <= W - 1, "rows of the last pass (r = %d)", r);
+    if (!c.ok()) return;
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int64_t by = 0; by < gy && c.ok(); ++by)
+        for (int64_t bx = 0; bx < gx && c.ok(); ++bx) {
+            const int32_t xs = (int32_t)(bx * BOX_T) - hp, ys = (int32_t)(by * BOX_T) - r;
+            int32_t c0, c1, r0, r1;
+            if (xs >= 0 && xs + W <= L.n1 && ys >= 0 && ys + rows <= L.n2) {
+                c0 = xs, c1 = xs + W - 1, r0 = ys, r1 = ys + rows - 1;
+            } else {
+                c.need(axis_span(L.b1, xs, xs + W - 1, L.n1, c0, c1), "tile columns from %d map off", xs);
+                c.need(axis_span(L.b2, ys, ys + rows - 1, L.n2, r0, r1), "tile rows from %d map off", ys);
+                if (c0 > c1 || r0 > r1) continue;
+            }
+            const int64_t a = (int64_t)r0 * L.pitch + c0, z = (int64_t)r1 * L.pitch + c1 + 1;
+            lo = a < lo ? a : lo;
+            hi = z > hi ? z : hi;
+        }
+    if (c.ok() && lo <= hi) c.range(cur, lo, hi, "tile reads");
+    lattice(c, next, L, 1, "state written");
+}
+
 // ---------------------------------------------------------------- k_lenia_mfma
 
 // One lattice of k_lenia_mfma<mode> over a gx x gy grid of 64 x 64 blocks:

@@ -48,6 +48,10 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+/* The largest neighbourhood a rule's neighbour values are gathered for: Moore(32),
+ * the widest the device kernels take (halo 32), has 65 * 65 - 1 cells. */
+#define REF_MAX_NB (65 * 65)
 #endif
 
 /* ------------------------------------------------------------------ Philox4x32-10 */
@@ -445,7 +449,7 @@ static void epi_inputs(const lat_t* L, const int8_t* u, int64_t x, int64_t y, in
 static void sweep_plain_i8(const lat_t* L, const int8_t* cur, int8_t* next, int64_t t,
                            int gate, const cah_params* p, const int8_t* fs) {
     const cah_desc* d = L->d;
-    int8_t nb[1024];
+    int8_t nb[REF_MAX_NB];
     for (int64_t y = 0; y < L->n2; ++y)
         for (int64_t x = 0; x < L->n1; ++x) {
             int64_t lin = x + y * L->n1;
@@ -481,6 +485,7 @@ static void sweep_plain_lenia(const lat_t* L, const float* cur, float* next, int
  * per gate; nsteps steps from t0. */
 int ref_advance_plain(const cah_desc* d, void* u, int64_t t0, int64_t nsteps,
                       const cah_params* p) {
+    if (d->n_offsets > REF_MAX_NB) return -1;
     lat_t L = lat_of(d);
     int64_t n = L.n1 * L.n2;
     size_t es = d->dtype == CAH_DTYPE_F32 ? 4 : 1;
@@ -530,7 +535,7 @@ int ref_update_cell(const cah_desc* d, const void* u, int64_t i1, int64_t i2, in
     int64_t lin = i1 + i2 * L.n1;
     int8_t c = cur[lin];
     if (d->mask && !d->mask[lin]) return c;
-    int8_t nb[1024];
+    int8_t nb[REF_MAX_NB];
     for (int32_t k = 0; k < d->n_offsets; ++k)
         nb[k] = read_i8(&L, cur, i1 + d->offsets[2 * k], i2 + d->offsets[2 * k + 1]);
     ctx_t cx = {d->seed, lin + 1, t, p, 0.0, 0.0f, 1.0};
@@ -591,7 +596,7 @@ static void sweep_slab_i8(const halo_t* H, const int8_t* cur, int8_t* next, int6
                           int gate, const cah_params* p, int64_t ylo, int64_t yhi) {
     const lat_t* L = &H->L;
     const cah_desc* d = L->d;
-    int8_t nb[1024];
+    int8_t nb[REF_MAX_NB];
     int32_t m = d->n_offsets;
     for (int64_t y = ylo; y < yhi; ++y)
         for (int64_t x = 0; x < L->n1; ++x) {
@@ -641,6 +646,7 @@ static void sweep_slab_lenia(const halo_t* H, const float* cur, float* next, int
 
 int ref_advance_halo(const cah_desc* d, void* u, int64_t t0, int64_t nsteps,
                      const cah_params* p, int32_t nthreads) {
+    if (d->n_offsets > REF_MAX_NB) return -1;
     halo_t H;
     H.L = lat_of(d);
     int32_t hn = max_abs_offset(d->offsets, d->n_offsets);

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <string>
 
+#include "kernels_box.hpp"
 #include "kernels_generic.hpp"
 #include "kernels_life_tb.hpp"
 #include "launch.hpp"
@@ -123,6 +124,34 @@ int main() {
         lchk::Chk i("general tile, halo past 32");
         lchk::generic_tile(i, L, cur, 33, 16, 19, (size_t)gen_w(33) * gen_rows(33));
         expect(i.kernel.c_str(), false, i);
+        // the window-sum kernel: 64 x 64 blocks, 16 x 5 of them over 1000 x 300
+        lchk::Chk j("box tile, Moore(8), right");
+        lchk::box_tile(j, L, cur, next, false, 8, 16, 5, box_lds(false, 8));
+        expect("box tile, Moore(8), right", true, j);
+        lchk::Chk k("box tile, Moore(32), right");
+        lchk::box_tile(k, L, cur, next, false, 32, 16, 5, box_lds(false, 32));
+        expect("box tile, Moore(32), right", true, k);
+        lchk::Chk l("box tile, VonNeumann(8), right");
+        lchk::box_tile(l, L, cur, next, true, 8, 16, 5, box_lds(true, 8));
+        expect("box tile, VonNeumann(8), right", true, l);
+        lchk::Chk m("box tile, VonNeumann(1), right");
+        lchk::box_tile(m, L, cur, next, true, 1, 16, 5, box_lds(true, 1));
+        expect("box tile, VonNeumann(1), right", true, m);
+        lchk::Chk n("box tile, Moore radius past 32");
+        lchk::box_tile(n, L, cur, next, false, 33, 16, 5, box_lds(false, 33));
+        expect(n.kernel.c_str(), false, n);
+        lchk::Chk o("box tile, VonNeumann radius past 8");
+        lchk::box_tile(o, L, cur, next, true, 9, 16, 5, box_lds(true, 9));
+        expect(o.kernel.c_str(), false, o);
+        lchk::Chk q("box tile, LDS sized for von Neumann's smaller planes");
+        lchk::box_tile(q, L, cur, next, true, 8, 16, 5, box_lds(true, 7));
+        expect(q.kernel.c_str(), false, q);
+        lchk::Chk u("box tile, grid a row of blocks short");
+        lchk::box_tile(u, L, cur, next, false, 8, 16, 4, box_lds(false, 8));
+        expect(u.kernel.c_str(), false, u);
+        lchk::Chk v("box tile, output a row short");
+        lchk::box_tile(v, L, cur, short_next, false, 8, 16, 5, box_lds(false, 8));
+        expect(v.kernel.c_str(), false, v);
     }
     // k_lenia_mfma: 2048^2 FP32, H = 12, rows -11 .. 11 (12 pairs): fragments of
     // the right size, then planted mistakes

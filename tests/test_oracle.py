@@ -222,6 +222,20 @@ def test_larger_than_life_brute_force(oracle):
         assert (got == brute_life_torus(u, range(34, 46), range(34, 59), 5)).all()
 
 
+def test_large_neighbourhoods_brute_force(oracle):
+    """Moore(8) and Moore(32) (288 and 4224 neighbours, the widest neighbourhood
+    the device kernels take) against the brute-force count on tori, one of them
+    narrower than the neighbourhood in one axis (read through a period)."""
+    for seed, dims, r, b, s in ((4, (40, 33), 8, range(20, 30), range(15, 35)),
+                                (5, (70, 66), 32, range(30, 50), range(25, 60)),
+                                (6, (90, 50), 32, range(30, 50), range(25, 60))):
+        u = rng_state(seed, dims, [0] * 40 + [1])
+        prob = CAProblem(M.Life(b, s, neighbourhood=M.Moore(r)), u, 1, space=M.Torus(*dims))
+        d, keep = desc_for(prob)
+        got = oracle.advance_plain(d, u, 0, 1)
+        assert (got == brute_life_torus(u, b, s, r)).all()
+
+
 def test_forest_fire_invariants(oracle):
     dims = (40, 30)
     u = rng_state(4, dims, [0, 1, 2])
