@@ -18,7 +18,7 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 OBJ = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libcahip.so")
 UNITS = ["cahip.hip", "forest.hip", "forest_batch.hip", "forest_gated.hip", "lenia.hip", "life_r1.hip",
-         "life_r2.hip", "life_r3.hip", "life_r4.hip", "life_r5.hip"]
+         "life_r2.hip", "life_r3.hip", "life_r4.hip", "life_r5.hip", "life_r6.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

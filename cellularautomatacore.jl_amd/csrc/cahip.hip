@@ -38,6 +38,10 @@
 
 // ---------------------------------------------------------------- Life dispatch
 
+// Dwords a lane of the Life band walk holds at least: a neighbour lane must hold
+// the R guard cells of each side (radius 5 and more take two).
+inline int life_min_vb(int R) { return R >= 5 ? 2 : 1; }
+
 inline void launch_life(int R, int vb, bool bin, int niv, int blocks, const LifeArgs& a,
                         hipStream_t st) {
     switch (R) {
@@ -45,7 +49,8 @@ inline void launch_life(int R, int vb, bool bin, int niv, int blocks, const Life
     case 2: launch_life_r<2>(vb, bin, niv, blocks, a, st); break;
     case 3: launch_life_r<3>(vb, bin, niv, blocks, a, st); break;
     case 4: launch_life_r<4>(vb, bin, niv, blocks, a, st); break;
-    default: launch_life_r<5>(vb, bin, niv, blocks, a, st); break;
+    case 5: launch_life_r<5>(vb, bin, niv, blocks, a, st); break;
+    default: launch_life_r<6>(vb, bin, niv, blocks, a, st); break;
     }
 }
 
@@ -55,7 +60,8 @@ inline void launch_life_gated(int R, int vb, bool bin, int niv, int blocks, cons
     case 2: launch_life_gated_r<2>(vb, bin, niv, blocks, a, st); break;
     case 3: launch_life_gated_r<3>(vb, bin, niv, blocks, a, st); break;
     case 4: launch_life_gated_r<4>(vb, bin, niv, blocks, a, st); break;
-    default: launch_life_gated_r<5>(vb, bin, niv, blocks, a, st); break;
+    case 5: launch_life_gated_r<5>(vb, bin, niv, blocks, a, st); break;
+    default: launch_life_gated_r<6>(vb, bin, niv, blocks, a, st); break;
     }
 }
 
@@ -65,7 +71,8 @@ inline int life_gated_blocks_per_cu(int R, int vb, bool bin, int niv) {
     case 2: return life_gated_blocks_per_cu_r<2>(vb, bin, niv);
     case 3: return life_gated_blocks_per_cu_r<3>(vb, bin, niv);
     case 4: return life_gated_blocks_per_cu_r<4>(vb, bin, niv);
-    default: return life_gated_blocks_per_cu_r<5>(vb, bin, niv);
+    case 5: return life_gated_blocks_per_cu_r<5>(vb, bin, niv);
+    default: return life_gated_blocks_per_cu_r<6>(vb, bin, niv);
     }
 }
 
@@ -77,7 +84,8 @@ inline void launch_life_batch(int R, int vb, bool bin, int niv, int blocks, cons
     case 2: launch_life_batch_r<2>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
     case 3: launch_life_batch_r<3>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
     case 4: launch_life_batch_r<4>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
-    default: launch_life_batch_r<5>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
+    case 5: launch_life_batch_r<5>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
+    default: launch_life_batch_r<6>(vb, bin, niv, blocks, it, wi, wb, nw, parity, st); break;
     }
 }
 
@@ -712,7 +720,7 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
         LifeArgs& la = s->life;
         int nb = life_intervals(d->birth, la.b_lo, la.b_hi);
         int ns = life_intervals(d->survive, la.s_lo, la.s_hi);
-        if (R >= 1 && R <= 5 && nb >= 0 && ns >= 0) {
+        if (R >= 1 && R <= 6 && nb >= 0 && ns >= 0) {
             la.nb = nb;
             la.ns = ns;
             if (d->rule == CAH_RULE_GENERATIONS) {
@@ -1139,7 +1147,7 @@ static int32_t band_rows(const cah_state* s, int64_t nbands, int R, int waves_cu
 static int life_vb(const cah_state* s) {
     const bool wide = s->n1 >= 2048 && !(s->path == P_LIFE_GATED && s->life_R == 1);
     int vb = s->tune_vb ? std::min(s->tune_vb, 2) : (wide ? 2 : 1);
-    if (s->life_R == 5 && vb < 2) vb = 2;
+    if (vb < life_min_vb(s->life_R)) vb = life_min_vb(s->life_R);
     return vb;
 }
 
@@ -2072,7 +2080,7 @@ static int batch_create(cah_ctx* ctx, int32_t n, const cah_desc* descs, cah_batc
     } else if (all_life) {
         b->kind = B_LIFE;
         b->life_R = R;
-        b->vb = R == 5 ? 2 : 1;
+        b->vb = life_min_vb(R);
         for (auto* s : b->members) b->niv = std::max(b->niv, life_niv(s->life));
     }
     if (b->kind != B_LOOP)
@@ -2398,10 +2406,10 @@ static int free_layout(cah_batch* b) {
 extern "C" int cah_batch_tune(cah_batch* b, int32_t vb, int32_t rows) {
     if (!b || vb < 0 || vb > 2 || rows < 0) return fail(CAH_ERR_ARGUMENT, "cah_batch_tune: bad arguments");
     if (b->kind == B_LOOP || b->tb) return CAH_OK;  // no band walk
-    if (b->kind == B_LIFE && b->life_R == 5 && vb == 1)
+    if (b->kind == B_LIFE && life_min_vb(b->life_R) == 2 && vb == 1)
         return fail(CAH_ERR_ARGUMENT, "radius-5 Life needs 8 cells a lane (vb = 2)");
     if (!b->dry) HIPCHK(hipSetDevice(b->device));
-    b->vb = vb ? vb : (b->kind == B_LIFE && b->life_R == 5 ? 2 : 1);
+    b->vb = vb ? vb : (b->kind == B_LIFE ? life_min_vb(b->life_R) : 1);
     b->rows = rows;
     if (int rc = free_layout(b)) return rc;
     return batch_layout(b);
@@ -2462,7 +2470,7 @@ extern "C" int cah_batch_autotune(cah_batch* b, int64_t t0, int32_t steps, const
     // cells a lane: 4 or 8 (radius 5 needs 8); rows a wave: 4 .. 64
     std::vector<std::pair<int, int32_t>> cands;
     for (int v : {1, 2})
-        if (!(b->kind == B_LIFE && b->life_R == 5 && v == 1))
+        if (!(b->kind == B_LIFE && life_min_vb(b->life_R) == 2 && v == 1))
             for (int32_t r : {4, 8, 16, 32, 64}) cands.push_back({v, r});
     for (auto [cand_vb, cand] : cands) {
         b->vb = cand_vb;
@@ -2992,7 +3000,7 @@ static int build_tick(cah_server* s) {
         std::vector<int32_t> map;
         int64_t work = 0;
         for (const SrvReq* r : g.reqs) {
-            const int64_t ow = 62 * 4 * (g.cls == SC_LIFE && r->st->life_R == 5 ? 2 : 1);
+            const int64_t ow = 62 * 4 * (g.cls == SC_LIFE ? life_min_vb(r->st->life_R) : 1);
             work += (r->st->n1 + ow - 1) / ow * r->st->n2;
         }
         g.rows = batch_rows(work);
@@ -3009,7 +3017,7 @@ static int build_tick(cah_server* s) {
             } else if (g.cls == SC_LENIA) {
                 n = (int32_t)(((st->n1 + LM_BX - 1) / LM_BX) * ((st->n2 + LM_BY - 1) / LM_BY));
             } else {
-                const int vb = g.cls == SC_LIFE && st->life_R == 5 ? 2 : 1;
+                const int vb = g.cls == SC_LIFE ? life_min_vb(st->life_R) : 1;
                 const int64_t ow = 62 * 4 * vb;
                 const int64_t rows = std::min<int64_t>(g.rows, st->n2);
                 n = (int32_t)(((st->n1 + ow - 1) / ow) * ((st->n2 + rows - 1) / rows));
@@ -3056,7 +3064,7 @@ static int build_tick(cah_server* s) {
                 std::memcpy(sl.host + g.off_items + k * sizeof a, &a, sizeof a);
             } else if (g.cls == SC_LIFE) {
                 LifeArgs a = st->life;
-                const int vb = st->life_R == 5 ? 2 : 1;
+                const int vb = life_min_vb(st->life_R);
                 a.cur = (const int8_t*)st->buf[st->cur];
                 a.next = (int8_t*)st->buf[1 - st->cur];
                 a.L = lat_of(st);
@@ -3148,7 +3156,7 @@ static int build_tick(cah_server* s) {
                             std::memcpy(&a, item + k * sizeof a, sizeof a);
                             a = swap(a);
                             const int R = g.key / 4;
-                            if (R == 5)
+                            if (life_min_vb(R) == 2)
                                 lchk::band_walk<2>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, R, 2 * R + 1);
                             else
                                 lchk::band_walk<1>(c, a.L, a.cur, a.next, a.nbands, a.rows, a.nwaves, a.nwaves, R, 2 * R + 1);
@@ -3193,7 +3201,7 @@ static int build_tick(cah_server* s) {
                                      s->st[SC_LIFE]);
             } else if (g.cls == SC_LIFE) {
                 const int R = g.key / 4, niv = (g.key / 2) % 2 + 1;
-                launch_life_batch(R, R == 5 ? 2 : 1, g.key % 2, niv, waves_to_blocks(nw),
+                launch_life_batch(R, life_min_vb(R), g.key % 2, niv, waves_to_blocks(nw),
                                   (const LifeArgs*)items, map, base, nw, k & 1, s->st[SC_LIFE]);
             } else if (g.cls == SC_FOREST) {
                 // the short byte tests once every lattice holds only 0, 1 and 2: from

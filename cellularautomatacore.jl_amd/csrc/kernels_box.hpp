@@ -1,6 +1,7 @@
-// kernels_box.hpp -- Life and Generations with a Moore(R) neighbourhood of
-// radius 6..32 or a VonNeumann(r) neighbourhood of radius 1..8: neighbour
-// counts as window sums (src/rules.jl:377-389, :446-456).
+// kernels_box.hpp -- Life and Generations with a Moore(R) neighbourhood the
+// band walks do not take (radius 7..32, or more count intervals than they test)
+// or a VonNeumann(r) neighbourhood of radius 1..8: neighbour counts as window
+// sums (src/rules.jl:377-389, :446-456).
 //
 //   k_box_i8<VN>  a 64 x 64 block of cells a launch block, from an LDS tile of
 //                 the state with the boundary conditions applied while it is
@@ -20,7 +21,7 @@
 #include "kernels_generic.hpp"
 
 constexpr int BOX_T = 64;                          // output block: BOX_T x BOX_T cells
-constexpr int BOX_MOORE_MIN = 6, BOX_MOORE_MAX = 32;  // Moore(R <= 5) takes the band walks
+constexpr int BOX_MOORE_MAX = 32;  // Moore(R <= 6) with at most two count intervals takes the band walks
 constexpr int BOX_VN_MAX = 8;
 
 // The LDS layout of a block, all row strides multiples of 4 bytes:

@@ -1,4 +1,4 @@
-// kernels_life.hpp -- Life and Generations with Moore(R <= 5).
+// kernels_life.hpp -- Life and Generations with Moore(R <= 6).
 //
 // Each lane owns 4*VB consecutive cells of a row and walks down a column band,
 // keeping the vertical (2R+1)-row running count in registers and forming the
@@ -258,6 +258,10 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         for (int i = 0; i < VB; ++i) {
             c1[i] = leave(s_c, i);
             n[i] = Wd[i] - c1[i];  // Moore leaves the centre out
+            // R = 6: counts up to (2R+1)^2 - 1 = 168 fit a byte, but the interval
+            // tests take bytes below 128; a count of 128 or more is set to 127,
+            // which no mask holds (counts 0..63)
+            if constexpr (R >= 6) n[i] = (n[i] | (((n[i] >> 7) & 0x01010101u) * 0x7Fu)) & 0x7F7F7F7Fu;
         }
         const int32_t yr = up ? mirror - y : y;  // the row written
         if constexpr (GATED) {

@@ -49,12 +49,20 @@ LIFE_CASES = [
     ("brians_brain", (640, 100), (C(0), C(1)), [0, 1], M.Stochastic(0.8), False, None),
     ("star_wars", (1024, 64), (P, P), [0, 1], M.Synchronous(), True, None),
     ("star_wars", (1024, 64), (R, P), [0, 1], M.Checkerboard(), False, (2, 9)),
+    # radius 6 (counts past 127 in the byte lanes); radius 7 takes the window-sum kernel
+    ("ltl6", (1100, 70), (P, R), [0, 0, 0, 1], M.Stochastic(0.6), True, None),
+    ("B45/S345/C5 Moore(6)", (600, 90), (C(1), P), [0] * 40 + [1, 2, 3, 4], M.Checkerboard(), False, None),
+    ("ltl7", (517, 61), (R, R), [0, 0, 0, 0, 1], M.Synchronous(), True, None),
 ]
 
 
 def life_rule(name):
     if name == "ltl":
         return ltl_rule()
+    if name == "ltl6":
+        return M.Life(range(30, 40), range(25, 45), neighbourhood=M.Moore(6))
+    if name == "ltl7":
+        return M.Life(range(40, 52), range(35, 60), neighbourhood=M.Moore(7))
     if name == "conway":
         return M.Life("conway")
     if " Moore(" in name:

@@ -3,7 +3,7 @@ bit at every save (src/backends.jl:444-455, src/schemes.jl:74-93,
 src/rules.jl:377-389, :446-456):
 
 - k_box_i8, Life and Generations with Moore(R <= 32) or VonNeumann(r <= 8)
-  where no band walk runs (Moore radii above 5, more than two count
+  where no band walk runs (Moore radii above 6, more than two count
   intervals, every von Neumann radius): counts as window sums of prefix
   counts;
 - k_generic_i8, everything else on Int8 lattices (1-D lattices, von Neumann
@@ -47,8 +47,10 @@ CASES = [
      M.Lattice((230, 130), (R, P)), [0, 1], M.Checkerboard(), False, BOX),
     ("vn8_reflecting", M.Life(range(20, 40), range(15, 45), neighbourhood=M.VonNeumann(8)),
      M.Lattice((333, 150), (R, R)), [0, 1], M.Synchronous(), True, BOX),
-    ("moore6_checkerboard", M.Life(range(25, 40), range(20, 45), neighbourhood=M.Moore(6)),
-     M.Lattice((250, 130), (P, R)), [0, 1], M.Checkerboard(), False, BOX),
+    ("moore9_checkerboard", M.Life(range(25, 40), range(20, 45), neighbourhood=M.Moore(9)),
+     M.Lattice((250, 130), (P, R)), SPARSE, M.Checkerboard(), False, BOX),
+    ("moore6_four_intervals", M.Life([30, 32, 34, 36], [25, 28, 31, 40], neighbourhood=M.Moore(6)),
+     M.Lattice((270, 110), (R, C(1))), [0, 0, 0, 0, 1], M.Synchronous(), False, BOX),
     ("moore8_masked", M.Life(range(20, 30), range(15, 35), neighbourhood=M.Moore(8)), M.Torus(200, 150), SPARSE,
      M.Synchronous(), True, BOX),
     ("moore8_small_torus", M.Life(range(20, 30), range(15, 35), neighbourhood=M.Moore(8)), M.Torus(45, 37), SPARSE,
@@ -65,7 +67,7 @@ CASES = [
      False, BOX),
     ("generations_vn2", M.Generations("B2/S13/C4", neighbourhood=M.VonNeumann(2)),
      M.Lattice((402, 75), (C(2), R)), [0, 1, 2, 3], M.Checkerboard(), True, BOX),
-    ("generations_moore7", M.Generations("B45/S345/C5", neighbourhood=M.Moore(7)),
+    ("generations_moore9", M.Generations("B45/S345/C5", neighbourhood=M.Moore(9)),
      M.Lattice((300, 140), (P, C(3))), [0] * 40 + [1, 2, 3, 4], M.Stochastic(0.7), True, BOX),
     ("one_dimensional", M.Life("B1/S1", neighbourhood=M.Moore(2)), M.Grid(5001, boundary=C(1)), [0, 1],
      M.Stochastic(0.7), False, GEN),

@@ -89,7 +89,7 @@ def test_rules_and_schemes(oracle, backend, case, si):
     assert_same(sol, oracle_saves(oracle, prob, SCHEMES[si], sol.t, halo=False))
 
 
-# ------------------------------------------------------------------ band-walk Life (Moore R <= 5)
+# ------------------------------------------------------------------ band-walk Life (Moore R <= 6)
 
 LIFE_SWAR = [
     (1, "B3/S23", (64, 48), (M.Periodic(), M.Periodic())),
@@ -101,20 +101,31 @@ LIFE_SWAR = [
     (5, None, (1100, 97), (M.Periodic(), M.Periodic())),
     (5, None, (517, 60), (M.Reflecting(), M.Constant(1))),
     (5, None, (7, 5), (M.Periodic(), M.Reflecting())),
+    # radius 6: counts past 127 (up to 168) in the byte lanes
+    (6, (range(30, 40), range(25, 45)), (1100, 97), (M.Periodic(), M.Periodic())),
+    (6, (range(30, 40), range(25, 45)), (517, 60), (M.Reflecting(), M.Constant(1))),
+    (6, (range(30, 40), range(25, 45)), (700, 130), (M.Periodic(), M.Reflecting())),
+    (6, (range(30, 40), range(25, 45)), (9, 6), (M.Periodic(), M.Periodic())),
 ]
 
 
-# every launch shape the band walk has for each case (radius 5 needs two dwords a lane)
+# every launch shape the band walk has for each case (radii 5 and more need two dwords a lane)
 @pytest.mark.parametrize("i,vb", [(i, vb) for i in range(len(LIFE_SWAR)) for vb in (0, 1, 2, 4)
-                                  if not (LIFE_SWAR[i][0] == 5 and vb == 1)])
+                                  if not (LIFE_SWAR[i][0] >= 5 and vb == 1)])
 def test_life_band_walk(oracle, i, vb):
     r, text, dims, bounds = LIFE_SWAR[i]
-    rule = ltl_rule() if text is None else M.Life(text, neighbourhood=M.Moore(r))
+    if text is None:
+        rule = ltl_rule()
+    elif isinstance(text, tuple):
+        rule = M.Life(text[0], text[1], neighbourhood=M.Moore(r))
+    else:
+        rule = M.Life(text, neighbourhood=M.Moore(r))
     space = M.Lattice(dims, bounds)
     u = rng_state(i, dims, [0, 1])
-    if r == 5:  # dense enough for births at 34..45
-        u = np.asfortranarray((np.random.default_rng(i).random(dims) < 0.35).astype(np.int8))
+    if r >= 5:  # dense enough for births at 34..45 (radius 5), sparse enough for counts below 64 (6)
+        u = np.asfortranarray((np.random.default_rng(i).random(dims) < (0.35 if r == 5 else 0.22)).astype(np.int8))
     prob = CAProblem(rule, u, 12, space=space, seed=1)
+    assert set(HIPBackend().check_launches(prob)) <= {"k_life_swar", "k_life_tb"}
     be = HIPBackend(tune=(vb, 8) if vb else None)
     sol = solve(prob, M.Synchronous(), backend=be, saveat=3)
     assert_same(sol, oracle_saves(oracle, prob, M.Synchronous(), sol.t, halo=False))
