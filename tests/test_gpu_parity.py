@@ -402,6 +402,24 @@ def test_batch_tiles(oracle, vb, rows):
             assert np.array_equal(out, oracle_final(oracle, p))
 
 
+def test_batch_larger_radii(oracle):
+    """A batch of Larger-than-Life lattices of radius 6 (the widest band walk,
+    counts saturated past 127) and of radius 5, each equal to its single run
+    and to the oracle."""
+    from cahip.problem import final_state
+    be = HIPBackend()
+    for r, b, s_, dens in ((6, range(30, 40), range(25, 45), 0.22), (5, range(34, 46), range(34, 59), 0.35)):
+        group = []
+        for i, dims in enumerate([(600, 80), (260, 130), (1100, 37)]):
+            u = np.asfortranarray((np.random.default_rng(40 + i).random(dims) < dens).astype(np.int8))
+            group.append(CAProblem(M.Life(b, s_, neighbourhood=M.Moore(r)), u, (0, 9),
+                                   space=M.Lattice(dims, (M.Periodic(), M.Reflecting())), seed=i))
+        outs = be.solve_batch(group, M.Synchronous())
+        for p, out in zip(group, outs):
+            assert np.array_equal(out, final_state(solve(p, backend=be, save_everystep=False)))
+            assert np.array_equal(out, oracle_final(oracle, p))
+
+
 def test_batch_forest_values_outside_the_rule(oracle):
     """A batch whose lattices hold values the forest fire never makes (3, -1, 5)
     runs the exact byte tests until a step has cleaned them, then the short
