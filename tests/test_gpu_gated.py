@@ -52,6 +52,7 @@ LIFE_CASES = [
     # radius 6 (counts past 127 in the byte lanes); radius 7 takes the window-sum kernel
     ("ltl6", (1100, 70), (P, R), [0, 0, 0, 1], M.Stochastic(0.6), True, None),
     ("B45/S345/C5 Moore(6)", (600, 90), (C(1), P), [0] * 40 + [1, 2, 3, 4], M.Checkerboard(), False, None),
+    ("B45/S345/C5 Moore(6)", (517, 61), (R, C(2)), [0] * 40 + [1, 2, 3, 4], M.Stochastic(0.5), True, None),
     ("ltl7", (517, 61), (R, R), [0, 0, 0, 0, 1], M.Synchronous(), True, None),
 ]
 
