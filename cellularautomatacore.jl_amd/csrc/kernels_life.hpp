@@ -142,6 +142,28 @@ CAH_HD void gen_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t 
     }
 }
 
+// gen_rule for C <= 3 on cells that are all 0, 1 or 2: bit 0 of such a byte is
+// its cell == 1 and a zero pair of low bits its cell == 0; a live cell that does
+// not survive becomes 2 (C = 3, gen_dying 2) or 0 (C = 2), and a dying cell (2)
+// becomes 0 (2 + 1 < C never holds).
+template <int VB, int NIV>
+CAH_HD void gen_rule_tri(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t (&c)[VB],
+                         uint32_t (&out)[VB]) {
+#pragma unroll
+    for (int i = 0; i < VB; ++i) {
+        uint32_t inb = 0, ins = 0;
+#pragma unroll
+        for (int k = 0; k < NIV; ++k) {
+            inb |= (n[i] + a.b_lo[k]) & ~(n[i] + a.b_hi[k]);
+            ins |= (n[i] + a.s_lo[k]) & ~(n[i] + a.s_hi[k]);
+        }
+        const uint32_t born = (inb & 0x80808080u) >> 7, keep = (ins & 0x80808080u) >> 7;
+        const uint32_t one = c[i] & 0x01010101u, zero = ~(c[i] | (c[i] >> 1)) & 0x01010101u;
+        const uint32_t dying = one & ~keep & (a.gen_dying >> 1);
+        out[i] = (zero & born) | (one & keep) | (dying << 1);
+    }
+}
+
 // The walk of one wave down rows [ys, ye) of its band (ys, ye wave-uniform), or
 // up them when `up`: the walk runs over rows y' = ys + ye - 1 - y (the Moore sums
 // are symmetric in the row offset).  FAST: the lattice suits load_mapped, so every
@@ -159,8 +181,10 @@ CAH_HD void gen_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t 
 // the mask or the gate keeps is written back unchanged.  The mask rows go
 // through a further ring (row r in slot (r - ys) mod 2P): the mask of row y is
 // loaded right after row y + R, the row whose wait precedes row y's output, so
-// it adds no wait of its own (loads complete in order).
-template <int R, int VB, bool BIN, int NIV, bool FAST, bool GATED>
+// it adds no wait of its own (loads complete in order).  TRI (gated walk only):
+// every value read is 0, 1 or 2 and the rule is Life or Generations with C <= 3,
+// so bit 0 of a byte is its cell == 1 and the rule takes gen_rule_tri.
+template <int R, int VB, bool BIN, int NIV, bool FAST, bool GATED, bool TRI = false>
 CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer, bool up) {
     constexpr int P = 2 * R + 1, S = 2 * P;
     constexpr int G = (R + 3) / 4;  // guard dwords needed on each side
@@ -184,11 +208,12 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
     };
     // a cell's contribution to the counts (the ring holds it converted unless
     // GATED); ONES: the gated walk's converted rows, in slots of their own
-    constexpr bool ONES = GATED && !BIN && R <= 2;
+    constexpr bool ONES = GATED && !BIN && !TRI && R <= 2;
     uint32_t ring[S][VB];
     uint32_t one[ONES ? S : 1][VB];
     auto live = [&](uint32_t v) -> uint32_t {
-        if constexpr (GATED && !BIN) return bytes_eq(v, 0x01010101u);
+        if constexpr (GATED && TRI) return v & 0x01010101u;
+        else if constexpr (GATED && !BIN) return bytes_eq(v, 0x01010101u);
         else return v;
     };
     // the counts of ring slot k: converted now (into its ONES slot), or again
@@ -265,8 +290,12 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         }
         const int32_t yr = up ? mirror - y : y;  // the row written
         if constexpr (GATED) {
-            if (a.gen_C) gen_rule<VB, NIV>(a, n, ring[s_c], out);
-            else life_rule<VB, NIV>(a, n, c1, out);
+            if (a.gen_C) {
+                if constexpr (TRI) gen_rule_tri<VB, NIV>(a, n, ring[s_c], out);
+                else gen_rule<VB, NIV>(a, n, ring[s_c], out);
+            } else {
+                life_rule<VB, NIV>(a, n, c1, out);
+            }
             if (band_gated(a.g)) {
                 uint32_t open[VB];
                 const uint64_t rowid = (uint64_t)yr * (uint64_t)a.L.n1 + (uint64_t)(int64_t)x + 1;
@@ -304,11 +333,13 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 // wave-uniform, so the row bookkeeping stays in scalar registers).  Odd chunks
 // walk up: the two chunks beside a chunk boundary then read the rows around it
 // at the same time, so the second read finds them in L2.
-template <int R, int VB, bool BIN, int NIV, bool GATED = false>
+template <int R, int VB, bool BIN, int NIV, bool GATED = false, bool TRI = false>
 CAH_DEV void life_band(const LifeArgs& a, int wid) {
     const BandPart b = band_part<VB>(a.nbands, a.rows, a.L, wid, threadIdx.x & 63);
-    if (mapped_ok<VB>(a.L, 2 * R + 1)) life_walk<R, VB, BIN, NIV, true, GATED>(a, b.x, b.ys, b.ye, b.writer, b.up);
-    else life_walk<R, VB, BIN, NIV, false, GATED>(a, b.x, b.ys, b.ye, b.writer, b.up);
+    if (mapped_ok<VB>(a.L, 2 * R + 1))
+        life_walk<R, VB, BIN, NIV, true, GATED, TRI>(a, b.x, b.ys, b.ye, b.writer, b.up);
+    else
+        life_walk<R, VB, BIN, NIV, false, GATED, TRI>(a, b.x, b.ys, b.ye, b.writer, b.up);
 }
 
 template <int R, int VB, bool BIN, int NIV>
@@ -319,12 +350,13 @@ __global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
 }
 
 // Life or Generations (a.gen_C) under the gate and mask of a.g, one sweep;
-// NIV intervals per count mask; BIN: every cell read is 0 or 1.
-template <int R, int VB, int NIV, bool BIN>
+// NIV intervals per count mask; BIN: every cell read is 0 or 1; TRI: every value
+// read is 0, 1 or 2 and a Generations rule has at most 3 states.
+template <int R, int VB, int NIV, bool BIN, bool TRI = false>
 __global__ __launch_bounds__(256) void k_life_gated(LifeArgs a) {
     const int wid = wave_id_xcd();
     if (wid >= a.nwaves) return;
-    life_band<R, VB, BIN, NIV, true>(a, wid);
+    life_band<R, VB, BIN, NIV, true, TRI>(a, wid);
 }
 
 // ---------------------------------------------------------------- batched ensembles
