@@ -54,25 +54,26 @@ inline void launch_life(int R, int vb, bool bin, int niv, int blocks, const Life
     }
 }
 
-inline void launch_life_gated(int R, int vb, int cells, int niv, int blocks, const LifeArgs& a, hipStream_t st) {
+inline void launch_life_gated(int R, int vb, int cells, int niv, bool masked, int blocks, const LifeArgs& a,
+                              hipStream_t st) {
     switch (R) {
-    case 1: launch_life_gated_r<1>(vb, cells, niv, blocks, a, st); break;
-    case 2: launch_life_gated_r<2>(vb, cells, niv, blocks, a, st); break;
-    case 3: launch_life_gated_r<3>(vb, cells, niv, blocks, a, st); break;
-    case 4: launch_life_gated_r<4>(vb, cells, niv, blocks, a, st); break;
-    case 5: launch_life_gated_r<5>(vb, cells, niv, blocks, a, st); break;
-    default: launch_life_gated_r<6>(vb, cells, niv, blocks, a, st); break;
+    case 1: launch_life_gated_r<1>(vb, cells, niv, masked, blocks, a, st); break;
+    case 2: launch_life_gated_r<2>(vb, cells, niv, masked, blocks, a, st); break;
+    case 3: launch_life_gated_r<3>(vb, cells, niv, masked, blocks, a, st); break;
+    case 4: launch_life_gated_r<4>(vb, cells, niv, masked, blocks, a, st); break;
+    case 5: launch_life_gated_r<5>(vb, cells, niv, masked, blocks, a, st); break;
+    default: launch_life_gated_r<6>(vb, cells, niv, masked, blocks, a, st); break;
     }
 }
 
-inline int life_gated_blocks_per_cu(int R, int vb, int cells, int niv) {
+inline int life_gated_blocks_per_cu(int R, int vb, int cells, int niv, bool masked) {
     switch (R) {
-    case 1: return life_gated_blocks_per_cu_r<1>(vb, cells, niv);
-    case 2: return life_gated_blocks_per_cu_r<2>(vb, cells, niv);
-    case 3: return life_gated_blocks_per_cu_r<3>(vb, cells, niv);
-    case 4: return life_gated_blocks_per_cu_r<4>(vb, cells, niv);
-    case 5: return life_gated_blocks_per_cu_r<5>(vb, cells, niv);
-    default: return life_gated_blocks_per_cu_r<6>(vb, cells, niv);
+    case 1: return life_gated_blocks_per_cu_r<1>(vb, cells, niv, masked);
+    case 2: return life_gated_blocks_per_cu_r<2>(vb, cells, niv, masked);
+    case 3: return life_gated_blocks_per_cu_r<3>(vb, cells, niv, masked);
+    case 4: return life_gated_blocks_per_cu_r<4>(vb, cells, niv, masked);
+    case 5: return life_gated_blocks_per_cu_r<5>(vb, cells, niv, masked);
+    default: return life_gated_blocks_per_cu_r<6>(vb, cells, niv, masked);
     }
 }
 
@@ -350,7 +351,8 @@ struct cah_state {
     bool tri_clean = false;  // every cell is 0, 1 or 2 (the forest fire's k_forest_vn1 variant)
     LifeArgs life{};
     int32_t tune_vb = 0, tune_rows = 0;
-    int gated_occ[3] = {-1, -1, -1};  // k_life_gated blocks a CU holds (by cell kind), -1: not asked yet
+    // k_life_gated blocks a CU holds (by cell kind, and with a mask), -1: not asked yet
+    int gated_occ[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
     struct Arena* arena = nullptr;        // device memory source (nullptr: hipMalloc)
     struct Arena* staging = nullptr;      // pinned host memory of a server's uploads
     bool own_stream = true;               // false: the stream belongs to a server
@@ -1212,8 +1214,10 @@ static int step_life(cah_state* s, int64_t t, int gate) {
     };
     const bool writes012 = a.gen_C <= 3;  // Life (0) or Generations with C <= 3
     const int cells = s->bin_clean ? GATED_BIN
-                      : (gated && writes012 && s->tri_clean && edge012(0) && edge012(1)) ? GATED_TRI
-                                                                                        : GATED_ANY;
+                      : (gated && writes012 && s->life_R <= 2 && s->tri_clean && edge012(0) && edge012(1))
+                          ? GATED_TRI
+                          : GATED_ANY;
+    const bool masked = gated && a.g.mask != nullptr;
     const int vb = life_vb(s);
     const int64_t ow = 62 * 4 * vb;
     a.nbands = (int32_t)((s->n1 + ow - 1) / ow);
@@ -1221,8 +1225,8 @@ static int step_life(cah_state* s, int64_t t, int gate) {
     // chunks are sized so that every wave is resident at once all the same
     int waves_cu = 16;
     if (gated && !s->dry) {
-        int& occ = s->gated_occ[cells];
-        if (occ < 0) occ = life_gated_blocks_per_cu(s->life_R, vb, cells, life_niv(a));
+        int& occ = s->gated_occ[cells][masked ? 1 : 0];
+        if (occ < 0) occ = life_gated_blocks_per_cu(s->life_R, vb, cells, life_niv(a), masked);
         if (occ > 0) waves_cu = std::min(16, 4 * occ);
     }
     a.rows = band_rows(s, a.nbands, s->life_R, waves_cu);
@@ -1242,7 +1246,7 @@ static int step_life(cah_state* s, int64_t t, int gate) {
         }
     });
     if (!rc && !s->dry) {
-        if (gated) launch_life_gated(R, vb, cells, life_niv(a), blocks, a, s->stream);
+        if (gated) launch_life_gated(R, vb, cells, life_niv(a), masked, blocks, a, s->stream);
         else launch_life(R, vb, s->bin_clean, life_niv(a), blocks, a, s->stream);
     }
     // Life keeps a lattice of 0 and 1 so (the gate and mask keep cells as they

@@ -183,8 +183,11 @@ CAH_HD void gen_rule_tri(const LifeArgs& a, const uint32_t (&n)[VB], const uint3
 // loaded right after row y + R, the row whose wait precedes row y's output, so
 // it adds no wait of its own (loads complete in order).  TRI (gated walk only):
 // every value read is 0, 1 or 2 and the rule is Life or Generations with C <= 3,
-// so bit 0 of a byte is its cell == 1 and the rule takes gen_rule_tri.
-template <int R, int VB, bool BIN, int NIV, bool FAST, bool GATED, bool TRI = false>
+// so bit 0 of a byte is its cell == 1 and the rule takes gen_rule_tri.  MASK
+// (gated walk): the walk may have a mask; without, it keeps no mask ring (2P
+// registers a lane's dword fewer: the radius-5 walk then holds three waves a
+// SIMD instead of two).
+template <int R, int VB, bool BIN, int NIV, bool FAST, bool GATED, bool TRI = false, bool MASK = true>
 CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer, bool up) {
     constexpr int P = 2 * R + 1, S = 2 * P;
     constexpr int G = (R + 3) / 4;  // guard dwords needed on each side
@@ -226,11 +229,12 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         else return live(ring[k][i]);
     };
     // the mask (GATED): its rows [ys, ye) only, the clamp repeats the chunk's last
-    const bool masked = GATED && a.g.mask != nullptr;
+    constexpr bool MR = GATED && MASK;  // a mask ring
+    const bool masked = MR && a.g.mask != nullptr;
     const __amdgpu_buffer_rsrc_t rmask = row_rsrc(reinterpret_cast<const int8_t*>(a.g.mask));
-    uint32_t mring[GATED ? S : 1][VB];
+    uint32_t mring[MR ? S : 1][VB];
     auto mload = [&](int32_t r, uint32_t (&X)[VB]) {
-        if constexpr (GATED) {
+        if constexpr (MR) {
             if (masked) mask_row<VB, FAST>(a.g.mask, rmask, a.L, lm.src, x, up ? mirror - r : r, X);
         }
     };
@@ -248,7 +252,7 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
     Unroll<2 * R, S>::run([&](auto K) {
         constexpr int k = decltype(K)::value;
         load(min(ys + k - R, last), ring[k]);
-        mload(min(ys + k - 2 * R, ye - 1), mring[GATED ? k - 2 * R : 0]);
+        mload(min(ys + k - 2 * R, ye - 1), mring[MR ? k - 2 * R : 0]);
     });
     uint32_t V[VB];
 #pragma unroll
@@ -299,8 +303,8 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
             if (band_gated(a.g)) {
                 uint32_t open[VB];
                 const uint64_t rowid = (uint64_t)yr * (uint64_t)a.L.n1 + (uint64_t)(int64_t)x + 1;
-                if (small) band_open<VB, true>(a.g, mring[ph % S], rowid, yr, a.seed, a.t, sp53, sp24, open);
-                else band_open<VB, false>(a.g, mring[ph % S], rowid, yr, a.seed, a.t, sp53, sp24, open);
+                if (small) band_open<VB, true>(a.g, mring[MR ? ph % S : 0], rowid, yr, a.seed, a.t, sp53, sp24, open);
+                else band_open<VB, false>(a.g, mring[MR ? ph % S : 0], rowid, yr, a.seed, a.t, sp53, sp24, open);
 #pragma unroll
                 for (int i = 0; i < VB; ++i) out[i] = open_select(open[i], out[i], ring[s_c][i]);
             }
@@ -318,7 +322,7 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         // last row), and the mask ring the mask of row y + P + 1
         if (more) {
             load(min(y + R + P + 1, last), ring[s_old]);
-            mload(min(y + P + 1, ye - 1), mring[GATED ? (ph + P + 1) % S : 0]);
+            mload(min(y + P + 1, ye - 1), mring[MR ? (ph + P + 1) % S : 0]);
         }
         ++y;
     };
@@ -333,13 +337,13 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 // wave-uniform, so the row bookkeeping stays in scalar registers).  Odd chunks
 // walk up: the two chunks beside a chunk boundary then read the rows around it
 // at the same time, so the second read finds them in L2.
-template <int R, int VB, bool BIN, int NIV, bool GATED = false, bool TRI = false>
+template <int R, int VB, bool BIN, int NIV, bool GATED = false, bool TRI = false, bool MASK = true>
 CAH_DEV void life_band(const LifeArgs& a, int wid) {
     const BandPart b = band_part<VB>(a.nbands, a.rows, a.L, wid, threadIdx.x & 63);
     if (mapped_ok<VB>(a.L, 2 * R + 1))
-        life_walk<R, VB, BIN, NIV, true, GATED, TRI>(a, b.x, b.ys, b.ye, b.writer, b.up);
+        life_walk<R, VB, BIN, NIV, true, GATED, TRI, MASK>(a, b.x, b.ys, b.ye, b.writer, b.up);
     else
-        life_walk<R, VB, BIN, NIV, false, GATED, TRI>(a, b.x, b.ys, b.ye, b.writer, b.up);
+        life_walk<R, VB, BIN, NIV, false, GATED, TRI, MASK>(a, b.x, b.ys, b.ye, b.writer, b.up);
 }
 
 template <int R, int VB, bool BIN, int NIV>
@@ -351,12 +355,13 @@ __global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
 
 // Life or Generations (a.gen_C) under the gate and mask of a.g, one sweep;
 // NIV intervals per count mask; BIN: every cell read is 0 or 1; TRI: every value
-// read is 0, 1 or 2 and a Generations rule has at most 3 states.
-template <int R, int VB, int NIV, bool BIN, bool TRI = false>
+// read is 0, 1 or 2 and a Generations rule has at most 3 states; MASK: a.g.mask
+// may be set (false: it is null).
+template <int R, int VB, int NIV, bool BIN, bool TRI = false, bool MASK = true>
 __global__ __launch_bounds__(256) void k_life_gated(LifeArgs a) {
     const int wid = wave_id_xcd();
     if (wid >= a.nwaves) return;
-    life_band<R, VB, BIN, NIV, true, TRI>(a, wid);
+    life_band<R, VB, BIN, NIV, true, TRI, MASK>(a, wid);
 }
 
 // ---------------------------------------------------------------- batched ensembles

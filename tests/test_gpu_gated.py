@@ -63,6 +63,10 @@ LIFE_CASES = [
     ("brians_brain", (512, 64), (C(3), P), [0, 1], M.Stochastic(0.5), True, None),
     ("B3/S23/C2 Moore(2)", (768, 90), (C(1), P), [0, 1], M.Stochastic(0.5), True, None),
     ("B2/S/C3 Moore(3)", (1024, 70), (P, P), [0, 1, 2], M.Stochastic(F32(0.4)), True, None),
+    # radii 3 and more without a mask take the walk that keeps no mask ring
+    ("B36/S23 Moore(3)", (1030, 50), (P, C(0)), [0, 1], M.Checkerboard(), False, None),
+    ("B34/S3456/C4 Moore(4)", (700, 64), (R, P), [0, 1, 2, 3], M.Stochastic(0.5), False, None),
+    ("B45/S345/C5 Moore(5)", (512, 300), (C(1), R), [0] * 40 + [1, 2, 3, 4], M.Synchronous(), False, None),
 ]
 
 
