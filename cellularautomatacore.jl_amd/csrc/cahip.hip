@@ -349,6 +349,7 @@ struct cah_state {
     bool box_vn = false;     // P_BOX: von Neumann (else Moore) of radius halo
     bool bin_clean = false;  // every cell is 0 or 1 (Life only)
     bool tri_clean = false;  // every cell is 0, 1 or 2 (the forest fire's k_forest_vn1 variant)
+    bool small_clean = false;  // every cell is 0, 1, 2 or 3 (k_life_gated's small-cell form)
     LifeArgs life{};
     int32_t tune_vb = 0, tune_rows = 0;
     // k_life_gated blocks a CU holds (by cell kind, and with a mask), -1: not asked yet
@@ -747,7 +748,7 @@ static int create_state(cah_ctx* ctx, const cah_desc* d, cah_state* s, Arena* ar
     } else if (d->ndims == 2 && d->rule == CAH_RULE_FOREST_FIRE && is_vn1(s->offsets)) {
         s->path = plain ? P_FOREST : P_FOREST_GATED;
     }
-    s->bin_clean = s->tri_clean = true;  // the buffers start all zero
+    s->bin_clean = s->tri_clean = s->small_clean = true;  // the buffers start all zero
     return CAH_OK;
 }
 
@@ -797,17 +798,16 @@ static bool scan_binary(const int8_t* h, size_t n) {
     return true;
 }
 
-// Whether every byte is 0, 1 or 2, eight bytes a step (a byte above 2 as
-// unsigned has a bit set above bit 1, or both low bits).
-static bool scan_tri(const int8_t* h, size_t n) {
+// Whether every byte is 0, 1, 2 or 3, eight bytes a step.
+static bool scan_small(const int8_t* h, size_t n) {
     size_t i = 0;
     for (; i + 8 <= n; i += 8) {
         uint64_t w;
         std::memcpy(&w, h + i, 8);
-        if ((w & 0xFCFCFCFCFCFCFCFCull) || (w & (w >> 1) & 0x0101010101010101ull)) return false;
+        if (w & 0xFCFCFCFCFCFCFCFCull) return false;
     }
     for (; i < n; ++i)
-        if ((uint8_t)h[i] > 2) return false;
+        if ((uint8_t)h[i] > 3) return false;
     return true;
 }
 
@@ -923,10 +923,10 @@ extern "C" int cah_state_set(cah_state* s, const void* host, size_t nbytes) {
     // (a scan for them here would cost a full pass over the host copy); the
     // gated walk of Generations with at most 3 states, which keeps such a state
     // on its exact path, has it scanned
-    if (s->es == 1) s->tri_clean = s->bin_clean;
+    if (s->es == 1) s->tri_clean = s->small_clean = s->bin_clean;
     if (s->es == 1 && !s->bin_clean && s->path == P_LIFE_GATED && s->d.rule == CAH_RULE_GENERATIONS &&
-        s->d.gen_states <= 3)
-        s->tri_clean = scan_tri((const int8_t*)host, nbytes);
+        s->d.gen_states <= 4)
+        s->small_clean = scan_small((const int8_t*)host, nbytes);
     if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
@@ -950,7 +950,7 @@ extern "C" int cah_state_set_device(cah_state* s, const void* dptr, size_t nbyte
     lchk::unreg(dptr);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (s->es == 1) s->bin_clean = s->tri_clean = false;  // not scanned
+    if (s->es == 1) s->bin_clean = s->tri_clean = s->small_clean = false;  // not scanned
     if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
@@ -1104,6 +1104,7 @@ extern "C" int cah_state_write_cell(cah_state* s, int64_t i1, int64_t i2, const 
     HIPCHK(hipStreamSynchronize(s->stream));
     if (s->es == 1 && *(const uint8_t*)value > 1) s->bin_clean = false;
     if (s->es == 1 && *(const uint8_t*)value > 2) s->tri_clean = false;
+    if (s->es == 1 && *(const uint8_t*)value > 3) s->small_clean = false;
     return CAH_OK;
 }
 
@@ -1208,14 +1209,14 @@ static int step_life(cah_state* s, int64_t t, int gate) {
     s->bin_clean = saved && edges01;
     // the gated walk's cell kinds: 0 and 1 only, or 0, 1 and 2 only (edges
     // included) under Life or a Generations rule of at most 3 states
-    auto edge012 = [&](int k) {
+    auto edge_small = [&](int k) {
         const double v = s->d.boundary_value[k];
-        return s->d.boundary[k] != CAH_BOUNDARY_CONSTANT || v == 0 || v == 1 || v == 2;
+        return s->d.boundary[k] != CAH_BOUNDARY_CONSTANT || v == 0 || v == 1 || v == 2 || v == 3;
     };
-    const bool writes012 = a.gen_C <= 3;  // Life (0) or Generations with C <= 3
+    const bool writes_small = a.gen_C <= 4;  // Life (0) or Generations with C <= 4
     const int cells = s->bin_clean ? GATED_BIN
-                      : (gated && writes012 && s->life_R <= 2 && s->tri_clean && edge012(0) && edge012(1))
-                          ? GATED_TRI
+                      : (gated && writes_small && s->life_R <= 2 && s->small_clean && edge_small(0) && edge_small(1))
+                          ? GATED_SMALL
                           : GATED_ANY;
     const bool masked = gated && a.g.mask != nullptr;
     const int vb = life_vb(s);
@@ -1251,12 +1252,12 @@ static int step_life(cah_state* s, int64_t t, int gate) {
     }
     // Life keeps a lattice of 0 and 1 so (the gate and mask keep cells as they
     // are); Generations writes 2 and more.  Life writes 0 or 1 from any value, so
-    // a sweep over every cell leaves 0, 1 and 2 only; Generations with at most 3
-    // states keeps a lattice of 0, 1 and 2 so, but moves a negative cell up by
-    // one, and with more states may write 3 and more.
+    // a sweep over every cell leaves cells below 4 only; Generations with at most
+    // 4 states keeps a lattice of cells below 4 so, but moves a negative cell up
+    // by one, and with more states may write 4 and more.
     s->bin_clean = saved && s->d.rule == CAH_RULE_LIFE;
-    if (!writes012) s->tri_clean = false;
-    else if (a.gen_C == 0 && (!gated || !band_gated(a.g))) s->tri_clean = true;
+    if (!writes_small) s->small_clean = false;
+    else if (a.gen_C == 0 && (!gated || !band_gated(a.g))) s->small_clean = true;
     return rc;
 }
 
@@ -1913,7 +1914,7 @@ extern "C" int cah_state_fill_bernoulli(cah_state* s, uint64_t seed, double p) {
     k_fill_bernoulli<<<grid, 256, 0, s->stream>>>((int8_t*)s->buf[s->cur], lat_of(s), seed, p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
-    s->bin_clean = s->tri_clean = true;
+    s->bin_clean = s->tri_clean = s->small_clean = true;
     if (s->sparse) return sparse_refresh(s);
     return CAH_OK;
 }
@@ -2905,7 +2906,7 @@ static int admit(cah_server* s, SrvReq* r) {
                                    st->n2, s->up))
                 return rc;
         if (st->es == 1) st->bin_clean = pinned ? r->u0_binary : scan_binary((const int8_t*)h, n);
-        if (st->es == 1) st->tri_clean = st->bin_clean;
+        if (st->es == 1) st->tri_clean = st->small_clean = st->bin_clean;
         std::vector<char>().swap(r->u0);
     }
     switch (st->path) {

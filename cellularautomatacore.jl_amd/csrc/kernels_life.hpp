@@ -142,13 +142,15 @@ CAH_HD void gen_rule(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t 
     }
 }
 
-// gen_rule for C <= 3 on cells that are all 0, 1 or 2: bit 0 of such a byte is
-// its cell == 1 and a zero pair of low bits its cell == 0; a live cell that does
-// not survive becomes 2 (C = 3, gen_dying 2) or 0 (C = 2), and a dying cell (2)
-// becomes 0 (2 + 1 < C never holds).
+// gen_rule for C <= 4 on cells that are all 0, 1, 2 or 3 (bits b1 b0 of each
+// byte): a dead cell (0) is born by the birth intervals, a live one (1) survives
+// by the survive intervals or starts dying (2 when C > 2, else 0), a dying cell
+// s (2 or 3) moves to s + 1 while s + 1 < C (only 2 -> 3 with C = 4) and to 0
+// after.
 template <int VB, int NIV>
-CAH_HD void gen_rule_tri(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t (&c)[VB],
-                         uint32_t (&out)[VB]) {
+CAH_HD void gen_rule_small(const LifeArgs& a, const uint32_t (&n)[VB], const uint32_t (&c)[VB],
+                           uint32_t (&out)[VB]) {
+    const uint32_t c4 = a.gen_C == 4 ? 0x01010101u : 0u;  // 2 -> 3
 #pragma unroll
     for (int i = 0; i < VB; ++i) {
         uint32_t inb = 0, ins = 0;
@@ -158,9 +160,10 @@ CAH_HD void gen_rule_tri(const LifeArgs& a, const uint32_t (&n)[VB], const uint3
             ins |= (n[i] + a.s_lo[k]) & ~(n[i] + a.s_hi[k]);
         }
         const uint32_t born = (inb & 0x80808080u) >> 7, keep = (ins & 0x80808080u) >> 7;
-        const uint32_t one = c[i] & 0x01010101u, zero = ~(c[i] | (c[i] >> 1)) & 0x01010101u;
+        const uint32_t b0 = c[i] & 0x01010101u, b1 = (c[i] >> 1) & 0x01010101u;
+        const uint32_t one = b0 & ~b1, zero = ~(b0 | b1) & 0x01010101u, to3 = ~b0 & b1 & c4;
         const uint32_t dying = one & ~keep & (a.gen_dying >> 1);
-        out[i] = (zero & born) | (one & keep) | (dying << 1);
+        out[i] = (zero & born) | (one & keep) | (dying << 1) | to3 | (to3 << 1);
     }
 }
 
@@ -181,13 +184,14 @@ CAH_HD void gen_rule_tri(const LifeArgs& a, const uint32_t (&n)[VB], const uint3
 // the mask or the gate keeps is written back unchanged.  The mask rows go
 // through a further ring (row r in slot (r - ys) mod 2P): the mask of row y is
 // loaded right after row y + R, the row whose wait precedes row y's output, so
-// it adds no wait of its own (loads complete in order).  TRI (gated walk only):
-// every value read is 0, 1 or 2 and the rule is Life or Generations with C <= 3,
-// so bit 0 of a byte is its cell == 1 and the rule takes gen_rule_tri.  MASK
+// it adds no wait of its own (loads complete in order).  SMALL (gated walk only):
+// every value read is 0, 1, 2 or 3 and the rule is Life or Generations with
+// C <= 4, so a cell == 1 is bit 0 without bit 1 and the rule takes
+// gen_rule_small.  MASK
 // (gated walk): the walk may have a mask; without, it keeps no mask ring (2P
 // registers a lane's dword fewer: the radius-5 walk then holds three waves a
 // SIMD instead of two).
-template <int R, int VB, bool BIN, int NIV, bool FAST, bool GATED, bool TRI = false, bool MASK = true>
+template <int R, int VB, bool BIN, int NIV, bool FAST, bool GATED, bool SMALL = false, bool MASK = true>
 CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, bool writer, bool up) {
     constexpr int P = 2 * R + 1, S = 2 * P;
     constexpr int G = (R + 3) / 4;  // guard dwords needed on each side
@@ -211,11 +215,11 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
     };
     // a cell's contribution to the counts (the ring holds it converted unless
     // GATED); ONES: the gated walk's converted rows, in slots of their own
-    constexpr bool ONES = GATED && !BIN && !TRI && R <= 2;
+    constexpr bool ONES = GATED && !BIN && !SMALL && R <= 2;
     uint32_t ring[S][VB];
     uint32_t one[ONES ? S : 1][VB];
     auto live = [&](uint32_t v) -> uint32_t {
-        if constexpr (GATED && TRI) return v & 0x01010101u;
+        if constexpr (GATED && SMALL) return v & ~(v >> 1) & 0x01010101u;
         else if constexpr (GATED && !BIN) return bytes_eq(v, 0x01010101u);
         else return v;
     };
@@ -295,7 +299,7 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
         const int32_t yr = up ? mirror - y : y;  // the row written
         if constexpr (GATED) {
             if (a.gen_C) {
-                if constexpr (TRI) gen_rule_tri<VB, NIV>(a, n, ring[s_c], out);
+                if constexpr (SMALL) gen_rule_small<VB, NIV>(a, n, ring[s_c], out);
                 else gen_rule<VB, NIV>(a, n, ring[s_c], out);
             } else {
                 life_rule<VB, NIV>(a, n, c1, out);
@@ -337,13 +341,13 @@ CAH_DEV void life_walk(const LifeArgs& a, int32_t x, int32_t ys, int32_t ye, boo
 // wave-uniform, so the row bookkeeping stays in scalar registers).  Odd chunks
 // walk up: the two chunks beside a chunk boundary then read the rows around it
 // at the same time, so the second read finds them in L2.
-template <int R, int VB, bool BIN, int NIV, bool GATED = false, bool TRI = false, bool MASK = true>
+template <int R, int VB, bool BIN, int NIV, bool GATED = false, bool SMALL = false, bool MASK = true>
 CAH_DEV void life_band(const LifeArgs& a, int wid) {
     const BandPart b = band_part<VB>(a.nbands, a.rows, a.L, wid, threadIdx.x & 63);
     if (mapped_ok<VB>(a.L, 2 * R + 1))
-        life_walk<R, VB, BIN, NIV, true, GATED, TRI, MASK>(a, b.x, b.ys, b.ye, b.writer, b.up);
+        life_walk<R, VB, BIN, NIV, true, GATED, SMALL, MASK>(a, b.x, b.ys, b.ye, b.writer, b.up);
     else
-        life_walk<R, VB, BIN, NIV, false, GATED, TRI, MASK>(a, b.x, b.ys, b.ye, b.writer, b.up);
+        life_walk<R, VB, BIN, NIV, false, GATED, SMALL, MASK>(a, b.x, b.ys, b.ye, b.writer, b.up);
 }
 
 template <int R, int VB, bool BIN, int NIV>
@@ -354,14 +358,14 @@ __global__ __launch_bounds__(256) void k_life_swar(LifeArgs a) {
 }
 
 // Life or Generations (a.gen_C) under the gate and mask of a.g, one sweep;
-// NIV intervals per count mask; BIN: every cell read is 0 or 1; TRI: every value
-// read is 0, 1 or 2 and a Generations rule has at most 3 states; MASK: a.g.mask
-// may be set (false: it is null).
-template <int R, int VB, int NIV, bool BIN, bool TRI = false, bool MASK = true>
+// NIV intervals per count mask; BIN: every cell read is 0 or 1; SMALL: every
+// value read is 0, 1, 2 or 3 and a Generations rule has at most 4 states; MASK:
+// a.g.mask may be set (false: it is null).
+template <int R, int VB, int NIV, bool BIN, bool SMALL = false, bool MASK = true>
 __global__ __launch_bounds__(256) void k_life_gated(LifeArgs a) {
     const int wid = wave_id_xcd();
     if (wid >= a.nwaves) return;
-    life_band<R, VB, BIN, NIV, true, TRI, MASK>(a, wid);
+    life_band<R, VB, BIN, NIV, true, SMALL, MASK>(a, wid);
 }
 
 // ---------------------------------------------------------------- batched ensembles

@@ -14,10 +14,10 @@
 template <int R>
 void launch_life_r(int vb, bool bin, int niv, int blocks, const LifeArgs& a, hipStream_t st);
 // Life or Generations under a gate and/or a mask (k_life_gated): vb in {1, 2};
-// cells: GATED_ANY, GATED_BIN (every value read is 0 or 1) or GATED_TRI (every
+// cells: GATED_ANY, GATED_BIN (every value read is 0 or 1) or GATED_SMALL (every
 // value read is 0, 1 or 2, and a Generations rule has at most 3 states);
 // niv = intervals per mask (1 or 2); masked = a.g.mask is set.
-enum { GATED_ANY = 0, GATED_BIN = 1, GATED_TRI = 2 };
+enum { GATED_ANY = 0, GATED_BIN = 1, GATED_SMALL = 2 };
 template <int R>
 void launch_life_gated_r(int vb, int cells, int niv, bool masked, int blocks, const LifeArgs& a, hipStream_t st);
 // Blocks of that kernel a CU holds at once (0: unknown).

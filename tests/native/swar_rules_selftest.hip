@@ -1,5 +1,5 @@
 // The byte-parallel rules of the band walks (csrc/kernels_life.hpp life_rule,
-// gen_rule, gen_rule_tri on cells 0, 1 and 2, life_intervals; csrc/gate.hpp checker_open, open_select), run on the
+// gen_rule, gen_rule_small on cells 0 to 3, life_intervals; csrc/gate.hpp checker_open, open_select), run on the
 // host over every cell value and live count against the rules stated one cell
 // at a time:
 //   Life        (src/rules.jl:385-389)  c == 1 ? survive bit n : birth bit n
@@ -72,10 +72,10 @@ static void check_rules(uint64_t birth, uint64_t survive, int C) {
     }
 }
 
-// gen_rule_tri against the cell rule on every (cell in {0, 1, 2}, live count
-// <= 120) pair, for the state counts it is used with (C <= 3); the live count's
-// source, bit 0 of each byte, against cell == 1.
-static void check_tri(uint64_t birth, uint64_t survive, int C) {
+// gen_rule_small against the cell rule on every (cell in 0..3, live count <= 120)
+// pair, for the state counts it is used with (C <= 4); the live count's source,
+// bit 0 without bit 1 of each byte, against cell == 1.
+static void check_small(uint64_t birth, uint64_t survive, int C) {
     LifeArgs a{};
     if (life_intervals(birth, a.b_lo, a.b_hi) < 0 || life_intervals(survive, a.s_lo, a.s_hi) < 0) {
         ++bad;
@@ -85,7 +85,7 @@ static void check_tri(uint64_t birth, uint64_t survive, int C) {
     a.gen_k = (uint32_t)(129 - C) * 0x01010101u;
     a.gen_dying = (C > 2 ? 2u : 0u) * 0x01010101u;
     std::vector<std::pair<int, int>> cases;
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 4; ++c)
         for (int n = 0; n <= 120; ++n) cases.push_back({c, n});
     for (size_t k = 0; k < cases.size(); ++k) {
         uint32_t cw[1] = {0}, nw[1] = {0}, gen[1];
@@ -94,9 +94,9 @@ static void check_tri(uint64_t birth, uint64_t survive, int C) {
             cw[0] |= (uint32_t)e.first << (8 * b);
             nw[0] |= (uint32_t)e.second << (8 * b);
         }
-        gen_rule_tri<1, LIFE_MAX_IV>(a, nw, cw, gen);
-        if ((cw[0] & 0x01010101u) != bytes_eq(cw[0], 0x01010101u)) {
-            std::printf("cells %08x: bit 0 is not cell == 1\n", cw[0]);
+        gen_rule_small<1, LIFE_MAX_IV>(a, nw, cw, gen);
+        if ((cw[0] & ~(cw[0] >> 1) & 0x01010101u) != bytes_eq(cw[0], 0x01010101u)) {
+            std::printf("cells %08x: bit 0 without bit 1 is not cell == 1\n", cw[0]);
             ++bad;
         }
         for (int b = 0; b < 4; ++b) {
@@ -105,7 +105,7 @@ static void check_tri(uint64_t birth, uint64_t survive, int C) {
             const int got = (int)(int8_t)(uint8_t)(gen[0] >> (8 * b));
             if (got != want) {
                 if (bad < 20)
-                    std::printf("B %llx S %llx C %d: cell %d, count %d: three-state generations %d (want %d)\n",
+                    std::printf("B %llx S %llx C %d: cell %d, count %d: small-cell generations %d (want %d)\n",
                                 (unsigned long long)birth, (unsigned long long)survive, C, e.first, e.second, got,
                                 want);
                 ++bad;
@@ -138,13 +138,14 @@ int main() {
     for (int k = 0; k < 40; ++k) check_rules(two_runs(g), two_runs(g), 2 + (int)(g() % 140));
     // masks with bits up to 63 (counts of 64 and more match nothing)
     check_rules(~0ull << 50, ~0ull, 7);
-    // the rule of lattices holding only 0, 1 and 2: Brian's Brain, C = 2 and 3
-    // with other masks, random masks
-    check_tri(1ull << 2, 0, 3);
-    check_tri(1ull << 3, (1ull << 2) | (1ull << 3), 2);
-    check_tri(0x0F0ull, 0x3Cull << 40, 3);
-    check_tri(~0ull << 50, ~0ull, 3);
-    for (int k = 0; k < 40; ++k) check_tri(two_runs(g), two_runs(g), 2 + (int)(g() % 2));
+    // the rule of lattices holding only cells below 4: Brian's Brain, Star Wars,
+    // C = 2, 3 and 4 with other masks, random masks
+    check_small(1ull << 2, 0, 3);
+    check_small(1ull << 2, (1ull << 3) | (1ull << 4) | (1ull << 5), 4);
+    check_small(1ull << 3, (1ull << 2) | (1ull << 3), 2);
+    check_small(0x0F0ull, 0x3Cull << 40, 3);
+    check_small(~0ull << 50, ~0ull, 4);
+    for (int k = 0; k < 60; ++k) check_small(two_runs(g), two_runs(g), 2 + (int)(g() % 3));
 
     // a third run does not fit
     uint32_t lo[LIFE_MAX_IV], hi[LIFE_MAX_IV];

@@ -54,13 +54,16 @@ LIFE_CASES = [
     ("B45/S345/C5 Moore(6)", (600, 90), (C(1), P), [0] * 40 + [1, 2, 3, 4], M.Checkerboard(), False, None),
     ("B45/S345/C5 Moore(6)", (517, 61), (R, C(2)), [0] * 40 + [1, 2, 3, 4], M.Stochastic(0.5), True, None),
     ("ltl7", (517, 61), (R, R), [0, 0, 0, 0, 1], M.Synchronous(), True, None),
-    # Generations of at most 3 states on lattices of 0, 1 and 2 (the three-state
+    # Generations of at most 4 states on lattices of cells below 4 (the small-cell
     # rule from the second sweep on, or from the first where the set state is
-    # scanned), with gates, masks, Constant(2) edges and odd shapes; a Constant(3)
-    # edge keeps the exact byte tests
+    # scanned), with gates, masks, Constant(2) and Constant(3) edges and odd
+    # shapes; a Constant(4) edge keeps the exact byte tests
     ("brians_brain", (1000, 37), (R, C(2)), [0, 1], M.Checkerboard(), True, None),
     ("brians_brain", (517, 61), (R, P), [0, 1, 2], M.Synchronous(), False, None),
     ("brians_brain", (512, 64), (C(3), P), [0, 1], M.Stochastic(0.5), True, None),
+    ("star_wars", (600, 50), (C(4), R), [0, 1, 2, 3], M.Checkerboard(), False, None),
+    ("star_wars", (1030, 70), (P, C(3)), [0, 1, 2, 3], M.Synchronous(), False, None),
+    ("B3/S23/C4 Moore(2)", (512, 80), (R, C(2)), [0, 1, 2, 3], M.Stochastic(0.5), True, None),
     ("B3/S23/C2 Moore(2)", (768, 90), (C(1), P), [0, 1], M.Stochastic(0.5), True, None),
     ("B2/S/C3 Moore(3)", (1024, 70), (P, P), [0, 1, 2], M.Stochastic(F32(0.4)), True, None),
     # radii 3 and more without a mask take the walk that keeps no mask ring
@@ -152,13 +155,14 @@ def test_cells_outside_the_states(oracle, scheme):
         assert not len(bad), f"{rule.__class__.__name__}: {len(bad)} cells differ, first {bad[0].tolist()}"
 
 
-def test_negative_cells_keep_the_exact_rule(oracle):
-    """Brian's Brain (3 states) moves a negative cell up by one a step, so a
-    sweep over every cell does not leave a lattice of 0, 1 and 2 only: the walk
-    must keep its exact byte tests on such a state for every step."""
+@pytest.mark.parametrize("rule", ["brians_brain", "star_wars"])
+def test_negative_cells_keep_the_exact_rule(oracle, rule):
+    """Generations of 3 or 4 states moves a negative cell up by one a step, so a
+    sweep over every cell does not leave a lattice of cells below 4 only: the
+    walk must keep its exact byte tests on such a state for every step."""
     dims = (1024, 64)
-    u = rng_state(400, dims, [-7, -3, -1, 0, 1, 2, 5])
-    prob = CAProblem(M.Generations("brians_brain"), np.zeros(dims, np.int8), 6, space=M.Lattice(dims, (P, R)))
+    u = rng_state(400, dims, [-7, -3, -1, 0, 1, 2, 3, 5])
+    prob = CAProblem(M.Generations(rule), np.zeros(dims, np.int8), 6, space=M.Lattice(dims, (P, R)))
     got, kernels = advance_raw(prob, M.Synchronous(), u, 0, 6)
     assert set(kernels) == {"k_life_gated"}, kernels
     d, keep = desc_for(prob, M.Synchronous(), field_method=0)
