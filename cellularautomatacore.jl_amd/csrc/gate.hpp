@@ -66,8 +66,13 @@ CAH_DEV void band_open(const BandGate& g, const uint32_t (&M)[VB], uint64_t rowi
 #pragma unroll
         for (int i = 0; i < VB; ++i) open[i] &= pat;
     } else if (g.scheme == CAH_SCHEME_STOCHASTIC) {
-        const uint32_t hp = hi53(sp53);
+        // One 32-bit limit on the draw's first word for both widths: the
+        // Float32 test (r0 >> 8) < sp24 is r0 < 256 sp24 exactly, and the
+        // Float64 one is decided by r0 against hi53 but on a tie.  A word equal
+        // to the limit (a Float64 tie, or 2^32 - 1 under p >= 1 in Float32)
+        // takes the lane's draws again with the exact tests.
         const bool f32 = g.sp_f32 != 0;
+        const uint32_t lim = f32 ? (sp24 >= (1u << 24) ? 0xFFFFFFFFu : sp24 << 8) : hi53(sp53);
         uint32_t bits[VB];
         bool tie = false;
         Unroll<0, VB>::run([&](auto I) {
@@ -77,9 +82,8 @@ CAH_DEV void band_open(const BandGate& g, const uint32_t (&M)[VB], uint64_t rowi
                 constexpr int b = decltype(B)::value;
                 uint32_t r0, r1;
                 cell_philox<SMALL, 4 * i + b>(0u, rowid, t, seed, r0, r1);
-                const bool hit = f32 ? (r0 >> 8) < sp24 : r0 < hp;
-                tie |= !f32 && r0 == hp;
-                bits[i] |= (hit ? 1u : 0u) << (8 * b);
+                tie |= r0 == lim;
+                bits[i] |= (r0 < lim ? 1u : 0u) << (8 * b);
             });
         });
         if (tie) {
@@ -90,7 +94,8 @@ CAH_DEV void band_open(const BandGate& g, const uint32_t (&M)[VB], uint64_t rowi
                     constexpr int b = decltype(B)::value;
                     uint32_t r0, r1;
                     cell_philox<SMALL, 4 * i + b>(0u, rowid, t, seed, r0, r1);
-                    bits[i] |= (draw53(r0, r1) < sp53 ? 1u : 0u) << (8 * b);
+                    const bool hit = f32 ? (r0 >> 8) < sp24 : draw53(r0, r1) < sp53;
+                    bits[i] |= (hit ? 1u : 0u) << (8 * b);
                 });
             });
         }
